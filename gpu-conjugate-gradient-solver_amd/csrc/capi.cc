@@ -1,0 +1,500 @@
+// C ABI over the host setup library and the solve engine
+// (declarations and reference citations in include/gpusolver.h).
+#include "../../include/gpusolver.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "host/setup.h"
+
+struct gs_csr_s {
+  gsx::Csr A;
+};
+struct gs_hier_s {
+  gsx::Hierarchy H;
+};
+struct gs_solver_s {
+  int precision = GS_PREC_DOUBLE;
+  gse::Options opt;
+  bool coarse_kind_set = false;
+  std::unique_ptr<gse::EngineBase> eng;
+  gse::SolveResult last;
+};
+
+namespace {
+thread_local std::string g_error;
+
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    return GS_OK;
+  } catch (const std::exception &e) {
+    g_error = e.what();
+    return GS_ERR;
+  } catch (...) {
+    g_error = "unknown error";
+    return GS_ERR;
+  }
+}
+
+void need(const void *p, const char *what) {
+  if (!p) throw std::runtime_error(std::string(what) + " must not be null.");
+}
+
+gsx::Csr make_csr(int64_t rows, int64_t cols, int64_t nnz, const int32_t *ptr,
+                  const int32_t *col, const double *val) {
+  if (rows <= 0 || cols <= 0)
+    throw std::runtime_error("HostCSRMatrix: matrix dimensions must be nonzero.");
+  if (!ptr || (nnz > 0 && (!col || !val)))
+    throw std::runtime_error("HostCSRMatrix: input pointers must not be null.");
+  gsx::Csr A;
+  A.rows = rows;
+  A.cols = cols;
+  A.ptr.assign(ptr, ptr + rows + 1);
+  A.col.assign(col, col + nnz);
+  A.val.assign(val, val + nnz);
+  A.check();
+  return A;
+}
+
+gsx::Perm perm_from_o2n(int64_t n, const int32_t *o2n) {
+  gsx::Perm p;
+  p.old_to_new.assign(o2n, o2n + n);
+  p.new_to_old.assign(n, -1);
+  for (int64_t i = 0; i < n; ++i) {
+    if (o2n[i] < 0 || o2n[i] >= n)
+      throw std::runtime_error("Permutation: oldToNew contains invalid index.");
+    p.new_to_old[o2n[i]] = static_cast<int32_t>(i);
+  }
+  p.check();
+  return p;
+}
+}  // namespace
+
+extern "C" {
+
+const char *gs_last_error(void) { return g_error.c_str(); }
+int gs_version(void) { return 1; }
+
+int gs_csr_create(int64_t rows, int64_t cols, int64_t nnz, const int32_t *row_ptr,
+                  const int32_t *col_idx, const double *values, gs_csr_h *out) {
+  return guard([&] {
+    need(out, "out");
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = make_csr(rows, cols, nnz, row_ptr, col_idx, values);
+    *out = h.release();
+  });
+}
+
+int gs_csr_info(gs_csr_h A, int64_t *rows, int64_t *cols, int64_t *nnz) {
+  return guard([&] {
+    need(A, "matrix");
+    if (rows) *rows = A->A.rows;
+    if (cols) *cols = A->A.cols;
+    if (nnz) *nnz = A->A.nnz();
+  });
+}
+
+int gs_csr_export(gs_csr_h A, int32_t *row_ptr, int32_t *col_idx, double *values) {
+  return guard([&] {
+    need(A, "matrix");
+    if (row_ptr) std::memcpy(row_ptr, A->A.ptr.data(), A->A.ptr.size() * sizeof(int32_t));
+    if (col_idx) std::memcpy(col_idx, A->A.col.data(), A->A.col.size() * sizeof(int32_t));
+    if (values) std::memcpy(values, A->A.val.data(), A->A.val.size() * sizeof(double));
+  });
+}
+
+int gs_csr_destroy(gs_csr_h A) {
+  delete A;
+  return GS_OK;
+}
+
+int gs_csr_permute_keep_order(gs_csr_h A, const int32_t *o2n, gs_csr_h *out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(o2n, "permutation");
+    need(out, "out");
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = gsx::permute_keep_order(A->A, perm_from_o2n(A->A.rows, o2n));
+    *out = h.release();
+  });
+}
+
+int gs_csr_permute_sorted(gs_csr_h A, const int32_t *ro2n, const int32_t *co2n, gs_csr_h *out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(ro2n, "row permutation");
+    need(co2n, "column permutation");
+    need(out, "out");
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = gsx::permute_sorted(A->A, ro2n, co2n);
+    *out = h.release();
+  });
+}
+
+int gs_ell_pack(gs_csr_h A, int32_t width, int32_t *ell_col, double *ell_val, int32_t *ovf_ptr,
+                int32_t *ovf_col, double *ovf_val, int64_t *overflow_nnz) {
+  return guard([&] {
+    need(A, "matrix");
+    gsx::Ell E = gsx::ell_pack(A->A, width);
+    if (overflow_nnz) *overflow_nnz = E.overflow_nnz;
+    if (!ell_col) return;
+    std::memcpy(ell_col, E.col.data(), E.col.size() * sizeof(int32_t));
+    std::memcpy(ell_val, E.val.data(), E.val.size() * sizeof(double));
+    std::memcpy(ovf_ptr, E.ovf_ptr.data(), E.ovf_ptr.size() * sizeof(int32_t));
+    if (E.overflow_nnz) {
+      std::memcpy(ovf_col, E.ovf_col.data(), E.ovf_col.size() * sizeof(int32_t));
+      std::memcpy(ovf_val, E.ovf_val.data(), E.ovf_val.size() * sizeof(double));
+    }
+  });
+}
+
+int gs_graph_coloring(int64_t n, const int32_t *ptr, const int32_t *col, int32_t *colors,
+                      int32_t *vertices, int32_t *offsets, int32_t *ncolors) {
+  return guard([&] {
+    need(ptr, "row pointer");
+    gsx::Coloring c = gsx::greedy_coloring(n, ptr, col);
+    if (colors) std::memcpy(colors, c.colors.data(), n * sizeof(int32_t));
+    if (vertices) std::memcpy(vertices, c.vertices.data(), n * sizeof(int32_t));
+    if (offsets) std::memcpy(offsets, c.offsets.data(), c.offsets.size() * sizeof(int32_t));
+    if (ncolors) *ncolors = c.ncolors;
+  });
+}
+
+int gs_rcm(int64_t n, const int32_t *ptr, const int32_t *col, int32_t *o2n, int32_t *n2o) {
+  return guard([&] {
+    need(ptr, "row pointer");
+    gsx::Perm p = gsx::rcm(n, ptr, col);
+    std::memcpy(o2n, p.old_to_new.data(), n * sizeof(int32_t));
+    std::memcpy(n2o, p.new_to_old.data(), n * sizeof(int32_t));
+  });
+}
+
+int gs_permutation_check(int64_t n, const int32_t *o2n, const int32_t *n2o) {
+  return guard([&] {
+    if (n == 0) throw std::runtime_error("Permutation: size must be nonzero.");
+    if (!o2n || !n2o) throw std::runtime_error("Permutation: input pointers must not be null.");
+    gsx::Perm p;
+    p.old_to_new.assign(o2n, o2n + n);
+    p.new_to_old.assign(n2o, n2o + n);
+    p.check();
+  });
+}
+
+int gs_block_graph(gs_csr_h A, int32_t bs, int32_t *gptr, int32_t *gcol, int64_t *gnnz) {
+  return guard([&] {
+    need(A, "matrix");
+    gsx::Graph g = gsx::block_graph(A->A, bs);
+    if (gnnz) *gnnz = static_cast<int64_t>(g.col.size());
+    if (gptr) std::memcpy(gptr, g.ptr.data(), g.ptr.size() * sizeof(int32_t));
+    if (gcol) std::memcpy(gcol, g.col.data(), g.col.size() * sizeof(int32_t));
+  });
+}
+
+int gs_expand_block_permutation(int64_t nodes, int32_t bs, const int32_t *node_o2n, int32_t *o2n,
+                                int32_t *n2o) {
+  return guard([&] {
+    gsx::Perm np = perm_from_o2n(nodes, node_o2n);
+    gsx::Perm p = gsx::expand_block_perm(np, nodes, bs);
+    std::memcpy(o2n, p.old_to_new.data(), p.old_to_new.size() * sizeof(int32_t));
+    std::memcpy(n2o, p.new_to_old.data(), p.new_to_old.size() * sizeof(int32_t));
+  });
+}
+
+int gs_inverse_diag_blocks3(gs_csr_h A, double *inv) {
+  return guard([&] {
+    need(A, "matrix");
+    std::vector<double> v = gsx::inverse_diag_blocks3(A->A);
+    std::memcpy(inv, v.data(), v.size() * sizeof(double));
+  });
+}
+
+int gs_problem_poisson2d(int32_t nx, int32_t ny, gs_csr_h *out) {
+  return guard([&] {
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = gsx::poisson2d(nx, ny);
+    *out = h.release();
+  });
+}
+
+int gs_problem_poisson3d(int32_t nx, int32_t ny, int32_t nz, gs_csr_h *out) {
+  return guard([&] {
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = gsx::poisson3d(nx, ny, nz);
+    *out = h.release();
+  });
+}
+
+int gs_problem_varcoef3d(int32_t n, uint64_t seed, double sigma, double scale, gs_csr_h *out) {
+  return guard([&] {
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = gsx::varcoef3d(n, seed, sigma, scale, nullptr);
+    *out = h.release();
+  });
+}
+
+int gs_problem_elasticity(int32_t nv, double E, double nu, double hs, double hscale, uint64_t seed,
+                          gs_csr_h *out, double *rhs, double *nullspace) {
+  return guard([&] {
+    gsx::ElasticityProblem P = gsx::elasticity(nv, E, nu, hs, hscale, seed);
+    if (rhs) std::memcpy(rhs, P.rhs.data(), P.rhs.size() * sizeof(double));
+    if (nullspace) std::memcpy(nullspace, P.nullspace.data(), P.nullspace.size() * sizeof(double));
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = std::move(P.A);
+    *out = h.release();
+  });
+}
+
+int gs_uniform_pm1(uint64_t seed, int64_t n, double *out) {
+  return guard([&] { gsx::uniform_pm1(seed, n, out); });
+}
+
+int gs_gaussian_field(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double sigma,
+                      double *out) {
+  return guard([&] {
+    std::vector<double> g = gsx::gaussian_random_field(nx, ny, nz, seed, sigma);
+    std::memcpy(out, g.data(), g.size() * sizeof(double));
+  });
+}
+
+int gs_hierarchy_build_sa(gs_csr_h A, int32_t bs, double eps, int32_t max_levels, int64_t min_dofs,
+                          const double *ns, int32_t ncols, gs_hier_h *out) {
+  return guard([&] {
+    need(A, "matrix");
+    gsx::SAParams prm;
+    prm.block_size = bs;
+    prm.eps_strong = eps;
+    prm.max_levels = max_levels;
+    prm.min_dofs = min_dofs;
+    std::vector<double> B;
+    if (ncols > 0) {
+      need(ns, "nullspace");
+      B.assign(ns, ns + A->A.rows * ncols);
+    }
+    auto h = std::make_unique<gs_hier_s>();
+    h->H = gsx::build_sa_hierarchy(A->A, prm, B, ncols);
+    *out = h.release();
+  });
+}
+
+int gs_hierarchy_create(gs_hier_h *out) {
+  return guard([&] { *out = new gs_hier_s(); });
+}
+
+int gs_hierarchy_push_level(gs_hier_h h, gs_csr_h A) {
+  return guard([&] {
+    need(h, "hierarchy");
+    need(A, "matrix");
+    h->H.A.push_back(A->A);
+  });
+}
+
+int gs_hierarchy_push_transfer(gs_hier_h h, gs_csr_h P, gs_csr_h R) {
+  return guard([&] {
+    need(h, "hierarchy");
+    need(P, "prolongation");
+    need(R, "restriction");
+    h->H.P.push_back(P->A);
+    h->H.R.push_back(R->A);
+  });
+}
+
+int gs_hierarchy_levels(gs_hier_h h, int32_t *nlev) {
+  return guard([&] {
+    need(h, "hierarchy");
+    *nlev = static_cast<int32_t>(h->H.A.size());
+  });
+}
+
+int gs_hierarchy_get(gs_hier_h h, int32_t which, int32_t k, gs_csr_h *out) {
+  return guard([&] {
+    need(h, "hierarchy");
+    const std::vector<gsx::Csr> &v = which == 0 ? h->H.A : which == 1 ? h->H.P : h->H.R;
+    if (k < 0 || k >= static_cast<int32_t>(v.size()))
+      throw std::runtime_error("hierarchy index out of range.");
+    auto c = std::make_unique<gs_csr_s>();
+    c->A = v[k];
+    *out = c.release();
+  });
+}
+
+int gs_hierarchy_aggregates(gs_hier_h h, int32_t k, int32_t *agg) {
+  return guard([&] {
+    need(h, "hierarchy");
+    if (k < 0 || k >= static_cast<int32_t>(h->H.aggregates.size()))
+      throw std::runtime_error("hierarchy index out of range.");
+    std::memcpy(agg, h->H.aggregates[k].data(), h->H.aggregates[k].size() * sizeof(int32_t));
+  });
+}
+
+int gs_hierarchy_destroy(gs_hier_h h) {
+  delete h;
+  return GS_OK;
+}
+
+int gs_solver_create(int32_t precision, gs_solver_h *out) {
+  return guard([&] {
+    if (precision < GS_PREC_DOUBLE || precision > GS_PREC_SINGLE)
+      throw std::runtime_error("unknown precision mode.");
+    auto s = std::make_unique<gs_solver_s>();
+    s->precision = precision;
+    *out = s.release();
+  });
+}
+
+int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
+  return guard([&] {
+    need(s, "solver");
+    need(key, "key");
+    const std::string k(key);
+    gse::Options &o = s->opt;
+    if (k == "preconditioner") o.precond = static_cast<int>(v);
+    else if (k == "smoother") {
+      o.fine_kind = static_cast<int>(v);
+      if (!s->coarse_kind_set) o.coarse_kind = o.fine_kind;
+    } else if (k == "coarse_smoother") {
+      o.coarse_kind = static_cast<int>(v);
+      s->coarse_kind_set = true;
+    } else if (k == "pre") o.pre = static_cast<int>(v);
+    else if (k == "post") o.post = static_cast<int>(v);
+    else if (k == "coarse_steps") o.coarse_steps = static_cast<int>(v);
+    else if (k == "vcycles") o.vcycles = static_cast<int>(v);
+    else if (k == "coarse_direct") o.coarse_direct = v != 0;
+    else if (k == "graph_iters") o.graph_iters = static_cast<int>(v);
+    else if (k == "cg_block3") o.cg_block3 = v != 0;
+    else throw std::runtime_error("unknown integer option: " + k);
+    if (o.pre < 0 || o.post < 0 || o.coarse_steps < 0 || o.vcycles < 0 || o.graph_iters < 0)
+      throw std::runtime_error("option values must be non-negative.");
+    if (o.fine_kind < 0 || o.fine_kind > 3 || o.coarse_kind < 0 || o.coarse_kind > 3)
+      throw std::runtime_error("MultigridSmoother: unsupported smoother type.");
+  });
+}
+
+int gs_solver_set_real(gs_solver_h s, const char *key, double v) {
+  return guard([&] {
+    need(s, "solver");
+    need(key, "key");
+    if (std::string(key) == "omega") s->opt.omega = v;
+    else throw std::runtime_error(std::string("unknown real option: ") + key);
+  });
+}
+
+int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega,
+                                const int32_t *sweeps) {
+  return guard([&] {
+    need(s, "solver");
+    s->opt.level_omega.assign(nlev, 0.0);
+    s->opt.level_sweeps.assign(nlev, 0);
+    for (int32_t k = 0; k < nlev; ++k) {
+      if (omega) s->opt.level_omega[k] = omega[k];
+      if (sweeps) s->opt.level_sweeps[k] = sweeps[k];
+    }
+  });
+}
+
+int gs_solver_setup(gs_solver_h s, gs_hier_h h, const int32_t *const *o2n) {
+  return guard([&] {
+    need(s, "solver");
+    need(h, "hierarchy");
+    std::vector<gsx::Perm> perms;
+    for (size_t k = 0; k < h->H.A.size(); ++k) {
+      const int64_t n = h->H.A[k].rows;
+      if (o2n && o2n[k]) perms.push_back(perm_from_o2n(n, o2n[k]));
+      else perms.push_back(gsx::Perm::identity(n));
+    }
+    s->eng = gse::make_engine(s->precision, s->opt);
+    s->eng->setup(h->H, perms);
+  });
+}
+
+int gs_solver_solve(gs_solver_h s, const double *b, double *x, double tol, int64_t maxit,
+                    int64_t *iters, double *relres2, double *true_relres, double *solve_ms) {
+  return guard([&] {
+    need(s, "solver");
+    if (!s->eng) throw std::runtime_error("CGSolver: solver is not set up.");
+    need(b, "b");
+    need(x, "x");
+    if (maxit < 0) throw std::runtime_error("maxIterations must be non-negative.");
+    s->last = s->eng->solve(b, x, tol, maxit);
+    if (iters) *iters = s->last.iterations;
+    if (relres2) *relres2 = s->last.relres2;
+    if (true_relres) *true_relres = s->last.true_relres;
+    if (solve_ms) *solve_ms = s->last.solve_ms;
+  });
+}
+
+int gs_solver_result(gs_solver_h s, const char *key, double *value) {
+  return guard([&] {
+    need(s, "solver");
+    const std::string k(key);
+    const gse::SolveResult &r = s->last;
+    if (k == "converged") *value = r.converged ? 1.0 : 0.0;
+    else if (k == "loop_iterations") *value = static_cast<double>(r.loop_iterations);
+    else if (k == "breakdown") *value = r.breakdown;
+    else if (k == "wall_ms") *value = r.wall_ms;
+    else if (k == "zero_rhs") *value = r.zero_rhs ? 1.0 : 0.0;
+    else throw std::runtime_error("unknown result key: " + k);
+  });
+}
+
+int gs_solver_history(gs_solver_h s, double *out, int64_t cap, int64_t *len) {
+  return guard([&] {
+    need(s, "solver");
+    if (!s->eng) throw std::runtime_error("solver is not set up.");
+    const auto &h = s->eng->history();
+    const int64_t n = std::min<int64_t>(cap, static_cast<int64_t>(h.size()));
+    if (out) std::memcpy(out, h.data(), n * sizeof(double));
+    if (len) *len = static_cast<int64_t>(h.size());
+  });
+}
+
+int gs_solver_vcycle(gs_solver_h s, const double *r, double *z) {
+  return guard([&] {
+    need(s, "solver");
+    if (!s->eng) throw std::runtime_error("solver is not set up.");
+    s->eng->vcycle_level_order(r, z);
+  });
+}
+
+int gs_solver_spmv(gs_solver_h s, const double *x, double *y) {
+  return guard([&] {
+    need(s, "solver");
+    if (!s->eng) throw std::runtime_error("solver is not set up.");
+    s->eng->spmv_level_order(x, y);
+  });
+}
+
+int gs_solver_time_kernel(gs_solver_h s, int32_t which, int32_t reps, double *ms, double *bytes) {
+  return guard([&] {
+    need(s, "solver");
+    if (!s->eng) throw std::runtime_error("solver is not set up.");
+    if (reps <= 0) throw std::runtime_error("reps must be positive.");
+    double b = 0.0;
+    *ms = s->eng->time_kernel(which, reps, &b);
+    if (bytes) *bytes = b;
+  });
+}
+
+int gs_solver_describe(gs_solver_h s, char *buf, int64_t cap) {
+  return guard([&] {
+    need(s, "solver");
+    if (!s->eng) throw std::runtime_error("solver is not set up.");
+    const std::string d = s->eng->describe();
+    if (cap > 0) {
+      const size_t m = std::min<size_t>(cap - 1, d.size());
+      std::memcpy(buf, d.data(), m);
+      buf[m] = '\0';
+    }
+  });
+}
+
+int gs_solver_destroy(gs_solver_h s) {
+  delete s;
+  return GS_OK;
+}
+
+}  // extern "C"

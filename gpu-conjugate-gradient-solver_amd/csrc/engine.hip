@@ -1,0 +1,729 @@
+// AMG-PCG solve engine: the reference's CGSolver::solve
+// (src/CGSolver.cc:87-306) driving MultigridSmoother::vCycle
+// (src/Smoothers/MultigridSmoother.cc:337-420), rebuilt for gfx950.
+//
+// Setup (host, once): every level matrix is permuted as the reference does
+// (sorted flavour for V-cycle operators, order-keeping flavour for the CG
+// operator), coloured with the reference's greedy colouring, and then laid
+// out on the device in colour-permuted ("internal") order, so each colour
+// of a Gauss-Seidel sweep is a contiguous range of rows and every launch
+// streams unit-stride SELL-64 / BSR3-SELL-64 data. The relabelling keeps
+// each row's entry order, so every row sum is the reference's.
+//
+// Solve (device): the PCG loop runs as fused kernels (SpMV + p.Ap with
+// alpha, x/r update + r.r with the stop test, V-cycle, r.z with beta,
+// p update), each iteration recorded into a hipGraph that is replayed
+// `graph_iters` times between host checks of the stop word; iterations
+// after convergence are no-ops. Precision is a template pair: CG type TC
+// and V-cycle type TM (double/double plain, double/float mixed,
+// float/float as-reference).
+#include "engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+namespace gse {
+
+using gsd::DBsr3;
+using gsd::DevBuf;
+using gsd::DOp;
+using gsd::DSell;
+using gsk::grid_for;
+using gsk::kBlock;
+
+namespace {
+
+// Keep-order relabelling: row i of the result is row rq[i] of A, column c
+// becomes cinv[c]. Entry order inside each row is unchanged.
+gsx::Csr relabel(const gsx::Csr &A, const std::vector<int32_t> &rq,
+                 const std::vector<int32_t> &cinv) {
+  gsx::Csr B;
+  B.rows = A.rows;
+  B.cols = A.cols;
+  B.ptr.assign(A.rows + 1, 0);
+  for (int64_t i = 0; i < A.rows; ++i) B.ptr[i + 1] = B.ptr[i] + (A.ptr[rq[i] + 1] - A.ptr[rq[i]]);
+  B.col.resize(A.nnz());
+  B.val.resize(A.nnz());
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < A.rows; ++i) {
+    int64_t d = B.ptr[i];
+    for (int32_t k = A.ptr[rq[i]]; k < A.ptr[rq[i] + 1]; ++k, ++d) {
+      B.col[d] = cinv[A.col[k]];
+      B.val[d] = A.val[k];
+    }
+  }
+  return B;
+}
+
+std::vector<int32_t> inverse_of(const std::vector<int32_t> &q) {
+  std::vector<int32_t> inv(q.size());
+  for (size_t i = 0; i < q.size(); ++i) inv[q[i]] = static_cast<int32_t>(i);
+  return inv;
+}
+
+std::vector<int32_t> iota_vec(int64_t n) {
+  std::vector<int32_t> v(n);
+  std::iota(v.begin(), v.end(), 0);
+  return v;
+}
+
+template <class T>
+void launch_fill(T *p, T v, int64_t n, hipStream_t s, const int32_t *stop) {
+  if (n <= 0) return;
+  gsk::k_fill<T><<<grid_for(n), kBlock, 0, s>>>(p, v, n, stop);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Level and engine types.
+// ---------------------------------------------------------------------------
+template <class TM>
+struct Level {
+  int64_t n = 0;
+  int kind = kSGS;
+  bool direct = false;  // coarsest level solved with the dense inverse
+  TM omega = TM(1);
+  int pre = 1, post = 1;
+  DOp<TM> A;
+  std::vector<int64_t> coff;  // colour offsets (rows, or blocks for block GS)
+  int ncolors = 0;
+  DevBuf diag, dinv, inv;
+  DevBuf rhs_buf, x_buf, res, tmp;
+  TM *rhs = nullptr, *x = nullptr;  // bound to own buffers or the CG vectors
+  DSell<TM> R, P;                    // transfer to level k + 1
+  std::vector<int32_t> q;            // internal index -> level index
+};
+
+template <class TC, class TM>
+class Engine final : public EngineBase {
+ public:
+  explicit Engine(const Options &o) : opt_(o) {
+    GS_HIP(hipStreamCreate(&stream_));
+  }
+  ~Engine() override {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (host_ctl_) (void)hipHostFree(host_ctl_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    (void)hipStreamDestroy(stream_);
+  }
+
+  void setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm> &perms) override;
+  SolveResult solve(const double *b, double *x, double tol, int64_t maxit) override;
+  void vcycle_level_order(const double *r, double *z) override;
+  void spmv_level_order(const double *x, double *y) override;
+  double time_kernel(int which, int reps, double *bytes) override;
+  std::string describe() const override;
+  int64_t size() const override { return n_; }
+  const std::vector<double> &history() const override { return hist_host_; }
+  void level_info(std::vector<LevelInfo> &out) const override;
+
+ private:
+  void build_level(int k, const gsx::Csr &Ak_level, bool coarsest);
+  void encode_vcycle(int k, const int32_t *stop, bool zero_x);
+  void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop);
+  void encode_iteration(const int32_t *stop);
+  void encode_residual(Level<TM> &lv, const int32_t *stop);
+  void capture_graph();
+  void apply_preconditioner(const int32_t *stop);
+
+  Options opt_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  int64_t n_ = 0;
+  int nlev_ = 0;
+  std::vector<Level<TM>> L_;
+  DOp<TC> Acg_;
+  std::vector<int32_t> int2orig_;  // internal index -> user (original) index
+  std::vector<int32_t> int2lvl0_;  // internal index -> level-0 (permuted) index
+  // CG vectors
+  DevBuf b_, x_, r_, p_, q_, z_, rm_, partials_, ctl_, hist_, tr_;
+  TM *rvc_ = nullptr;  // V-cycle view of r (r itself or its rounded copy)
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  gsk::Ctl<TC> *host_ctl_ = nullptr;
+  int64_t hist_cap_ = 0;
+  std::vector<double> hist_host_;
+  bool setup_done_ = false;
+};
+
+template <class TC, class TM>
+void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
+  Level<TM> &lv = L_[k];
+  lv.n = Ak.rows;
+  lv.kind = (k == 0) ? opt_.fine_kind : opt_.coarse_kind;
+  lv.direct = coarsest && opt_.coarse_direct;
+  double om = opt_.omega;
+  int pre = opt_.pre, post = opt_.post;
+  if (k < static_cast<int>(opt_.level_omega.size()) && opt_.level_omega[k] > 0.0)
+    om = opt_.level_omega[k];
+  if (k < static_cast<int>(opt_.level_sweeps.size()) && opt_.level_sweeps[k] > 0)
+    pre = post = opt_.level_sweeps[k];
+  lv.omega = static_cast<TM>(om);
+  lv.pre = pre;
+  lv.post = post;
+  const bool blocky = lv.kind == kBlockGS || lv.kind == kBlockJacobi;
+  if (!lv.direct) {
+    if (lv.kind == kSGS) {
+      if (!std::isfinite(om) || om <= 0.0 || om >= 2.0)
+        throw std::runtime_error("SymmetricGaussSeidelSmoother: invalid omega.");
+      const int steps_needed = coarsest ? opt_.coarse_steps : std::min(pre, post);
+      if (steps_needed == 0)
+        throw std::runtime_error("Preconditioner requires at least one smoothing step.");
+    }
+    if (lv.kind == kBlockGS && (om <= 0.0 || om >= 2.0))
+      throw std::runtime_error("BlockGaussSeidelSmoother: omega must be in (0, 2).");
+    if (blocky && lv.n % 3 != 0)
+      throw std::runtime_error("BlockGaussSeidelSmoother: number of DOFs must be divisible by 3.");
+  }
+  // Internal ordering.
+  lv.coff.clear();
+  if (!lv.direct && lv.kind == kSGS) {
+    gsx::Coloring c = gsx::greedy_coloring(Ak.rows, Ak.ptr.data(), Ak.col.data());
+    lv.q = c.vertices;
+    lv.coff.assign(c.offsets.begin(), c.offsets.end());
+    lv.ncolors = c.ncolors;
+  } else if (!lv.direct && lv.kind == kBlockGS) {
+    gsx::Graph g = gsx::block_graph(Ak, 3);
+    gsx::Coloring c = gsx::greedy_coloring(Ak.rows / 3, g.ptr.data(), g.col.data());
+    lv.q.resize(Ak.rows);
+    for (int64_t i = 0; i < Ak.rows / 3; ++i)
+      for (int l = 0; l < 3; ++l) lv.q[3 * i + l] = 3 * c.vertices[i] + l;
+    lv.coff.assign(c.offsets.begin(), c.offsets.end());
+    lv.ncolors = c.ncolors;
+  } else {
+    lv.q = iota_vec(Ak.rows);
+    lv.ncolors = 0;
+  }
+  const std::vector<int32_t> qinv = inverse_of(lv.q);
+  gsx::Csr Ai = relabel(Ak, lv.q, qinv);
+  if (lv.direct) {
+    std::vector<double> dense(static_cast<size_t>(lv.n) * lv.n, 0.0);
+    for (int64_t r = 0; r < lv.n; ++r)
+      for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e) dense[r * lv.n + Ai.col[e]] += Ai.val[e];
+    gsx::spd_inverse(lv.n, dense);
+    lv.inv.upload(dense);
+  }
+  lv.A.build(Ai, blocky);
+  if (!lv.direct) {
+    if (lv.kind == kSGS || lv.kind == kDampedJacobi) {
+      std::vector<TM> d(lv.n);
+      std::vector<TM> di(lv.n);
+      for (int64_t r = 0; r < lv.n; ++r) {
+        double dv = 0.0;
+        bool found = false;
+        for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e)
+          if (Ai.col[e] == r) { dv = Ai.val[e]; found = true; }
+        if (!found) {
+          if (lv.kind == kSGS)
+            throw std::runtime_error("CPUBackend::encodeGaussSeidelColor: zero diagonal entry.");
+          throw std::runtime_error("DampedJacobiPreconditioner::initialize: missing diagonal entry.");
+        }
+        if (std::abs(static_cast<TM>(dv)) < TM(1e-12)) {
+          if (lv.kind == kSGS)
+            throw std::runtime_error("CPUBackend::encodeGaussSeidelColor: zero diagonal entry.");
+          throw std::runtime_error("DampedJacobiPreconditioner::initialize: zero diagonal entry.");
+        }
+        d[r] = static_cast<TM>(dv);
+        di[r] = TM(1) / static_cast<TM>(dv);
+      }
+      if (lv.kind == kSGS) lv.diag.upload(d);
+      else lv.dinv.upload(di);
+    } else {
+      lv.dinv.upload(gsd::cast_values<TM>(gsx::inverse_diag_blocks3(Ai)));
+    }
+  }
+  if (k > 0) {
+    lv.rhs_buf.alloc(lv.n * sizeof(TM));
+    lv.x_buf.alloc(lv.n * sizeof(TM));
+    lv.rhs = lv.rhs_buf.template as<TM>();
+    lv.x = lv.x_buf.template as<TM>();
+  }
+  lv.res.alloc(lv.n * sizeof(TM));
+  lv.tmp.alloc(lv.n * sizeof(TM));
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm> &perms) {
+  nlev_ = static_cast<int>(H.A.size());
+  if (nlev_ == 0) throw std::runtime_error("MultigridSmoother: empty hierarchy.");
+  if (static_cast<int>(perms.size()) != nlev_)
+    throw std::runtime_error("MultigridSmoother: invalid number of permutations.");
+  if (static_cast<int>(H.P.size()) + 1 != nlev_ || static_cast<int>(H.R.size()) + 1 != nlev_)
+    throw std::runtime_error("MultigridSmoother: invalid number of transfers.");
+  for (int k = 0; k < nlev_; ++k) {
+    H.A[k].check();
+    if (H.A[k].rows != H.A[k].cols) throw std::runtime_error("CGSolver: matrix must be square.");
+    if (static_cast<int64_t>(perms[k].old_to_new.size()) != H.A[k].rows)
+      throw std::runtime_error("MultigridSmoother: permutation size does not match level size.");
+    perms[k].check();
+  }
+  if (opt_.precond == kPrecIdentity && !std::is_same<TC, TM>::value)
+    throw std::runtime_error("identity preconditioner requires one precision for CG and V-cycle.");
+  n_ = H.A[0].rows;
+  L_.clear();
+  L_.resize(nlev_);
+  const int used_levels = opt_.precond == kPrecMultigrid ? nlev_ : 1;
+  for (int k = 0; k < used_levels; ++k) {
+    const gsx::Csr Ak = gsx::permute_sorted(H.A[k], perms[k].old_to_new.data(),
+                                            perms[k].old_to_new.data());
+    if (opt_.precond == kPrecMultigrid) build_level(k, Ak, k == nlev_ - 1);
+    else {
+      L_[0].n = Ak.rows;
+      L_[0].q = iota_vec(Ak.rows);
+    }
+  }
+  if (opt_.precond == kPrecMultigrid) {
+    for (int k = 0; k + 1 < nlev_; ++k) {
+      const auto &o2n_f = perms[k].old_to_new;
+      const auto &o2n_c = perms[k + 1].old_to_new;
+      gsx::Csr Pk = gsx::permute_sorted(H.P[k], o2n_f.data(), o2n_c.data());
+      gsx::Csr Rk = gsx::permute_sorted(H.R[k], o2n_c.data(), o2n_f.data());
+      if (Pk.rows != L_[k].n || Pk.cols != L_[k + 1].n || Rk.rows != L_[k + 1].n ||
+          Rk.cols != L_[k].n)
+        throw std::runtime_error("MultigridSmoother: transfer operator shape mismatch.");
+      const std::vector<int32_t> qf_inv = inverse_of(L_[k].q);
+      const std::vector<int32_t> qc_inv = inverse_of(L_[k + 1].q);
+      L_[k].P.build(relabel(Pk, L_[k].q, qc_inv));
+      L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv));
+    }
+  }
+  // CG operator: level-0 input with the order-keeping permutation, then the
+  // internal relabelling of level 0.
+  {
+    gsx::Csr Acg = gsx::permute_keep_order(H.A[0], perms[0]);
+    const std::vector<int32_t> q0inv = inverse_of(L_[0].q);
+    gsx::Csr Ai = relabel(Acg, L_[0].q, q0inv);
+    const bool blocky = opt_.fine_kind == kBlockGS || opt_.fine_kind == kBlockJacobi ||
+                        opt_.cg_block3;
+    Acg_.build(Ai, blocky && n_ % 3 == 0);
+  }
+  int2lvl0_ = L_[0].q;
+  int2orig_.resize(n_);
+  for (int64_t i = 0; i < n_; ++i) int2orig_[i] = perms[0].new_to_old[L_[0].q[i]];
+  // CG vectors and control block.
+  b_.alloc(n_ * sizeof(TC));
+  x_.alloc(n_ * sizeof(TC));
+  r_.alloc(n_ * sizeof(TC));
+  p_.alloc(n_ * sizeof(TC));
+  q_.alloc(n_ * sizeof(TC));
+  z_.alloc(n_ * sizeof(TM));
+  if (!std::is_same<TC, TM>::value) {
+    rm_.alloc(n_ * sizeof(TM));
+    rvc_ = rm_.template as<TM>();
+  } else {
+    rvc_ = reinterpret_cast<TM *>(r_.template as<TC>());
+  }
+  L_[0].rhs = rvc_;
+  L_[0].x = z_.template as<TM>();
+  partials_.alloc((3 * static_cast<size_t>(grid_for(n_)) + 64) * sizeof(double));
+  ctl_.alloc(sizeof(gsk::Ctl<TC>));
+  tr_.alloc(2 * sizeof(double));
+  if (!host_ctl_) GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_ctl_), sizeof(gsk::Ctl<TC>)));
+  std::memset(host_ctl_, 0, sizeof(gsk::Ctl<TC>));
+  if (!ev0_) GS_HIP(hipEventCreate(&ev0_));
+  if (!ev1_) GS_HIP(hipEventCreate(&ev1_));
+  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+  if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+  setup_done_ = true;
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle encoding.
+// ---------------------------------------------------------------------------
+template <class TC, class TM>
+void Engine<TC, TM>::encode_residual(Level<TM> &lv, const int32_t *stop) {
+  const int64_t n = lv.n;
+  if (lv.A.b3)
+    gsk::k_residual_b3<TM><<<grid_for(n / 3), kBlock, 0, stream_>>>(lv.A.b.view(), lv.rhs, lv.x,
+                                                                    lv.res.template as<TM>(), stop);
+  else
+    gsk::k_residual<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.A.s.view(), lv.rhs, lv.x,
+                                                             lv.res.template as<TM>(), stop);
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop) {
+  const int64_t n = lv.n;
+  bool x_ready = !zero_x;
+  if (lv.kind == kSGS) {
+    for (int it = 0; it < iters; ++it) {
+      for (int c = 0; c < lv.ncolors; ++c) {
+        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
+        if (!x_ready && c == 0) {
+          gsk::k_gs_color_zero<TM><<<grid_for(n), kBlock, 0, stream_>>>(
+              lv.diag.template as<TM>(), e, n, lv.rhs, lv.x, lv.omega, stop);
+          x_ready = true;
+          continue;
+        }
+        if (e > b)
+          gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
+              lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, stop);
+      }
+      for (int c = lv.ncolors - 1; c >= 0; --c) {
+        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
+        if (e > b)
+          gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
+              lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, stop);
+      }
+    }
+  } else if (lv.kind == kBlockGS) {
+    const int64_t nb = n / 3;
+    for (int it = 0; it < iters; ++it) {
+      for (int c = 0; c < lv.ncolors; ++c) {
+        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
+        if (!x_ready && c == 0) {
+          gsk::k_block_gs_color_zero<TM><<<grid_for(nb), kBlock, 0, stream_>>>(
+              lv.dinv.template as<TM>(), e, nb, lv.rhs, lv.x, lv.omega, stop);
+          x_ready = true;
+          continue;
+        }
+        if (e <= b) continue;
+        if (lv.A.b3)
+          gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
+              lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, stop);
+        else
+          throw std::runtime_error("block Gauss-Seidel level is not 3x3-block representable.");
+      }
+      for (int c = lv.ncolors - 1; c >= 0; --c) {
+        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
+        if (e <= b) continue;
+        gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
+            lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, stop);
+      }
+    }
+  } else {
+    if (!x_ready && iters > 0) {
+      launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
+      x_ready = true;
+    }
+    for (int it = 0; it < iters; ++it) {
+      if (lv.kind == kDampedJacobi) {
+        gsk::k_jacobi<TM><<<grid_for(n), kBlock, 0, stream_>>>(
+            lv.A.s.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
+      } else if (lv.A.b3) {
+        gsk::k_block_jacobi_b3<TM><<<grid_for(n / 3), kBlock, 0, stream_>>>(
+            lv.A.b.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
+      } else {
+        gsk::k_block_jacobi<TM><<<grid_for(n / 3), kBlock, 0, stream_>>>(
+            lv.A.s.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
+      }
+      gsk::k_copy<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.tmp.template as<TM>(), lv.x, n, stop);
+    }
+  }
+  if (!x_ready) launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
+  Level<TM> &lv = L_[k];
+  if (k == nlev_ - 1) {
+    if (lv.direct) {
+      gsk::k_dense_solve<TM><<<static_cast<uint32_t>((lv.n + 3) / 4), kBlock, 0, stream_>>>(
+          lv.inv.template as<double>(), lv.n, lv.rhs, lv.x, stop);
+    } else {
+      encode_smooth(lv, opt_.coarse_steps, zero_x, stop);
+    }
+    return;
+  }
+  encode_smooth(lv, lv.pre, zero_x, stop);
+  encode_residual(lv, stop);
+  Level<TM> &cl = L_[k + 1];
+  // The coarse correction starts at zero; the coarse level's first kernel
+  // zeroes it itself when it is a zero-start Gauss-Seidel colour.
+  const bool coarse_self_zero =
+      !cl.direct && ((k + 1 == nlev_ - 1) ? opt_.coarse_steps > 0 : cl.pre > 0) &&
+      (cl.kind == kSGS || cl.kind == kBlockGS);
+  gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(
+      lv.R.view(), lv.res.template as<TM>(), cl.rhs, coarse_self_zero ? nullptr : cl.x, stop);
+  encode_vcycle(k + 1, stop, true);
+  gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.view(), cl.x, lv.x, stop);
+  encode_smooth(lv, lv.post, false, stop);
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::apply_preconditioner(const int32_t *stop) {
+  if (opt_.precond == kPrecIdentity) {
+    gsk::k_copy<TM><<<grid_for(n_), kBlock, 0, stream_>>>(rvc_, z_.template as<TM>(), n_, stop);
+    return;
+  }
+  for (int v = 0; v < opt_.vcycles; ++v) encode_vcycle(0, stop, v == 0);
+  if (opt_.vcycles == 0) launch_fill<TM>(z_.template as<TM>(), TM(0), n_, stream_, stop);
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
+  auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
+  TC *parts = partials_.template as<TC>();
+  const uint32_t g = grid_for(Acg_.b3 ? n_ / 3 : n_);
+  if (Acg_.b3)
+    gsk::k_cg_spmv_pap_b3<TC><<<g, kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>(),
+                                                          ctl, parts);
+  else
+    gsk::k_cg_spmv_pap<TC><<<g, kBlock, 0, stream_>>>(Acg_.s.view(), p_.template as<TC>(), q_.template as<TC>(),
+                                                       ctl, parts);
+  TM *rm = std::is_same<TC, TM>::value ? nullptr : rm_.template as<TM>();
+  gsk::k_cg_update_xr<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(
+      x_.template as<TC>(), r_.template as<TC>(), rm, p_.template as<TC>(), q_.template as<TC>(), n_, ctl, parts);
+  apply_preconditioner(stop);
+  gsk::k_cg_rz<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(r_.template as<TC>(), z_.template as<TM>(), n_, ctl,
+                                                             parts);
+  gsk::k_cg_update_p<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(p_.template as<TC>(), z_.template as<TM>(), n_,
+                                                                   ctl);
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::capture_graph() {
+  if (graph_exec_) return;
+  auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
+  const int32_t *stop = &ctl->stop;
+  GS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < opt_.graph_iters; ++i) encode_iteration(stop);
+  GS_HIP(hipStreamEndCapture(stream_, &graph_));
+  GS_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+}
+
+template <class TC, class TM>
+SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_t maxit) {
+  if (!setup_done_) throw std::runtime_error("CGSolver: solver is not set up.");
+  SolveResult res;
+  // src/CGSolver.cc:113-118: a right-hand side below 1e-12 in norm returns
+  // x = 0 without iterating.
+  double nb = 0.0;
+  for (int64_t i = 0; i < n_; ++i) nb += b[i] * b[i];
+  if (std::sqrt(nb) < 1e-12) {
+    for (int64_t i = 0; i < n_; ++i) x[i] = 0.0;
+    res.iterations = 0;
+    res.zero_rhs = true;
+    return res;
+  }
+  std::vector<TC> bi(n_), xi(n_);
+  for (int64_t i = 0; i < n_; ++i) {
+    bi[i] = static_cast<TC>(b[int2orig_[i]]);
+    xi[i] = static_cast<TC>(x[int2orig_[i]]);
+  }
+  GS_HIP(hipMemcpyAsync(b_.get(), bi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
+  GS_HIP(hipMemcpyAsync(x_.get(), xi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
+  if (hist_cap_ < maxit + 1) {
+    hist_.alloc((maxit + 1) * sizeof(TC));
+    hist_cap_ = maxit + 1;
+  }
+  gsk::Ctl<TC> c0{};
+  c0.tol2 = static_cast<TC>(tol) * static_cast<TC>(tol);
+  c0.maxit = maxit;
+  c0.hist = hist_.template as<TC>();
+  GS_HIP(hipMemcpyAsync(ctl_.get(), &c0, sizeof(c0), hipMemcpyHostToDevice, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
+
+  GS_HIP(hipEventRecord(ev0_, stream_));
+  auto t0 = std::chrono::steady_clock::now();
+  auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
+  TM *rm = std::is_same<TC, TM>::value ? nullptr : rm_.template as<TM>();
+  if (Acg_.b3)
+    gsk::k_cg_init_r_b3<TC, TM><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(
+        Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm);
+  else
+    gsk::k_cg_init_r<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(
+        Acg_.s.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm);
+  apply_preconditioner(nullptr);
+  gsk::k_cg_init_dots<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(
+      b_.template as<TC>(), r_.template as<TC>(), z_.template as<TM>(), p_.template as<TC>(), n_, ctl, partials_.template as<TC>());
+  const int32_t *stop = &ctl->stop;
+  int64_t launched = 0;
+  if (maxit > 0) {
+    if (opt_.graph_iters > 0) capture_graph();
+    const int per = opt_.graph_iters > 0 ? opt_.graph_iters : 1;
+    for (;;) {
+      GS_HIP(hipMemcpyAsync(host_ctl_, ctl_.get(), sizeof(gsk::Ctl<TC>), hipMemcpyDeviceToHost,
+                            stream_));
+      GS_HIP(hipStreamSynchronize(stream_));
+      if (host_ctl_->stop != 0 || launched >= maxit) break;
+      if (graph_exec_) GS_HIP(hipGraphLaunch(graph_exec_, stream_));
+      else encode_iteration(stop);
+      launched += per;
+    }
+  } else {
+    GS_HIP(hipMemcpyAsync(host_ctl_, ctl_.get(), sizeof(gsk::Ctl<TC>), hipMemcpyDeviceToHost,
+                          stream_));
+  }
+  GS_HIP(hipEventRecord(ev1_, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
+  auto t1 = std::chrono::steady_clock::now();
+  float ms = 0.f;
+  GS_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  res.solve_ms = ms;
+  res.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  const gsk::Ctl<TC> &hc = *host_ctl_;
+  if (hc.stop == 3) {
+    if (hc.iter == 0)
+      throw std::runtime_error("CGSolver: initial residual is NaN/Inf - check matrix and RHS.");
+    throw std::runtime_error("CGSolver: solver diverged (NaN/Inf residual).");
+  }
+  res.iterations = (maxit == 0) ? 0 : hc.iters_out;
+  res.loop_iterations = hc.iter;
+  res.relres2 = static_cast<double>(hc.rel);
+  res.converged = hc.stop == 1;
+  res.breakdown = hc.breakdown;
+  // History.
+  const int64_t hl = std::min<int64_t>(hc.iter + 1, maxit + 1);
+  std::vector<TC> h(hl);
+  GS_HIP(hipMemcpy(h.data(), hist_.get(), hl * sizeof(TC), hipMemcpyDeviceToHost));
+  hist_host_.assign(h.begin(), h.end());
+  // Solution back to the caller's ordering.
+  GS_HIP(hipMemcpy(xi.data(), x_.get(), n_ * sizeof(TC), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n_; ++i) x[int2orig_[i]] = static_cast<double>(xi[i]);
+  // True residual of the CG operator in FP64 arithmetic.
+  if (Acg_.b3)
+    gsk::k_true_residual_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(
+        Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), tr_.template as<double>(), partials_.template as<double>(),
+        &ctl->ticket[0]);
+  else
+    gsk::k_true_residual<TC><<<grid_for(n_), kBlock, 0, stream_>>>(
+        Acg_.s.view(), b_.template as<TC>(), x_.template as<TC>(), tr_.template as<double>(), partials_.template as<double>(),
+        &ctl->ticket[0]);
+  double tr[2];
+  GS_HIP(hipMemcpyAsync(tr, tr_.get(), sizeof(tr), hipMemcpyDeviceToHost, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
+  res.true_relres = std::sqrt(tr[0] / tr[1]);
+  return res;
+}
+
+// z = M r for r given in level-0 (permuted) ordering; one preconditioner
+// application exactly as inside the loop.
+template <class TC, class TM>
+void Engine<TC, TM>::vcycle_level_order(const double *r, double *z) {
+  if (!setup_done_) throw std::runtime_error("solver is not set up.");
+  std::vector<TM> ri(n_), zi(n_);
+  for (int64_t i = 0; i < n_; ++i) ri[i] = static_cast<TM>(r[int2lvl0_[i]]);
+  GS_HIP(hipMemcpy(rvc_, ri.data(), n_ * sizeof(TM), hipMemcpyHostToDevice));
+  apply_preconditioner(nullptr);
+  GS_HIP(hipStreamSynchronize(stream_));
+  GS_HIP(hipMemcpy(zi.data(), z_.get(), n_ * sizeof(TM), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n_; ++i) z[int2lvl0_[i]] = static_cast<double>(zi[i]);
+}
+
+// y = A x with the CG operator, vectors in level-0 (permuted) ordering.
+template <class TC, class TM>
+void Engine<TC, TM>::spmv_level_order(const double *x, double *y) {
+  if (!setup_done_) throw std::runtime_error("solver is not set up.");
+  std::vector<TC> xi(n_), yi(n_);
+  for (int64_t i = 0; i < n_; ++i) xi[i] = static_cast<TC>(x[int2lvl0_[i]]);
+  GS_HIP(hipMemcpy(p_.get(), xi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice));
+  if (Acg_.b3)
+    gsk::k_spmv_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(),
+                                                                 q_.template as<TC>(), nullptr);
+  else
+    gsk::k_spmv<TC><<<grid_for(n_), kBlock, 0, stream_>>>(Acg_.s.view(), p_.template as<TC>(), q_.template as<TC>(),
+                                                          nullptr);
+  GS_HIP(hipStreamSynchronize(stream_));
+  GS_HIP(hipMemcpy(yi.data(), q_.get(), n_ * sizeof(TC), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < n_; ++i) y[int2lvl0_[i]] = static_cast<double>(yi[i]);
+}
+
+// Average duration (ms) of one launch of a hot kernel, timed with HIP events
+// on the solver's own stream over `reps` back-to-back launches. which: 0 =
+// CG SpMV (y = A p), 1 = one V-cycle, 2 = one full PCG iteration, 3 = level-0
+// forward+backward smoothing sweep. *bytes receives the algorithmic bytes
+// of one launch (0 when not defined).
+template <class TC, class TM>
+double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
+  if (!setup_done_) throw std::runtime_error("solver is not set up.");
+  *bytes = 0.0;
+  auto body = [&]() {
+    if (which == 0) {
+      if (Acg_.b3)
+        gsk::k_spmv_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(),
+                                                                     q_.template as<TC>(), nullptr);
+      else
+        gsk::k_spmv<TC><<<grid_for(n_), kBlock, 0, stream_>>>(Acg_.s.view(), p_.template as<TC>(),
+                                                              q_.template as<TC>(), nullptr);
+    } else if (which == 1) {
+      apply_preconditioner(nullptr);
+    } else if (which == 2) {
+      encode_iteration(nullptr);
+    } else {
+      encode_smooth(L_[0], 1, false, nullptr);
+    }
+  };
+  if (which == 0) *bytes = Acg_.spmv_bytes();
+  if (which == 3 && !L_.empty()) {
+    const double s = sizeof(TM);
+    const auto &lv = L_[0];
+    *bytes = lv.A.b3 ? 2 * (lv.A.b.nblocks * (9 * s + 4) + 3.0 * lv.n * s + 9.0 * (lv.n / 3) * s)
+                     : 2 * (lv.A.s.nnz * (s + 4) + 3.0 * lv.n * s);
+  }
+  // Keep the iteration kernels live: a zeroed control block with stop = 0.
+  if (which == 2) {
+    gsk::Ctl<TC> c0{};
+    c0.tol2 = TC(0);
+    c0.maxit = int64_t(1) << 40;
+    c0.rho = TC(1);
+    c0.bb = TC(1);
+    c0.hist = nullptr;
+    GS_HIP(hipMemcpyAsync(ctl_.get(), &c0, sizeof(c0), hipMemcpyHostToDevice, stream_));
+  }
+  body();  // warm-up
+  GS_HIP(hipStreamSynchronize(stream_));
+  GS_HIP(hipEventRecord(ev0_, stream_));
+  for (int i = 0; i < reps; ++i) body();
+  GS_HIP(hipEventRecord(ev1_, stream_));
+  GS_HIP(hipEventSynchronize(ev1_));
+  float ms = 0.f;
+  GS_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+  return ms / reps;
+}
+
+template <class TC, class TM>
+void Engine<TC, TM>::level_info(std::vector<LevelInfo> &out) const {
+  out.clear();
+  for (int k = 0; k < nlev_ && k < static_cast<int>(L_.size()); ++k) {
+    const auto &lv = L_[k];
+    LevelInfo li;
+    li.n = lv.n;
+    li.nnz = lv.A.b3 ? lv.A.b.nblocks * 9 : lv.A.s.nnz;
+    li.blocks3 = lv.A.b3 ? lv.A.b.nblocks : 0;
+    li.ncolors = lv.ncolors;
+    li.kind = lv.kind;
+    li.direct = lv.direct;
+    li.omega = static_cast<double>(lv.omega);
+    li.pre = lv.pre;
+    li.post = lv.post;
+    out.push_back(li);
+  }
+}
+
+template <class TC, class TM>
+std::string Engine<TC, TM>::describe() const {
+  std::ostringstream os;
+  os << "{\"n\": " << n_ << ", \"levels\": [";
+  std::vector<LevelInfo> li;
+  level_info(li);
+  for (size_t k = 0; k < li.size(); ++k) {
+    if (k) os << ", ";
+    os << "{\"n\": " << li[k].n << ", \"nnz\": " << li[k].nnz << ", \"blocks3\": " << li[k].blocks3
+       << ", \"colors\": " << li[k].ncolors << ", \"kind\": " << li[k].kind
+       << ", \"direct\": " << (li[k].direct ? "true" : "false") << ", \"omega\": " << li[k].omega
+       << ", \"pre\": " << li[k].pre << ", \"post\": " << li[k].post << "}";
+  }
+  os << "], \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
+     << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters
+     << "}";
+  return os.str();
+}
+
+std::unique_ptr<EngineBase> make_engine(int precision, const Options &o) {
+  switch (precision) {
+    case kPrecDouble: return std::unique_ptr<EngineBase>(new Engine<double, double>(o));
+    case kPrecMixed: return std::unique_ptr<EngineBase>(new Engine<double, float>(o));
+    case kPrecSingle: return std::unique_ptr<EngineBase>(new Engine<float, float>(o));
+    default: throw std::runtime_error("unknown precision mode.");
+  }
+}
+
+}  // namespace gse

@@ -1,0 +1,69 @@
+// Solve-engine interface shared by the engine and the C ABI.
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "host/setup.h"
+
+namespace gse {
+
+// Smoother kinds in the order of MultigridSmoother::SmootherType
+// (include/GPUSolver/Smoothers/MultigridSmoother.h:18-23).
+enum { kDampedJacobi = 0, kSGS = 1, kBlockJacobi = 2, kBlockGS = 3 };
+enum { kPrecDouble = 0, kPrecMixed = 1, kPrecSingle = 2 };
+enum { kPrecIdentity = 0, kPrecMultigrid = 1 };
+
+struct Options {
+  int fine_kind = kSGS;
+  int coarse_kind = kSGS;
+  int pre = 1, post = 1, coarse_steps = 2, vcycles = 1;
+  double omega = 1.0;
+  bool coarse_direct = false;
+  int precond = kPrecMultigrid;
+  int graph_iters = 8;
+  bool cg_block3 = false;  // store the CG operator as 3x3 blocks when possible
+  std::vector<double> level_omega;  // per-level overrides (> 0 applies)
+  std::vector<int> level_sweeps;    // per-level pre = post sweeps (> 0 applies)
+};
+
+struct SolveResult {
+  int64_t iterations = 0;       // reference convention (src/CGSolver.cc:257-268)
+  int64_t loop_iterations = 0;  // PCG updates actually applied
+  double relres2 = 0.0;         // final recursive ||r||^2 / ||b||^2
+  double true_relres = 0.0;     // ||b - A x|| / ||b|| recomputed in FP64
+  double solve_ms = 0.0;        // HIP-event time of the solve on the device
+  double wall_ms = 0.0;         // host wall time of the same region
+  bool converged = false;
+  bool zero_rhs = false;
+  int breakdown = 0;            // 1: p.Ap <= 0 seen, 2: r.z <= 0 seen
+};
+
+struct LevelInfo {
+  int64_t n = 0, nnz = 0, blocks3 = 0;
+  int ncolors = 0, kind = 0;
+  bool direct = false;
+  double omega = 1.0;
+  int pre = 1, post = 1;
+};
+
+class EngineBase {
+ public:
+  virtual ~EngineBase() = default;
+  virtual void setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm> &perms) = 0;
+  virtual SolveResult solve(const double *b, double *x, double tol, int64_t maxit) = 0;
+  virtual void vcycle_level_order(const double *r, double *z) = 0;
+  virtual void spmv_level_order(const double *x, double *y) = 0;
+  virtual double time_kernel(int which, int reps, double *bytes) = 0;
+  virtual std::string describe() const = 0;
+  virtual int64_t size() const = 0;
+  virtual const std::vector<double> &history() const = 0;
+  virtual void level_info(std::vector<LevelInfo> &out) const = 0;
+};
+
+std::unique_ptr<EngineBase> make_engine(int precision, const Options &o);
+
+}  // namespace gse

@@ -1,0 +1,140 @@
+// Greedy multicolouring, reverse Cuthill-McKee, block graphs and block
+// permutation expansion.
+#include "setup.h"
+
+#include <algorithm>
+#include <limits>
+#include <queue>
+
+namespace gsx {
+
+// src/GraphColoring.cc:9-72: first-fit in vertex order; a colour is
+// forbidden for vertex i when an already coloured neighbour holds it. The
+// forbidden table is stamped with the vertex index, so it never needs
+// clearing.
+Coloring greedy_coloring(int64_t n, const int32_t *ptr, const int32_t *col) {
+  Coloring out;
+  out.colors.assign(n, -1);
+  std::vector<int64_t> stamp(n + 1, -1);
+  int32_t nc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int32_t j = col[k];
+      if (j != i && out.colors[j] >= 0) stamp[out.colors[j]] = i;
+    }
+    int32_t c = 0;
+    while (stamp[c] == i) ++c;
+    out.colors[i] = c;
+    nc = std::max(nc, c + 1);
+  }
+  out.ncolors = nc;
+  out.offsets.assign(nc + 1, 0);
+  for (int64_t v = 0; v < n; ++v) out.offsets[out.colors[v] + 1]++;
+  for (int32_t c = 0; c < nc; ++c) out.offsets[c + 1] += out.offsets[c];
+  out.vertices.assign(n, -1);
+  std::vector<int32_t> next(out.offsets.begin(), out.offsets.end() - 1);
+  for (int64_t v = 0; v < n; ++v)
+    out.vertices[next[out.colors[v]]++] = static_cast<int32_t>(v);
+  return out;
+}
+
+// src/ReorderingStrategies/RCMReordering.cc:11-110. Degree counts the
+// off-diagonal entries of a row; each connected component starts at the
+// lowest-index unvisited vertex of minimum degree; neighbours are visited in
+// std::sort order of degree (the same comparator and input order as the
+// reference, so libstdc++ breaks ties the same way); the Cuthill-McKee
+// order is reversed at the end.
+Perm rcm(int64_t n, const int32_t *ptr, const int32_t *col) {
+  std::vector<int> degree(n);
+  std::vector<char> visited(n, 0);
+  for (int64_t r = 0; r < n; ++r) {
+    int d = 0;
+    for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k)
+      if (col[k] != r) ++d;
+    degree[r] = d;
+  }
+  std::vector<int32_t> cm;
+  cm.reserve(n);
+  std::queue<int32_t> q;
+  std::vector<int> nb;
+  const int *deg = degree.data();
+  while (static_cast<int64_t>(cm.size()) < n) {
+    int32_t start = -1;
+    int best = std::numeric_limits<int>::max();
+    for (int64_t r = 0; r < n; ++r)
+      if (!visited[r] && degree[r] < best) {
+        best = degree[r];
+        start = static_cast<int32_t>(r);
+      }
+    q.push(start);
+    visited[start] = 1;
+    while (!q.empty()) {
+      const int32_t v = q.front();
+      q.pop();
+      cm.push_back(v);
+      nb.clear();
+      for (int32_t k = ptr[v]; k < ptr[v + 1]; ++k)
+        if (col[k] != v) nb.push_back(col[k]);
+      std::sort(nb.begin(), nb.end(), [deg](int a, int b) { return deg[a] < deg[b]; });
+      for (int w : nb)
+        if (!visited[w]) {
+          visited[w] = 1;
+          q.push(w);
+        }
+    }
+  }
+  Perm p;
+  p.new_to_old.resize(n);
+  p.old_to_new.resize(n);
+  for (int64_t i = 0; i < n; ++i) p.new_to_old[i] = cm[n - 1 - i];
+  for (int64_t i = 0; i < n; ++i) p.old_to_new[p.new_to_old[i]] = static_cast<int32_t>(i);
+  return p;
+}
+
+// src/BlockUtils.cc:126-216: every scalar entry coupling two different
+// blocks adds the edge in both directions; lists are sorted and
+// de-duplicated.
+Graph block_graph(const Csr &A, int32_t bs) {
+  if (A.rows != A.cols) throw std::runtime_error("buildBlockGraph: matrix must be square.");
+  if (A.rows % bs != 0)
+    throw std::runtime_error("buildBlockGraph: matrix size is not divisible by block size.");
+  const int64_t nb = A.rows / bs;
+  std::vector<std::vector<int32_t>> adj(nb);
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int64_t br = r / bs;
+    for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+      const int64_t bc = A.col[k] / bs;
+      if (bc == br) continue;
+      adj[br].push_back(static_cast<int32_t>(bc));
+      adj[bc].push_back(static_cast<int32_t>(br));
+    }
+  }
+  Graph g;
+  g.ptr.assign(nb + 1, 0);
+  for (int64_t b = 0; b < nb; ++b) {
+    auto &v = adj[b];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    g.ptr[b + 1] = g.ptr[b] + static_cast<int32_t>(v.size());
+  }
+  g.col.reserve(g.ptr[nb]);
+  for (auto &v : adj) g.col.insert(g.col.end(), v.begin(), v.end());
+  return g;
+}
+
+// src/BlockUtils.cc:7-55.
+Perm expand_block_perm(const Perm &np, int64_t nodes, int32_t bs) {
+  Perm p;
+  p.old_to_new.resize(nodes * bs);
+  p.new_to_old.resize(nodes * bs);
+  for (int64_t o = 0; o < nodes; ++o)
+    for (int32_t c = 0; c < bs; ++c)
+      p.old_to_new[bs * o + c] = bs * np.old_to_new[o] + c;
+  for (int64_t nw = 0; nw < nodes; ++nw)
+    for (int32_t c = 0; c < bs; ++c)
+      p.new_to_old[bs * nw + c] = bs * np.new_to_old[nw] + c;
+  p.check();
+  return p;
+}
+
+}  // namespace gsx

@@ -1,0 +1,189 @@
+// Host-side setup for the AMG-PCG solve phase: CSR containers, the two
+// permutation flavours, ELL / BSR3-ELL packing, greedy multicolouring, RCM,
+// block utilities, smoothed-aggregation coarsening and the problem
+// generators that stand in for the reference's meshes.
+//
+// Everything here runs once per matrix on the CPU. The per-iteration work
+// lives in the HIP sources next to this directory.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gsx {
+
+// Compressed sparse row matrix with int32 indices (the reference's index
+// type, include/GPUSolver/HostSparseMatrix.h:285-291) and FP64 values.
+// Values are held in FP64 on the host; FP32 device copies are rounded from
+// them once at upload.
+struct Csr {
+  int64_t rows = 0;
+  int64_t cols = 0;
+  std::vector<int32_t> ptr;  // rows + 1
+  std::vector<int32_t> col;  // nnz
+  std::vector<double> val;   // nnz
+  int64_t nnz() const { return static_cast<int64_t>(col.size()); }
+  void check() const;        // structural validation, throws on error
+};
+
+// ---------------------------------------------------------------------------
+// Permutations (src/Permutation.cc:8-35).
+// ---------------------------------------------------------------------------
+struct Perm {
+  std::vector<int32_t> old_to_new;
+  std::vector<int32_t> new_to_old;
+  static Perm identity(int64_t n);
+  static Perm from_new_to_old(const std::vector<int32_t> &n2o);
+  void check() const;  // bijectivity, throws like the reference ctor
+};
+
+// A' = P A P^T keeping every row's entry order (columns relabelled only),
+// as HostCSRMatrix::applyPermutation (src/HostSparseMatrix.cc:118-249).
+Csr permute_keep_order(const Csr &A, const Perm &p);
+
+// A' with rows relabelled by rowOldToNew and columns by colOldToNew, columns
+// sorted ascending within each row and duplicates summed, as permuteMatrix
+// (src/SparseMatrixUtils.cc:8-31) through Eigen::setFromTriplets.
+Csr permute_sorted(const Csr &A, const int32_t *row_old_to_new,
+                   const int32_t *col_old_to_new);
+
+Csr transpose(const Csr &A);                // sorted output
+Csr spgemm(const Csr &A, const Csr &B);     // sorted output, OpenMP
+void sort_rows(Csr &A);                     // sort + sum duplicates
+
+// ---------------------------------------------------------------------------
+// ELL packing (src/HostELLMatrix.cc:10-206): slot-major
+// index = slot * rows + row, padding (col 0, value 0), rows longer than the
+// width spill into an overflow CSR in row order.
+// ---------------------------------------------------------------------------
+struct Ell {
+  int64_t rows = 0, cols = 0, nnz = 0;
+  int32_t width = 0;
+  int64_t overflow_nnz = 0;
+  std::vector<int32_t> col;   // width * rows
+  std::vector<double> val;    // width * rows
+  std::vector<int32_t> ovf_ptr;  // rows + 1
+  std::vector<int32_t> ovf_col;
+  std::vector<double> ovf_val;
+};
+Ell ell_pack(const Csr &A, int32_t width);
+int32_t max_row_length(const Csr &A);
+
+// 3x3-block ELL: block row b holds up to `width` blocks stored block-slot
+// major; the 9 values of block slot s of block row b sit at
+// val[(s * 9 + e) * nb + b] for e = 3 * r + c, so a wavefront reading 64
+// consecutive block rows issues 64-wide unit-stride loads per element.
+// Requires every scalar row of a block row to list the same block columns
+// in the same order (true for the assembled elasticity operator and every
+// Galerkin level built from it); blocks missing a scalar entry are padded
+// with exact zeros.
+struct Bsr3Ell {
+  int64_t nb = 0;            // block rows
+  int32_t width = 0;         // max blocks per block row
+  std::vector<int32_t> bcol; // width * nb, block column index (padding: own row)
+  std::vector<double> val;   // 9 * width * nb
+  std::vector<int32_t> nblk; // blocks per block row
+  int64_t nblocks = 0;       // stored (non-padding) blocks
+};
+// Returns false (and leaves out untouched) when A is not representable as
+// 3x3 blocks with sorted, block-aligned rows.
+bool bsr3_pack(const Csr &A, Bsr3Ell &out);
+
+// ---------------------------------------------------------------------------
+// Colouring and orderings.
+// ---------------------------------------------------------------------------
+// Sequential greedy first-fit colouring in vertex order plus the counting
+// sort into colour groups (src/GraphColoring.cc:9-72).
+struct Coloring {
+  int32_t ncolors = 0;
+  std::vector<int32_t> colors;    // n
+  std::vector<int32_t> vertices;  // n, grouped by colour, ascending inside
+  std::vector<int32_t> offsets;   // ncolors + 1
+};
+Coloring greedy_coloring(int64_t n, const int32_t *ptr, const int32_t *col);
+
+// Reverse Cuthill-McKee with the reference's start-vertex rule, degree
+// definition and std::sort comparator
+// (src/ReorderingStrategies/RCMReordering.cc:11-110).
+Perm rcm(int64_t n, const int32_t *ptr, const int32_t *col);
+
+// Block graph of a scalar matrix (src/BlockUtils.cc:126-216).
+struct Graph {
+  std::vector<int32_t> ptr, col;
+};
+Graph block_graph(const Csr &A, int32_t bs);
+// Node permutation expanded to DOFs (src/BlockUtils.cc:7-55).
+Perm expand_block_perm(const Perm &node_perm, int64_t nodes, int32_t bs);
+
+// Inverse of every 3x3 diagonal block through a Cholesky factorisation in
+// FP64; throws when a block is not SPD (src/BlockUtils.cc:218-277).
+std::vector<double> inverse_diag_blocks3(const Csr &A);
+// Inverse diagonal, throws on zero / missing diagonal
+// (src/Smoothers/DampedJacobiSmoother.cc:14-56).
+std::vector<double> inverse_diagonal(const Csr &A);
+
+// ---------------------------------------------------------------------------
+// Smoothed aggregation (amgcl smoothed_aggregation as called from
+// examples/amgSolver/src/SmoothedAggregationCoarsener.cc:64-108).
+// ---------------------------------------------------------------------------
+struct SAParams {
+  int32_t block_size = 1;
+  double eps_strong = 0.1;
+  double relax = 1.0;          // prolongation smoother weight factor (x 2/3)
+  int32_t max_levels = 10;
+  int64_t min_dofs = 1000;
+};
+struct Hierarchy {
+  std::vector<Csr> A;  // levels
+  std::vector<Csr> P;  // levels - 1, P[k]: level k+1 -> level k
+  std::vector<Csr> R;  // levels - 1
+  std::vector<std::vector<int32_t>> aggregates;  // per transfer, fine DOF -> aggregate (or -1)
+};
+// nullspace: n x ncols row-major near-nullspace (may be empty).
+Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
+                             const std::vector<double> &nullspace,
+                             int32_t ncols);
+
+// ---------------------------------------------------------------------------
+// Problem generators (counter-based SplitMix64 streams, reproducible in
+// numpy with uint64 arithmetic).
+// ---------------------------------------------------------------------------
+uint64_t splitmix_at(uint64_t seed, uint64_t i);
+// U[-1,1) stream: 2 * ((splitmix_at(seed, i) >> 11) * 2^-53) - 1.
+void uniform_pm1(uint64_t seed, int64_t n, double *out);
+// Smoothed Gaussian random field on an nx*ny*nz cell grid: white noise by
+// Box-Muller, separable Gaussian filter of width sigma (cells, truncated at
+// 4 sigma, zero padded), renormalised to zero mean and unit variance.
+std::vector<double> gaussian_random_field(int nx, int ny, int nz,
+                                          uint64_t seed, double sigma);
+
+Csr poisson2d(int nx, int ny);          // 5-point (4,-1), Dirichlet eliminated
+Csr poisson3d(int nx, int ny, int nz);  // 7-point (6,-1)
+// -div(k grad u) on n^3 cells, k = exp(scale * g), harmonic face means,
+// Dirichlet faces with coefficient 2 k.
+Csr varcoef3d(int n, uint64_t seed, double sigma, double scale,
+              std::vector<double> *kfield);
+
+struct ElasticityProblem {
+  Csr A;
+  std::vector<double> rhs;        // 3 * free vertices
+  std::vector<double> nullspace;  // (3 * free) x 6 row-major rigid body modes
+  int64_t free_vertices = 0;
+  int64_t tets = 0;
+};
+// Unit cube with nv^3 vertices, 6 Kuhn tets per hex (split along the
+// (1,1,1) diagonal), linear elasticity element matrices as
+// examples/elasticity/LinearElasticity.ipp:239-266, duplicates summed
+// (:346-348), the x = 0 face clamped and eliminated. hetero_sigma > 0 draws
+// E = E0 * exp(hetero_scale * g) per hex from a Gaussian random field.
+ElasticityProblem elasticity(int nv, double E0, double nu, double hetero_sigma,
+                             double hetero_scale, uint64_t seed);
+
+// Dense SPD inverse through Cholesky in FP64 (coarsest-level direct solve).
+// a: n x n row-major, overwritten with the inverse. Throws if not SPD.
+void spd_inverse(int64_t n, std::vector<double> &a);
+
+}  // namespace gsx

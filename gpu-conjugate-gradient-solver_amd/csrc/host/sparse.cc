@@ -1,0 +1,483 @@
+// CSR containers, permutations, transposes, SpGEMM, ELL and BSR3-ELL
+// packing, diagonal (block) inverses and the dense SPD inverse used by the
+// coarsest-level direct solve.
+#include "setup.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace gsx {
+
+void Csr::check() const {
+  if (rows <= 0 || cols <= 0)
+    throw std::runtime_error("HostCSRMatrix: matrix dimensions must be nonzero.");
+  if (static_cast<int64_t>(ptr.size()) != rows + 1)
+    throw std::runtime_error("HostCSRMatrix: row pointer has the wrong length.");
+  if (ptr[0] != 0 || static_cast<int64_t>(ptr[rows]) != nnz() ||
+      col.size() != val.size())
+    throw std::runtime_error("HostCSRMatrix: inconsistent nonzero count.");
+  for (int64_t r = 0; r < rows; ++r)
+    if (ptr[r + 1] < ptr[r])
+      throw std::runtime_error("HostCSRMatrix: invalid CSR row pointer.");
+  for (int32_t c : col)
+    if (c < 0 || c >= cols)
+      throw std::runtime_error("HostCSRMatrix: column index out of bounds.");
+}
+
+Perm Perm::identity(int64_t n) {
+  Perm p;
+  p.old_to_new.resize(n);
+  std::iota(p.old_to_new.begin(), p.old_to_new.end(), 0);
+  p.new_to_old = p.old_to_new;
+  return p;
+}
+
+Perm Perm::from_new_to_old(const std::vector<int32_t> &n2o) {
+  Perm p;
+  p.new_to_old = n2o;
+  p.old_to_new.assign(n2o.size(), -1);
+  for (size_t i = 0; i < n2o.size(); ++i) {
+    if (n2o[i] < 0 || static_cast<size_t>(n2o[i]) >= n2o.size())
+      throw std::runtime_error("Permutation: newToOld contains invalid index.");
+    p.old_to_new[n2o[i]] = static_cast<int32_t>(i);
+  }
+  p.check();
+  return p;
+}
+
+void Perm::check() const {
+  const size_t n = old_to_new.size();
+  if (n == 0) throw std::runtime_error("Permutation: size must be nonzero.");
+  if (new_to_old.size() != n)
+    throw std::runtime_error("Permutation: mappings have different sizes.");
+  for (size_t o = 0; o < n; ++o) {
+    const int32_t nw = old_to_new[o];
+    if (nw < 0 || static_cast<size_t>(nw) >= n)
+      throw std::runtime_error("Permutation: oldToNew contains invalid index.");
+    if (new_to_old[nw] != static_cast<int32_t>(o))
+      throw std::runtime_error("Permutation: mappings are not inverses.");
+  }
+}
+
+Csr permute_keep_order(const Csr &A, const Perm &p) {
+  if (A.rows != A.cols)
+    throw std::runtime_error(
+        "HostCSRMatrix::applyPermutation: matrix must be square.");
+  if (static_cast<int64_t>(p.old_to_new.size()) != A.rows)
+    throw std::runtime_error(
+        "HostCSRMatrix::applyPermutation: permutation size does not match "
+        "matrix dimensions.");
+  Csr B;
+  B.rows = A.rows;
+  B.cols = A.cols;
+  B.ptr.assign(A.rows + 1, 0);
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int32_t o = p.new_to_old[r];
+    B.ptr[r + 1] = B.ptr[r] + (A.ptr[o + 1] - A.ptr[o]);
+  }
+  B.col.resize(A.nnz());
+  B.val.resize(A.nnz());
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int32_t o = p.new_to_old[r];
+    int64_t d = B.ptr[r];
+    for (int32_t k = A.ptr[o]; k < A.ptr[o + 1]; ++k, ++d) {
+      B.col[d] = p.old_to_new[A.col[k]];
+      B.val[d] = A.val[k];
+    }
+  }
+  return B;
+}
+
+// Sort each row by column and sum duplicates in their stored order.
+void sort_rows(Csr &A) {
+  std::vector<int32_t> newptr(A.rows + 1, 0);
+  std::vector<int32_t> counts(A.rows, 0);
+#pragma omp parallel
+  {
+    std::vector<std::pair<int32_t, int64_t>> tmp;
+#pragma omp for schedule(dynamic, 256)
+    for (int64_t r = 0; r < A.rows; ++r) {
+      const int64_t b = A.ptr[r], e = A.ptr[r + 1];
+      tmp.clear();
+      for (int64_t k = b; k < e; ++k) tmp.emplace_back(A.col[k], k);
+      std::stable_sort(tmp.begin(), tmp.end(),
+                       [](const std::pair<int32_t, int64_t> &x,
+                          const std::pair<int32_t, int64_t> &y) {
+                         return x.first < y.first;
+                       });
+      // Write the collapsed row in place after buffering values.
+      std::vector<double> v;
+      std::vector<int32_t> c;
+      v.reserve(tmp.size());
+      c.reserve(tmp.size());
+      for (size_t i = 0; i < tmp.size(); ++i) {
+        if (!c.empty() && c.back() == tmp[i].first)
+          v.back() += A.val[tmp[i].second];
+        else {
+          c.push_back(tmp[i].first);
+          v.push_back(A.val[tmp[i].second]);
+        }
+      }
+      for (size_t i = 0; i < c.size(); ++i) {
+        A.col[b + i] = c[i];
+        A.val[b + i] = v[i];
+      }
+      counts[r] = static_cast<int32_t>(c.size());
+    }
+  }
+  for (int64_t r = 0; r < A.rows; ++r) newptr[r + 1] = newptr[r] + counts[r];
+  if (newptr[A.rows] != A.ptr[A.rows]) {
+    std::vector<int32_t> col(newptr[A.rows]);
+    std::vector<double> val(newptr[A.rows]);
+    for (int64_t r = 0; r < A.rows; ++r) {
+      std::memcpy(&col[newptr[r]], &A.col[A.ptr[r]], counts[r] * sizeof(int32_t));
+      std::memcpy(&val[newptr[r]], &A.val[A.ptr[r]], counts[r] * sizeof(double));
+    }
+    A.col.swap(col);
+    A.val.swap(val);
+  }
+  A.ptr.swap(newptr);
+}
+
+Csr permute_sorted(const Csr &A, const int32_t *row_o2n,
+                   const int32_t *col_o2n) {
+  Csr B;
+  B.rows = A.rows;
+  B.cols = A.cols;
+  B.ptr.assign(A.rows + 1, 0);
+  std::vector<int32_t> src_row(A.rows);
+  for (int64_t r = 0; r < A.rows; ++r) src_row[row_o2n[r]] = static_cast<int32_t>(r);
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int32_t o = src_row[r];
+    B.ptr[r + 1] = B.ptr[r] + (A.ptr[o + 1] - A.ptr[o]);
+  }
+  B.col.resize(A.nnz());
+  B.val.resize(A.nnz());
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int32_t o = src_row[r];
+    int64_t d = B.ptr[r];
+    for (int32_t k = A.ptr[o]; k < A.ptr[o + 1]; ++k, ++d) {
+      B.col[d] = col_o2n[A.col[k]];
+      B.val[d] = A.val[k];
+    }
+  }
+  sort_rows(B);
+  return B;
+}
+
+Csr transpose(const Csr &A) {
+  Csr T;
+  T.rows = A.cols;
+  T.cols = A.rows;
+  T.ptr.assign(A.cols + 1, 0);
+  for (int32_t c : A.col) T.ptr[c + 1]++;
+  for (int64_t i = 0; i < A.cols; ++i) T.ptr[i + 1] += T.ptr[i];
+  T.col.resize(A.nnz());
+  T.val.resize(A.nnz());
+  std::vector<int32_t> next(T.ptr.begin(), T.ptr.end() - 1);
+  for (int64_t r = 0; r < A.rows; ++r)
+    for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+      const int32_t d = next[A.col[k]]++;
+      T.col[d] = static_cast<int32_t>(r);
+      T.val[d] = A.val[k];
+    }
+  return T;  // rows of T are visited in ascending r: already sorted
+}
+
+Csr spgemm(const Csr &A, const Csr &B) {
+  if (A.cols != B.rows) throw std::runtime_error("spgemm: shape mismatch.");
+  Csr C;
+  C.rows = A.rows;
+  C.cols = B.cols;
+  C.ptr.assign(A.rows + 1, 0);
+  std::vector<int32_t> rowlen(A.rows, 0);
+#pragma omp parallel
+  {
+    std::vector<int32_t> mark(B.cols, -1);
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t r = 0; r < A.rows; ++r) {
+      int32_t cnt = 0;
+      for (int32_t ka = A.ptr[r]; ka < A.ptr[r + 1]; ++ka) {
+        const int32_t j = A.col[ka];
+        for (int32_t kb = B.ptr[j]; kb < B.ptr[j + 1]; ++kb) {
+          const int32_t c = B.col[kb];
+          if (mark[c] != static_cast<int32_t>(r)) {
+            mark[c] = static_cast<int32_t>(r);
+            ++cnt;
+          }
+        }
+      }
+      rowlen[r] = cnt;
+    }
+  }
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int64_t next = static_cast<int64_t>(C.ptr[r]) + rowlen[r];
+    if (next > INT32_MAX) throw std::runtime_error("spgemm: nnz exceeds int32.");
+    C.ptr[r + 1] = static_cast<int32_t>(next);
+  }
+  C.col.resize(C.ptr[A.rows]);
+  C.val.resize(C.ptr[A.rows]);
+#pragma omp parallel
+  {
+    std::vector<int32_t> pos(B.cols, -1);
+    std::vector<int32_t> cols;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t r = 0; r < A.rows; ++r) {
+      cols.clear();
+      const int32_t base = C.ptr[r];
+      for (int32_t ka = A.ptr[r]; ka < A.ptr[r + 1]; ++ka) {
+        const int32_t j = A.col[ka];
+        const double a = A.val[ka];
+        for (int32_t kb = B.ptr[j]; kb < B.ptr[j + 1]; ++kb) {
+          const int32_t c = B.col[kb];
+          if (pos[c] < 0) {
+            pos[c] = static_cast<int32_t>(cols.size());
+            cols.push_back(c);
+            C.val[base + pos[c]] = a * B.val[kb];
+          } else {
+            C.val[base + pos[c]] += a * B.val[kb];
+          }
+        }
+      }
+      // Sort the row by column, carrying values.
+      std::vector<std::pair<int32_t, double>> tmp(cols.size());
+      for (size_t i = 0; i < cols.size(); ++i)
+        tmp[i] = {cols[i], C.val[base + i]};
+      std::sort(tmp.begin(), tmp.end(),
+                [](const std::pair<int32_t, double> &x,
+                   const std::pair<int32_t, double> &y) { return x.first < y.first; });
+      for (size_t i = 0; i < tmp.size(); ++i) {
+        C.col[base + i] = tmp[i].first;
+        C.val[base + i] = tmp[i].second;
+        pos[tmp[i].first] = -1;
+      }
+    }
+  }
+  return C;
+}
+
+int32_t max_row_length(const Csr &A) {
+  int32_t w = 0;
+  for (int64_t r = 0; r < A.rows; ++r) w = std::max(w, A.ptr[r + 1] - A.ptr[r]);
+  return w;
+}
+
+Ell ell_pack(const Csr &A, int32_t width) {
+  if (A.rows == 0 || A.cols == 0)
+    throw std::runtime_error("HostELLMatrix: matrix dimensions must be nonzero.");
+  if (width <= 0) throw std::runtime_error("HostELLMatrix: ELL width must be nonzero.");
+  Ell E;
+  E.rows = A.rows;
+  E.cols = A.cols;
+  E.nnz = A.nnz();
+  E.width = width;
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int32_t len = A.ptr[r + 1] - A.ptr[r];
+    if (len < 0) throw std::runtime_error("HostELLMatrix: invalid CSR row pointer.");
+    if (len > width) E.overflow_nnz += len - width;
+  }
+  const int64_t slots = static_cast<int64_t>(width) * A.rows;
+  E.col.assign(slots, 0);
+  E.val.assign(slots, 0.0);
+  E.ovf_ptr.assign(A.rows + 1, 0);
+  E.ovf_col.resize(E.overflow_nnz);
+  E.ovf_val.resize(E.overflow_nnz);
+  int64_t op = 0;
+  for (int64_t r = 0; r < A.rows; ++r) {
+    const int32_t b = A.ptr[r], e = A.ptr[r + 1];
+    const int32_t inell = std::min(width, e - b);
+    for (int32_t s = 0; s < inell; ++s) {
+      E.col[static_cast<int64_t>(s) * A.rows + r] = A.col[b + s];
+      E.val[static_cast<int64_t>(s) * A.rows + r] = A.val[b + s];
+    }
+    for (int32_t k = b + inell; k < e; ++k, ++op) {
+      E.ovf_col[op] = A.col[k];
+      E.ovf_val[op] = A.val[k];
+    }
+    E.ovf_ptr[r + 1] = static_cast<int32_t>(op);
+  }
+  if (op != E.overflow_nnz)
+    throw std::runtime_error("HostELLMatrix: inconsistent overflow size.");
+  return E;
+}
+
+bool bsr3_pack(const Csr &Ain, Bsr3Ell &out) {
+  if (Ain.rows != Ain.cols || Ain.rows % 3 != 0) return false;
+  const Csr *Ap = &Ain;
+  Csr sorted;
+  bool is_sorted = true;
+  for (int64_t r = 0; r < Ain.rows && is_sorted; ++r)
+    for (int32_t k = Ain.ptr[r] + 1; k < Ain.ptr[r + 1]; ++k)
+      if (Ain.col[k] <= Ain.col[k - 1]) { is_sorted = false; break; }
+  if (!is_sorted) {
+    sorted = Ain;
+    sort_rows(sorted);
+    Ap = &sorted;
+  }
+  const Csr &A = *Ap;
+  const int64_t nb = A.rows / 3;
+  // Block columns of each block row: union over its three scalar rows.
+  std::vector<std::vector<int32_t>> bcols(nb);
+  int32_t width = 0;
+  int64_t nblocks = 0;
+#pragma omp parallel for schedule(static) reduction(max : width) reduction(+ : nblocks)
+  for (int64_t b = 0; b < nb; ++b) {
+    std::vector<int32_t> &bc = bcols[b];
+    for (int l = 0; l < 3; ++l) {
+      const int64_t r = 3 * b + l;
+      for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) bc.push_back(A.col[k] / 3);
+    }
+    std::sort(bc.begin(), bc.end());
+    bc.erase(std::unique(bc.begin(), bc.end()), bc.end());
+    width = std::max(width, static_cast<int32_t>(bc.size()));
+    nblocks += static_cast<int64_t>(bc.size());
+  }
+  out.nb = nb;
+  out.width = width;
+  out.nblocks = nblocks;
+  out.bcol.assign(static_cast<int64_t>(width) * nb, 0);
+  out.val.assign(static_cast<int64_t>(9) * width * nb, 0.0);
+  out.nblk.assign(nb, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) {
+    const std::vector<int32_t> &bc = bcols[b];
+    out.nblk[b] = static_cast<int32_t>(bc.size());
+    for (int32_t s = 0; s < width; ++s)
+      out.bcol[static_cast<int64_t>(s) * nb + b] =
+          s < static_cast<int32_t>(bc.size()) ? bc[s] : static_cast<int32_t>(b);
+    for (int l = 0; l < 3; ++l) {
+      const int64_t r = 3 * b + l;
+      size_t s = 0;
+      for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+        const int32_t cb = A.col[k] / 3;
+        while (bc[s] != cb) ++s;
+        const int e = 3 * l + (A.col[k] % 3);
+        out.val[(static_cast<int64_t>(s) * 9 + e) * nb + b] = A.val[k];
+      }
+    }
+  }
+  return true;
+}
+
+static void inverse3x3_spd(const double a[9], double inv[9], bool &ok) {
+  // Cholesky a = L L^T, then inv = L^-T L^-1.
+  double l00 = a[0];
+  if (!(l00 > 0.0)) { ok = false; return; }
+  l00 = std::sqrt(l00);
+  const double l10 = a[3] / l00, l20 = a[6] / l00;
+  double d1 = a[4] - l10 * l10;
+  if (!(d1 > 0.0)) { ok = false; return; }
+  const double l11 = std::sqrt(d1);
+  const double l21 = (a[7] - l20 * l10) / l11;
+  double d2 = a[8] - l20 * l20 - l21 * l21;
+  if (!(d2 > 0.0)) { ok = false; return; }
+  const double l22 = std::sqrt(d2);
+  // M = L^-1 (lower triangular)
+  const double m00 = 1.0 / l00, m11 = 1.0 / l11, m22 = 1.0 / l22;
+  const double m10 = -l10 * m00 / l11;
+  const double m21 = -l21 * m11 / l22;
+  const double m20 = -(l20 * m00 + l21 * m10) / l22;
+  const double M[3][3] = {{m00, 0, 0}, {m10, m11, 0}, {m20, m21, m22}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += M[k][i] * M[k][j];
+      inv[3 * i + j] = s;
+    }
+  ok = true;
+}
+
+std::vector<double> inverse_diag_blocks3(const Csr &A) {
+  if (A.rows % 3 != 0)
+    throw std::runtime_error(
+        "3x3 block smoother requires number of DOFs divisible by 3.");
+  const int64_t nb = A.rows / 3;
+  std::vector<double> inv(9 * nb);
+  bool all_ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : all_ok)
+  for (int64_t b = 0; b < nb; ++b) {
+    double blk[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int l = 0; l < 3; ++l) {
+      const int64_t r = 3 * b + l;
+      for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+        const int64_t c = A.col[k];
+        if (c >= 3 * b && c < 3 * b + 3) blk[3 * l + (c - 3 * b)] = A.val[k];
+      }
+    }
+    bool ok = false;
+    inverse3x3_spd(blk, &inv[9 * b], ok);
+    all_ok = all_ok && ok;
+  }
+  if (!all_ok) throw std::runtime_error("Block diagonal is not SPD.");
+  return inv;
+}
+
+std::vector<double> inverse_diagonal(const Csr &A) {
+  std::vector<double> d(A.rows);
+  for (int64_t r = 0; r < A.rows; ++r) {
+    bool found = false;
+    for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k)
+      if (A.col[k] == r) {
+        if (std::abs(A.val[k]) < 1e-12)
+          throw std::runtime_error(
+              "DampedJacobiPreconditioner::initialize: zero diagonal entry.");
+        d[r] = 1.0 / A.val[k];
+        found = true;
+        break;
+      }
+    if (!found)
+      throw std::runtime_error(
+          "DampedJacobiPreconditioner::initialize: missing diagonal entry.");
+  }
+  return d;
+}
+
+void spd_inverse(int64_t n, std::vector<double> &a) {
+  if (static_cast<int64_t>(a.size()) != n * n)
+    throw std::runtime_error("spd_inverse: size mismatch.");
+  // Right-looking Cholesky, lower triangle, a = L L^T.
+  for (int64_t j = 0; j < n; ++j) {
+    double d = a[j * n + j];
+    for (int64_t k = 0; k < j; ++k) d -= a[j * n + k] * a[j * n + k];
+    if (!(d > 0.0)) throw std::runtime_error("coarse level matrix is not SPD.");
+    const double ljj = std::sqrt(d);
+    a[j * n + j] = ljj;
+#pragma omp parallel for schedule(static) if (n - j > 256)
+    for (int64_t i = j + 1; i < n; ++i) {
+      double s = a[i * n + j];
+      const double *ri = &a[i * n];
+      const double *rj = &a[j * n];
+      for (int64_t k = 0; k < j; ++k) s -= ri[k] * rj[k];
+      a[i * n + j] = s / ljj;
+    }
+  }
+  // M = L^-1, column by column (forward substitution on unit vectors).
+  std::vector<double> m(n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t c = 0; c < n; ++c) {
+    for (int64_t i = c; i < n; ++i) {
+      double s = (i == c) ? 1.0 : 0.0;
+      for (int64_t k = c; k < i; ++k) s -= a[i * n + k] * m[k * n + c];
+      m[i * n + c] = s / a[i * n + i];
+    }
+  }
+  // inv = M^T M.
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j <= i; ++j) {
+      double s = 0.0;
+      for (int64_t k = i; k < n; ++k) s += m[k * n + i] * m[k * n + j];
+      a[i * n + j] = s;
+      a[j * n + i] = s;
+    }
+}
+
+}  // namespace gsx

@@ -1,0 +1,807 @@
+// HIP kernels of the AMG-PCG solve phase for gfx950 (CDNA4, wave64).
+//
+// Matrix storage on the device is SELL-64: rows are cut into slices of 64
+// (one wavefront), each slice stores its rows slot-major with the slice's
+// own width, so lane l of a wave reads entry j of its row at
+// slice_ptr[s] + 64 * j + l -- unit-stride 64-wide loads for column
+// indices and values, and no padding beyond the widest row of a slice. A
+// row's entries keep their CSR order, so every row sum here is computed in
+// exactly the reference's order (src/CPU/CPUBackend.cc:62-86). Padding
+// slots hold value 0 and a valid column, and add nothing.
+//
+// 3x3-block operators (elasticity) use BSR3-SELL-64: one thread per block
+// row, 64 block rows per slice, per block slot one block-column index and
+// nine value planes of 64 lanes each.
+//
+// Reductions are deterministic: each workgroup reduces its chunk with wave
+// shuffles and LDS in a fixed tree, writes one partial, and the last
+// workgroup to arrive (device-scope atomic ticket) reduces the partials in
+// index order and applies the scalar epilogue. The ticket resets itself,
+// so the kernels can be replayed from a captured graph.
+//
+// Every kernel that runs inside the PCG loop takes a pointer to the
+// solver's stop word and returns at once when it is set, so iterations
+// captured past convergence are no-ops.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gsk {
+
+constexpr int kBlock = 256;
+constexpr int kSlice = 64;
+
+template <class T>
+struct Sell {
+  int64_t rows;
+  int64_t cols;
+  const int64_t *slice_ptr;  // entry offset of each slice (multiple of 64)
+  const int32_t *slice_w;    // width of each slice
+  const int32_t *col;
+  const T *val;
+};
+
+template <class T>
+struct Bsr3 {
+  int64_t nb;                // block rows
+  const int64_t *slice_ptr;  // block-slot offset of each slice (multiple of 64)
+  const int32_t *slice_w;    // width (blocks) of each slice
+  const int32_t *bcol;       // block column per block slot
+  const T *val;              // 9 planes of 64 per block slot
+};
+
+// Solver control block, one per solve, in device memory.
+template <class T>
+struct Ctl {
+  T bb;        // b . b
+  T rho;       // r . z of the current iterate
+  T rho_new;
+  T alpha;
+  T beta;
+  T pap;
+  T rr;
+  T rel;       // rr / bb, the recursive relres^2
+  T tol2;
+  int64_t iter;       // loop index of the next iteration
+  int64_t maxit;
+  int64_t iters_out;  // reported iteration count (reference convention)
+  int32_t stop;       // 0 run, 1 converged, 2 maxit, 3 non-finite, 4 breakdown
+  int32_t breakdown;  // 1: p.Ap <= 0, 2: r.z <= 0 (recorded, not fatal)
+  uint32_t ticket[4];
+  T *hist;            // relres^2 per iteration (length maxit + 1) or null
+};
+
+// Blocks are dispatched round-robin over the eight XCDs; remap so each XCD
+// walks one contiguous range of rows, which keeps stencil neighbours of x
+// inside the same L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  const uint32_t g = gridDim.x, b = blockIdx.x;
+  const uint32_t full = g & ~7u;
+  if (b >= full) return b;
+  const uint32_t per = full >> 3;
+  return static_cast<int64_t>(b & 7u) * per + (b >> 3);
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sum over a 256-thread block in a fixed tree; result valid in thread 0.
+template <class T>
+__device__ __forceinline__ T block_sum(T v) {
+  __shared__ T lds[kBlock / 64];
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) lds[w] = v;
+  __syncthreads();
+  T r = T(0);
+  if (threadIdx.x == 0) r = (lds[0] + lds[1]) + (lds[2] + lds[3]);
+  __syncthreads();
+  return r;
+}
+
+// Writes this block's partial and reports whether it arrived last. The
+// last block must then call final_sum over `nparts` partials.
+template <class T>
+__device__ __forceinline__ bool publish_partial(T v, T *partials, uint32_t *ticket) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = v;
+    __threadfence();
+    const uint32_t t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+template <class T>
+__device__ __forceinline__ T final_sum(const T *partials, uint32_t nparts, uint32_t *ticket) {
+  T s = T(0);
+  for (uint32_t i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
+  s = block_sum(s);
+  if (threadIdx.x == 0) *ticket = 0u;
+  return s;
+}
+
+template <class T>
+__device__ __forceinline__ bool finite_val(T v) {
+  return v == v && v - v == T(0);
+}
+
+// ---------------------------------------------------------------------------
+// Row kernels (device helpers).
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ T sell_row_dot(const Sell<T> &A, int64_t row, const T *x) {
+  const int64_t s = row >> 6;
+  const int64_t base = A.slice_ptr[s] + (row & 63);
+  const int w = A.slice_w[s];
+  T sum = T(0);
+#pragma unroll 4
+  for (int j = 0; j < w; ++j) {
+    const int64_t e = base + static_cast<int64_t>(j) * kSlice;
+    sum += A.val[e] * x[A.col[e]];
+  }
+  return sum;
+}
+
+// Off-diagonal sum and diagonal of one row (rowDotExcluding with
+// exclusion [row, row + 1), src/CPU/CPUBackend.cc:62-86).
+template <class T>
+__device__ __forceinline__ T sell_row_offdiag(const Sell<T> &A, int64_t row, const T *x, T &diag) {
+  const int64_t s = row >> 6;
+  const int64_t base = A.slice_ptr[s] + (row & 63);
+  const int w = A.slice_w[s];
+  T sum = T(0);
+  diag = T(0);
+#pragma unroll 4
+  for (int j = 0; j < w; ++j) {
+    const int64_t e = base + static_cast<int64_t>(j) * kSlice;
+    const int32_t c = A.col[e];
+    const T v = A.val[e];
+    if (c == row) {
+      diag = v;
+    } else {
+      sum += v * x[c];
+    }
+  }
+  return sum;
+}
+
+// y(3) = A(3 rows of block row b) x, scalar-row order kept: blocks in slot
+// order, within a block columns 0, 1, 2.
+template <class T>
+__device__ __forceinline__ void bsr3_row_dot(const Bsr3<T> &A, int64_t b, const T *x, T y[3],
+                                             bool skip_diag) {
+  const int64_t s = b >> 6;
+  const int64_t lane = b & 63;
+  const int64_t base = A.slice_ptr[s];
+  const int w = A.slice_w[s];
+  T y0 = T(0), y1 = T(0), y2 = T(0);
+#pragma unroll 2
+  for (int j = 0; j < w; ++j) {
+    const int64_t slot = base + static_cast<int64_t>(j) * kSlice;
+    const int32_t bc = A.bcol[slot + lane];
+    if (skip_diag && bc == b) continue;
+    const T *v = A.val + slot * 9 + lane;
+    const T x0 = x[3 * static_cast<int64_t>(bc)];
+    const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
+    const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
+    y0 += v[0 * kSlice] * x0;
+    y0 += v[1 * kSlice] * x1;
+    y0 += v[2 * kSlice] * x2;
+    y1 += v[3 * kSlice] * x0;
+    y1 += v[4 * kSlice] * x1;
+    y1 += v[5 * kSlice] * x2;
+    y2 += v[6 * kSlice] * x0;
+    y2 += v[7 * kSlice] * x1;
+    y2 += v[8 * kSlice] * x2;
+  }
+  y[0] = y0;
+  y[1] = y1;
+  y[2] = y2;
+}
+
+__device__ __forceinline__ bool stopped(const int32_t *stop) { return stop && *stop; }
+
+// ---------------------------------------------------------------------------
+// SpMV family.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spmv(Sell<T> A, const T *__restrict__ x,
+                                                 T *__restrict__ y, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= A.rows) return;
+  y[row] = sell_row_dot(A, row, x);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_spmv_b3(Bsr3<T> A, const T *__restrict__ x,
+                                                    T *__restrict__ y, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  if (b >= A.nb) return;
+  T v[3];
+  bsr3_row_dot(A, b, x, v, false);
+  y[3 * b] = v[0];
+  y[3 * b + 1] = v[1];
+  y[3 * b + 2] = v[2];
+}
+
+// res = rhs - A x, as rhs + (-1) * (A x) (src/Smoothers/MultigridSmoother.cc:367-371).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_residual(Sell<T> A, const T *__restrict__ rhs,
+                                                     const T *__restrict__ x,
+                                                     T *__restrict__ res, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= A.rows) return;
+  res[row] = rhs[row] - sell_row_dot(A, row, x);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_residual_b3(Bsr3<T> A, const T *__restrict__ rhs,
+                                                        const T *__restrict__ x,
+                                                        T *__restrict__ res, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  if (b >= A.nb) return;
+  T v[3];
+  bsr3_row_dot(A, b, x, v, false);
+  res[3 * b] = rhs[3 * b] - v[0];
+  res[3 * b + 1] = rhs[3 * b + 1] - v[1];
+  res[3 * b + 2] = rhs[3 * b + 2] - v[2];
+}
+
+// rhs_c = R res and corr_c = 0 (the coarse correction is zeroed in the same
+// pass, src/Smoothers/MultigridSmoother.cc:381-388).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict(Sell<T> R, const T *__restrict__ res,
+                                                     T *__restrict__ rhs_c, T *__restrict__ corr_c,
+                                                     const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= R.rows) return;
+  rhs_c[row] = sell_row_dot(R, row, res);
+  if (corr_c) corr_c[row] = T(0);
+}
+
+// x += P e (src/Smoothers/MultigridSmoother.cc:401-404).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__restrict__ e,
+                                                        T *__restrict__ x, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= P.rows) return;
+  const T c = sell_row_dot(P, row, e);
+  x[row] = x[row] + T(1) * c;
+}
+
+// ---------------------------------------------------------------------------
+// Smoothers.
+// ---------------------------------------------------------------------------
+// One colour of multicolour Gauss-Seidel over contiguous (colour-permuted)
+// rows [begin, end): x_i = (1 - w) x_i + w (b_i - sum_{j != i} a_ij x_j) / a_ii
+// (src/CPU/CPUBackend.cc:238-268).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, int64_t begin, int64_t end,
+                                                     const T *__restrict__ rhs, T *x, T omega,
+                                                     const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = begin + xcd_block() * kBlock + threadIdx.x;
+  if (row >= end) return;
+  T diag;
+  const T off = sell_row_offdiag(A, row, x, diag);
+  const T nv = (rhs[row] - off) / diag;
+  x[row] = (T(1) - omega) * x[row] + omega * nv;
+}
+
+// First colour of a forward sweep started from x = 0: the off-diagonal sum
+// of a colour-0 row is exactly zero, so x_i = w * (b_i / a_ii) bit for bit
+// without reading the row; rows of later colours are zeroed in the same
+// pass (the reference's setZero before the V-cycle).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_gs_color_zero(const T *__restrict__ diag,
+                                                          int64_t end0, int64_t n,
+                                                          const T *__restrict__ rhs, T *x,
+                                                          T omega, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= n) return;
+  if (row < end0) {
+    const T nv = rhs[row] / diag[row];
+    x[row] = omega * nv;
+  } else {
+    x[row] = T(0);
+  }
+}
+
+// One colour of 3x3 block Gauss-Seidel over contiguous block rows
+// [begin, end) (src/CPU/CPUBackend.cc:350-387): r_l = b_l - sum over
+// off-diagonal blocks, x_b = (1 - w) x_b + w D_b^-1 r.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_block_gs_color(Bsr3<T> A, const T *__restrict__ dinv,
+                                                           int64_t begin, int64_t end,
+                                                           const T *__restrict__ rhs, T *x,
+                                                           T omega, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t b = begin + xcd_block() * kBlock + threadIdx.x;
+  if (b >= end) return;
+  T off[3];
+  bsr3_row_dot(A, b, x, off, true);
+  T lr[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) lr[l] = rhs[3 * b + l] - off[l];
+  const T *bi = dinv + 9 * b;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T nv = T(0);
+    nv += bi[3 * l + 0] * lr[0];
+    nv += bi[3 * l + 1] * lr[1];
+    nv += bi[3 * l + 2] * lr[2];
+    x[3 * b + l] = (T(1) - omega) * x[3 * b + l] + omega * nv;
+  }
+}
+
+// Block GS colour started from x = 0: r = b, x_b = w D_b^-1 b for the
+// blocks of colour 0, zero for the rest.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_block_gs_color_zero(const T *__restrict__ dinv,
+                                                                int64_t end0, int64_t nb,
+                                                                const T *__restrict__ rhs, T *x,
+                                                                T omega, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  if (b >= nb) return;
+  if (b >= end0) {
+    x[3 * b] = T(0);
+    x[3 * b + 1] = T(0);
+    x[3 * b + 2] = T(0);
+    return;
+  }
+  const T *bi = dinv + 9 * b;
+  const T r0 = rhs[3 * b], r1 = rhs[3 * b + 1], r2 = rhs[3 * b + 2];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T nv = T(0);
+    nv += bi[3 * l + 0] * r0;
+    nv += bi[3 * l + 1] * r1;
+    nv += bi[3 * l + 2] * r2;
+    x[3 * b + l] = omega * nv;
+  }
+}
+
+// Damped Jacobi sweep into a second buffer:
+// xo = x + w * (D^-1 (b + (-1) A x)) (src/Smoothers/DampedJacobiSmoother.cc:79-87).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_jacobi(Sell<T> A, const T *__restrict__ dinv,
+                                                   const T *__restrict__ rhs,
+                                                   const T *__restrict__ x, T *__restrict__ xo,
+                                                   T omega, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= A.rows) return;
+  T c = rhs[row] - sell_row_dot(A, row, x);
+  c = c * dinv[row];
+  xo[row] = x[row] + omega * c;
+}
+
+// 3x3 block Jacobi sweep into a second buffer
+// (src/Smoothers/BlockJacobiSmoother.cc:17-87).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_block_jacobi(Sell<T> A, const T *__restrict__ dinv,
+                                                         const T *__restrict__ rhs,
+                                                         const T *__restrict__ x,
+                                                         T *__restrict__ xo, T omega,
+                                                         const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  if (3 * b >= A.rows) return;
+  T r[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) r[l] = rhs[3 * b + l] - sell_row_dot(A, 3 * b + l, x);
+  const T *bi = dinv + 9 * b;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T s = T(0);
+    s += bi[3 * l + 0] * r[0];
+    s += bi[3 * l + 1] * r[1];
+    s += bi[3 * l + 2] * r[2];
+    xo[3 * b + l] = x[3 * b + l] + omega * s;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_block_jacobi_b3(Bsr3<T> A, const T *__restrict__ dinv,
+                                                            const T *__restrict__ rhs,
+                                                            const T *__restrict__ x,
+                                                            T *__restrict__ xo, T omega,
+                                                            const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  if (b >= A.nb) return;
+  T v[3];
+  bsr3_row_dot(A, b, x, v, false);
+  T r[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) r[l] = rhs[3 * b + l] - v[l];
+  const T *bi = dinv + 9 * b;
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    T s = T(0);
+    s += bi[3 * l + 0] * r[0];
+    s += bi[3 * l + 1] * r[1];
+    s += bi[3 * l + 2] * r[2];
+    xo[3 * b + l] = x[3 * b + l] + omega * s;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_copy(const T *__restrict__ x, T *__restrict__ y,
+                                                 int64_t n, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ y, T v, int64_t n,
+                                                 const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+// Coarsest level: x = A_c^-1 rhs with the FP64 dense inverse from the
+// Cholesky factorisation, one wavefront per row.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dense_solve(const double *__restrict__ inv, int64_t n,
+                                                        const T *__restrict__ rhs,
+                                                        T *__restrict__ x, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double *a = inv + row * n;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s += a[j] * static_cast<double>(rhs[j]);
+  s = wave_sum(s);
+  if (lane == 0) x[row] = static_cast<T>(s);
+}
+
+// ---------------------------------------------------------------------------
+// PCG kernels (src/CGSolver.cc:208-265), CG type TC, V-cycle type TM.
+// ---------------------------------------------------------------------------
+// Initial residual r = b + (-1) A x (src/CGSolver.cc:134-140) and, for a
+// V-cycle in another precision, its rounded copy.
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_init_r(Sell<TC> A, const TC *__restrict__ b,
+                                                      const TC *__restrict__ x, TC *__restrict__ r,
+                                                      TM *__restrict__ rm) {
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= A.rows) return;
+  const TC v = b[row] - sell_row_dot(A, row, x);
+  r[row] = v;
+  if (rm) rm[row] = static_cast<TM>(v);
+}
+
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_init_r_b3(Bsr3<TC> A, const TC *__restrict__ b,
+                                                         const TC *__restrict__ x,
+                                                         TC *__restrict__ r, TM *__restrict__ rm) {
+  const int64_t bb = xcd_block() * kBlock + threadIdx.x;
+  if (bb >= A.nb) return;
+  TC v[3];
+  bsr3_row_dot(A, bb, x, v, false);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    const TC t = b[3 * bb + l] - v[l];
+    r[3 * bb + l] = t;
+    if (rm) rm[3 * bb + l] = static_cast<TM>(t);
+  }
+}
+
+// Initial scalars: bb = b.b, rho = r.z, rr = r.r, rel = rr / bb, p = z
+// (src/CGSolver.cc:148-176). z may be in the V-cycle type.
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_init_dots(const TC *__restrict__ b,
+                                                         const TC *__restrict__ r,
+                                                         const TM *__restrict__ z,
+                                                         TC *__restrict__ p, int64_t n,
+                                                         Ctl<TC> *ctl, TC *partials) {
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  TC sbb = TC(0), srz = TC(0), srr = TC(0);
+  if (i < n) {
+    const TC zi = static_cast<TC>(z[i]);
+    const TC ri = r[i];
+    sbb = b[i] * b[i];
+    srz = ri * zi;
+    srr = ri * ri;
+    p[i] = zi;
+  }
+  sbb = block_sum(sbb);
+  srz = block_sum(srz);
+  srr = block_sum(srr);
+  const uint32_t G = gridDim.x;
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = sbb;
+    partials[G + blockIdx.x] = srz;
+    partials[2 * G + blockIdx.x] = srr;
+    __threadfence();
+    last = atomicAdd(&ctl->ticket[0], 1u) == G - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  TC a = TC(0), c = TC(0), d = TC(0);
+  for (uint32_t k = threadIdx.x; k < G; k += kBlock) {
+    a += partials[k];
+    c += partials[G + k];
+    d += partials[2 * G + k];
+  }
+  a = block_sum(a);
+  c = block_sum(c);
+  d = block_sum(d);
+  if (threadIdx.x == 0) {
+    ctl->ticket[0] = 0u;
+    ctl->bb = a;
+    ctl->rho = c;
+    ctl->rr = d;
+    const TC rel = d / a;
+    ctl->rel = rel;
+    if (ctl->hist) ctl->hist[0] = rel;
+    ctl->iter = 0;
+    ctl->iters_out = ctl->maxit;
+    ctl->breakdown = (c <= TC(0)) ? 2 : 0;
+    if (!finite_val(rel)) {
+      ctl->stop = 3;
+    } else if (rel < ctl->tol2) {
+      ctl->stop = 1;
+      ctl->iters_out = 0;
+    } else {
+      ctl->stop = 0;
+    }
+  }
+}
+
+// q = A p and p.q; the last block forms alpha = rho / pAp
+// (src/CGSolver.cc:210-215).
+template <class TC>
+__device__ __forceinline__ void cg_pap_epilogue(TC v, Ctl<TC> *ctl, TC *partials) {
+  v = block_sum(v);
+  if (!publish_partial(v, partials, &ctl->ticket[1])) return;
+  const TC pap = final_sum(partials, gridDim.x, &ctl->ticket[1]);
+  if (threadIdx.x == 0) {
+    ctl->pap = pap;
+    ctl->alpha = ctl->rho / pap;
+    if (!(pap > TC(0)) && ctl->breakdown == 0) ctl->breakdown = 1;
+  }
+}
+
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(Sell<TC> A, const TC *__restrict__ p,
+                                                        TC *__restrict__ q, Ctl<TC> *ctl,
+                                                        TC *partials) {
+  if (ctl->stop) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  TC v = TC(0);
+  if (row < A.rows) {
+    const TC qi = sell_row_dot(A, row, p);
+    q[row] = qi;
+    v = p[row] * qi;
+  }
+  cg_pap_epilogue(v, ctl, partials);
+}
+
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3(Bsr3<TC> A, const TC *__restrict__ p,
+                                                           TC *__restrict__ q, Ctl<TC> *ctl,
+                                                           TC *partials) {
+  if (ctl->stop) return;
+  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  TC v = TC(0);
+  if (b < A.nb) {
+    TC y[3];
+    bsr3_row_dot(A, b, p, y, false);
+    q[3 * b] = y[0];
+    q[3 * b + 1] = y[1];
+    q[3 * b + 2] = y[2];
+    v = p[3 * b] * y[0];
+    v += p[3 * b + 1] * y[1];
+    v += p[3 * b + 2] * y[2];
+  }
+  cg_pap_epilogue(v, ctl, partials);
+}
+
+// x += alpha p; r = r + (-alpha) q; rr = r.r; rel = rr / bb; stop test
+// (src/CGSolver.cc:218-230, 252-268). Writes the V-cycle-type copy of r.
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(TC *__restrict__ x, TC *__restrict__ r,
+                                                         TM *__restrict__ rm,
+                                                         const TC *__restrict__ p,
+                                                         const TC *__restrict__ q, int64_t n,
+                                                         Ctl<TC> *ctl, TC *partials) {
+  if (ctl->stop) return;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  const TC alpha = ctl->alpha;
+  const TC na = -alpha;
+  TC v = TC(0);
+  if (i < n) {
+    x[i] = x[i] + alpha * p[i];
+    const TC ri = r[i] + na * q[i];
+    r[i] = ri;
+    if (rm) rm[i] = static_cast<TM>(ri);
+    v = ri * ri;
+  }
+  v = block_sum(v);
+  if (!publish_partial(v, partials, &ctl->ticket[2])) return;
+  const TC rr = final_sum(partials, gridDim.x, &ctl->ticket[2]);
+  if (threadIdx.x == 0) {
+    const TC rel = rr / ctl->bb;
+    ctl->rr = rr;
+    ctl->rel = rel;
+    const int64_t it = ctl->iter;
+    if (ctl->hist) ctl->hist[it + 1] = rel;
+    ctl->iter = it + 1;
+    if (!finite_val(rel)) {
+      ctl->stop = 3;
+    } else if (rel < ctl->tol2) {
+      ctl->stop = 1;
+      ctl->iters_out = it;
+    } else if (it + 1 >= ctl->maxit) {
+      ctl->stop = 2;
+      ctl->iters_out = ctl->maxit;
+    }
+  }
+}
+
+// rho_new = r.z, beta = rho_new / rho (src/CGSolver.cc:236-239).
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_rz(const TC *__restrict__ r,
+                                                  const TM *__restrict__ z, int64_t n,
+                                                  Ctl<TC> *ctl, TC *partials) {
+  if (ctl->stop) return;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  TC v = TC(0);
+  if (i < n) v = r[i] * static_cast<TC>(z[i]);
+  v = block_sum(v);
+  if (!publish_partial(v, partials, &ctl->ticket[3])) return;
+  const TC rz = final_sum(partials, gridDim.x, &ctl->ticket[3]);
+  if (threadIdx.x == 0) {
+    ctl->rho_new = rz;
+    ctl->beta = rz / ctl->rho;
+    ctl->rho = rz;
+    if (!(rz > TC(0)) && ctl->breakdown == 0) ctl->breakdown = 2;
+  }
+}
+
+// p = beta p + 1 * z (src/CGSolver.cc:241-245).
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(TC *__restrict__ p,
+                                                        const TM *__restrict__ z, int64_t n,
+                                                        const Ctl<TC> *ctl) {
+  if (ctl->stop) return;
+  const int64_t i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const TC beta = ctl->beta;
+  const TC pi = p[i] * beta;
+  p[i] = pi + TC(1) * static_cast<TC>(z[i]);
+}
+
+// True residual ||b - A x||^2 and ||b||^2 in FP64 (verification only).
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_true_residual(Sell<TC> A, const TC *__restrict__ b,
+                                                          const TC *__restrict__ x,
+                                                          double *out2, double *partials,
+                                                          uint32_t *ticket) {
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (row < A.rows) {
+    const int64_t s = row >> 6;
+    const int64_t base = A.slice_ptr[s] + (row & 63);
+    const int w = A.slice_w[s];
+    double sum = 0.0;
+    for (int j = 0; j < w; ++j) {
+      const int64_t e = base + static_cast<int64_t>(j) * kSlice;
+      sum += static_cast<double>(A.val[e]) * static_cast<double>(x[A.col[e]]);
+    }
+    const double bi = static_cast<double>(b[row]);
+    const double d = bi - sum;
+    rr = d * d;
+    bb = bi * bi;
+  }
+  rr = block_sum(rr);
+  bb = block_sum(bb);
+  const uint32_t G = gridDim.x;
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = rr;
+    partials[G + blockIdx.x] = bb;
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == G - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double a = 0.0, c = 0.0;
+  for (uint32_t k = threadIdx.x; k < G; k += kBlock) {
+    a += partials[k];
+    c += partials[G + k];
+  }
+  a = block_sum(a);
+  c = block_sum(c);
+  if (threadIdx.x == 0) {
+    *ticket = 0u;
+    out2[0] = a;
+    out2[1] = c;
+  }
+}
+
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_true_residual_b3(Bsr3<TC> A, const TC *__restrict__ b,
+                                                             const TC *__restrict__ x,
+                                                             double *out2, double *partials,
+                                                             uint32_t *ticket) {
+  const int64_t blk = xcd_block() * kBlock + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (blk < A.nb) {
+    const int64_t s = blk >> 6;
+    const int64_t lane = blk & 63;
+    const int w = A.slice_w[s];
+    double y[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < w; ++j) {
+      const int64_t slot = A.slice_ptr[s] + static_cast<int64_t>(j) * kSlice;
+      const int64_t bc = A.bcol[slot + lane];
+      const TC *v = A.val + slot * 9 + lane;
+      for (int l = 0; l < 3; ++l)
+        for (int c = 0; c < 3; ++c)
+          y[l] += static_cast<double>(v[(3 * l + c) * kSlice]) * static_cast<double>(x[3 * bc + c]);
+    }
+    for (int l = 0; l < 3; ++l) {
+      const double bi = static_cast<double>(b[3 * blk + l]);
+      const double d = bi - y[l];
+      rr += d * d;
+      bb += bi * bi;
+    }
+  }
+  rr = block_sum(rr);
+  bb = block_sum(bb);
+  const uint32_t G = gridDim.x;
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = rr;
+    partials[G + blockIdx.x] = bb;
+    __threadfence();
+    last = atomicAdd(ticket, 1u) == G - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double a = 0.0, c = 0.0;
+  for (uint32_t k = threadIdx.x; k < G; k += kBlock) {
+    a += partials[k];
+    c += partials[G + k];
+  }
+  a = block_sum(a);
+  c = block_sum(c);
+  if (threadIdx.x == 0) {
+    *ticket = 0u;
+    out2[0] = a;
+    out2[1] = c;
+  }
+}
+
+inline uint32_t grid_for(int64_t n) {
+  return static_cast<uint32_t>((n + kBlock - 1) / kBlock);
+}
+
+}  // namespace gsk

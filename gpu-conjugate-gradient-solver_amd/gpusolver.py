@@ -1,0 +1,463 @@
+"""ctypes binding of libgpusolver.so (C ABI in include/gpusolver.h).
+
+Thin host-side plumbing for the test suite and the benchmark: CSR arrays
+in numpy, the problem generators, the AMG hierarchy and the GPU solver.
+The library is loaded from this directory; importing fails loudly when it
+has not been built (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libgpusolver.so"
+
+PREC_DOUBLE, PREC_MIXED, PREC_SINGLE = 0, 1, 2
+DAMPED_JACOBI, SGS, BLOCK_JACOBI, BLOCK_GS = 0, 1, 2, 3
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i64 = C.c_int64
+_i32 = C.c_int32
+
+
+class GpuSolverError(RuntimeError):
+    pass
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python {HERE / 'build.py'}` "
+            "(the solver has no CPU fallback)")
+    return C.CDLL(str(LIB_PATH))
+
+
+lib = _load()
+
+_SIGS = {
+    "gs_last_error": (C.c_char_p, []),
+    "gs_version": (C.c_int, []),
+    "gs_csr_create": (C.c_int, [_i64, _i64, _i64, _i32p, _i32p, _f64p, C.POINTER(_vp)]),
+    "gs_csr_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "gs_csr_export": (C.c_int, [_vp, _i32p, _i32p, _f64p]),
+    "gs_csr_destroy": (C.c_int, [_vp]),
+    "gs_csr_permute_keep_order": (C.c_int, [_vp, _i32p, C.POINTER(_vp)]),
+    "gs_csr_permute_sorted": (C.c_int, [_vp, _i32p, _i32p, C.POINTER(_vp)]),
+    "gs_ell_pack": (C.c_int, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64)]),
+    "gs_graph_coloring": (C.c_int, [_i64, _i32p, _i32p, _i32p, _i32p, _i32p, C.POINTER(_i32)]),
+    "gs_rcm": (C.c_int, [_i64, _i32p, _i32p, _i32p, _i32p]),
+    "gs_permutation_check": (C.c_int, [_i64, _i32p, _i32p]),
+    "gs_block_graph": (C.c_int, [_vp, _i32, _vp, _vp, C.POINTER(_i64)]),
+    "gs_expand_block_permutation": (C.c_int, [_i64, _i32, _i32p, _i32p, _i32p]),
+    "gs_inverse_diag_blocks3": (C.c_int, [_vp, _f64p]),
+    "gs_problem_poisson2d": (C.c_int, [_i32, _i32, C.POINTER(_vp)]),
+    "gs_problem_poisson3d": (C.c_int, [_i32, _i32, _i32, C.POINTER(_vp)]),
+    "gs_problem_varcoef3d": (C.c_int, [_i32, C.c_uint64, C.c_double, C.c_double, C.POINTER(_vp)]),
+    "gs_problem_elasticity": (C.c_int, [_i32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_uint64, C.POINTER(_vp), _vp, _vp]),
+    "gs_uniform_pm1": (C.c_int, [C.c_uint64, _i64, _f64p]),
+    "gs_gaussian_field": (C.c_int, [_i32, _i32, _i32, C.c_uint64, C.c_double, _f64p]),
+    "gs_hierarchy_build_sa": (C.c_int, [_vp, _i32, C.c_double, _i32, _i64, _vp, _i32,
+                                        C.POINTER(_vp)]),
+    "gs_hierarchy_create": (C.c_int, [C.POINTER(_vp)]),
+    "gs_hierarchy_push_level": (C.c_int, [_vp, _vp]),
+    "gs_hierarchy_push_transfer": (C.c_int, [_vp, _vp, _vp]),
+    "gs_hierarchy_levels": (C.c_int, [_vp, C.POINTER(_i32)]),
+    "gs_hierarchy_get": (C.c_int, [_vp, _i32, _i32, C.POINTER(_vp)]),
+    "gs_hierarchy_aggregates": (C.c_int, [_vp, _i32, _i32p]),
+    "gs_hierarchy_destroy": (C.c_int, [_vp]),
+    "gs_solver_create": (C.c_int, [_i32, C.POINTER(_vp)]),
+    "gs_solver_set_int": (C.c_int, [_vp, C.c_char_p, _i64]),
+    "gs_solver_set_real": (C.c_int, [_vp, C.c_char_p, C.c_double]),
+    "gs_solver_set_level_weights": (C.c_int, [_vp, _i32, _vp, _vp]),
+    "gs_solver_setup": (C.c_int, [_vp, _vp, _vp]),
+    "gs_solver_solve": (C.c_int, [_vp, _f64p, _f64p, C.c_double, _i64, C.POINTER(_i64),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]),
+    "gs_solver_result": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double)]),
+    "gs_solver_history": (C.c_int, [_vp, _f64p, _i64, C.POINTER(_i64)]),
+    "gs_solver_vcycle": (C.c_int, [_vp, _f64p, _f64p]),
+    "gs_solver_spmv": (C.c_int, [_vp, _f64p, _f64p]),
+    "gs_solver_time_kernel": (C.c_int, [_vp, _i32, _i32, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]),
+    "gs_solver_describe": (C.c_int, [_vp, C.c_char_p, _i64]),
+    "gs_solver_destroy": (C.c_int, [_vp]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise GpuSolverError(lib.gs_last_error().decode())
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGS)
+
+
+# ---------------------------------------------------------------------------
+# CSR
+# ---------------------------------------------------------------------------
+@dataclass
+class Csr:
+    """CSR matrix with int32 indices and FP64 values (host)."""
+    ptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    shape: tuple
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.size)
+
+    @property
+    def rows(self) -> int:
+        return int(self.shape[0])
+
+    @classmethod
+    def from_scipy(cls, m) -> "Csr":
+        m = m.tocsr()
+        return cls(np.ascontiguousarray(m.indptr, np.int32), np.ascontiguousarray(m.indices, np.int32),
+                   np.ascontiguousarray(m.data, np.float64), tuple(m.shape))
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.ptr), shape=self.shape)
+
+    def handle(self) -> "CsrHandle":
+        h = _vp()
+        check(lib.gs_csr_create(self.shape[0], self.shape[1], self.nnz,
+                                np.ascontiguousarray(self.ptr, np.int32),
+                                np.ascontiguousarray(self.col, np.int32),
+                                np.ascontiguousarray(self.val, np.float64), C.byref(h)))
+        return CsrHandle(h)
+
+
+class CsrHandle:
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gs_csr_destroy(self.h)
+            self.h = None
+
+    def to_csr(self) -> Csr:
+        r, c, z = _i64(), _i64(), _i64()
+        check(lib.gs_csr_info(self.h, C.byref(r), C.byref(c), C.byref(z)))
+        ptr = np.empty(r.value + 1, np.int32)
+        col = np.empty(max(z.value, 1), np.int32)
+        val = np.empty(max(z.value, 1), np.float64)
+        check(lib.gs_csr_export(self.h, ptr, col, val))
+        return Csr(ptr, col[:z.value].copy(), val[:z.value].copy(), (r.value, c.value))
+
+
+def _from_handle_fn(fn, *args) -> Csr:
+    h = _vp()
+    check(fn(*args, C.byref(h)))
+    return CsrHandle(h).to_csr()
+
+
+def poisson2d(nx: int, ny: int) -> Csr:
+    return _from_handle_fn(lib.gs_problem_poisson2d, nx, ny)
+
+
+def poisson3d(nx: int, ny: int | None = None, nz: int | None = None) -> Csr:
+    return _from_handle_fn(lib.gs_problem_poisson3d, nx, ny or nx, nz or nx)
+
+
+def varcoef3d(n: int, seed: int, sigma: float = 8.0, scale: float = 2.0) -> Csr:
+    return _from_handle_fn(lib.gs_problem_varcoef3d, n, seed, sigma, scale)
+
+
+def elasticity(nv: int, young: float = 1.0, poisson: float = 0.3, hetero_sigma: float = 0.0,
+               hetero_scale: float = 1.5, seed: int = 0):
+    """Returns (A, rhs, nullspace[n x 6])."""
+    h = _vp()
+    # Size the outputs: free DOFs = 3 * nv^2 * (nv - 1).
+    n = 3 * nv * nv * (nv - 1)
+    rhs = np.empty(n, np.float64)
+    ns = np.empty(n * 6, np.float64)
+    check(lib.gs_problem_elasticity(nv, young, poisson, hetero_sigma, hetero_scale, seed,
+                                    C.byref(h), rhs.ctypes.data_as(_vp), ns.ctypes.data_as(_vp)))
+    A = CsrHandle(h).to_csr()
+    assert A.rows == n
+    return A, rhs, ns.reshape(n, 6)
+
+
+def uniform_pm1(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    check(lib.gs_uniform_pm1(seed, n, out))
+    return out
+
+
+def gaussian_field(nx: int, ny: int, nz: int, seed: int, sigma: float) -> np.ndarray:
+    out = np.empty(nx * ny * nz, np.float64)
+    check(lib.gs_gaussian_field(nx, ny, nz, seed, sigma, out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Setup artifacts
+# ---------------------------------------------------------------------------
+def graph_coloring(A: Csr):
+    n = A.rows
+    colors = np.empty(n, np.int32)
+    verts = np.empty(n, np.int32)
+    offs = np.empty(n + 1, np.int32)
+    nc = _i32()
+    check(lib.gs_graph_coloring(n, A.ptr, A.col, colors, verts, offs, C.byref(nc)))
+    return colors, verts, offs[:nc.value + 1].copy(), nc.value
+
+
+def rcm(A: Csr):
+    n = A.rows
+    o2n = np.empty(n, np.int32)
+    n2o = np.empty(n, np.int32)
+    check(lib.gs_rcm(n, A.ptr, A.col, o2n, n2o))
+    return o2n, n2o
+
+
+def permutation_check(o2n, n2o) -> None:
+    check(lib.gs_permutation_check(len(o2n), np.ascontiguousarray(o2n, np.int32),
+                                   np.ascontiguousarray(n2o, np.int32)))
+
+
+def permute_keep_order(A: Csr, o2n) -> Csr:
+    h = A.handle()
+    return _from_handle_fn(lib.gs_csr_permute_keep_order, h.h, np.ascontiguousarray(o2n, np.int32))
+
+
+def permute_sorted(A: Csr, row_o2n, col_o2n) -> Csr:
+    h = A.handle()
+    return _from_handle_fn(lib.gs_csr_permute_sorted, h.h, np.ascontiguousarray(row_o2n, np.int32),
+                           np.ascontiguousarray(col_o2n, np.int32))
+
+
+def ell_pack(A: Csr, width: int):
+    h = A.handle()
+    ov = _i64()
+    check(lib.gs_ell_pack(h.h, width, None, None, None, None, None, C.byref(ov)))
+    n = A.rows
+    ec = np.empty(n * width, np.int32)
+    ev = np.empty(n * width, np.float64)
+    op = np.empty(n + 1, np.int32)
+    oc = np.empty(max(ov.value, 1), np.int32)
+    ovv = np.empty(max(ov.value, 1), np.float64)
+    check(lib.gs_ell_pack(h.h, width, ec.ctypes.data_as(_vp), ev.ctypes.data_as(_vp),
+                          op.ctypes.data_as(_vp), oc.ctypes.data_as(_vp), ovv.ctypes.data_as(_vp),
+                          C.byref(ov)))
+    return ec, ev, op, oc[:ov.value].copy(), ovv[:ov.value].copy()
+
+
+def block_graph(A: Csr, bs: int = 3):
+    h = A.handle()
+    nnz = _i64()
+    check(lib.gs_block_graph(h.h, bs, None, None, C.byref(nnz)))
+    gp = np.empty(A.rows // bs + 1, np.int32)
+    gc = np.empty(max(nnz.value, 1), np.int32)
+    check(lib.gs_block_graph(h.h, bs, gp.ctypes.data_as(_vp), gc.ctypes.data_as(_vp), C.byref(nnz)))
+    return gp, gc[:nnz.value].copy()
+
+
+def expand_block_permutation(node_o2n, bs: int = 3):
+    nodes = len(node_o2n)
+    o2n = np.empty(nodes * bs, np.int32)
+    n2o = np.empty(nodes * bs, np.int32)
+    check(lib.gs_expand_block_permutation(nodes, bs, np.ascontiguousarray(node_o2n, np.int32),
+                                          o2n, n2o))
+    return o2n, n2o
+
+
+def inverse_diag_blocks3(A: Csr) -> np.ndarray:
+    h = A.handle()
+    out = np.empty(9 * (A.rows // 3), np.float64)
+    check(lib.gs_inverse_diag_blocks3(h.h, out))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Hierarchy
+# ---------------------------------------------------------------------------
+class Hierarchy:
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gs_hierarchy_destroy(self.h)
+            self.h = None
+
+    @classmethod
+    def smoothed_aggregation(cls, A: Csr, block_size: int = 1, eps_strong: float = 0.1,
+                             max_levels: int = 10, min_dofs: int = 1000,
+                             nullspace: np.ndarray | None = None) -> "Hierarchy":
+        ah = A.handle()
+        h = _vp()
+        ncols = 0
+        nsp = None
+        if nullspace is not None:
+            ns = np.ascontiguousarray(nullspace, np.float64)
+            ncols = ns.shape[1]
+            nsp = ns.ctypes.data_as(_vp)
+            keep = ns  # noqa: F841  (keeps the buffer alive during the call)
+        check(lib.gs_hierarchy_build_sa(ah.h, block_size, eps_strong, max_levels, min_dofs, nsp,
+                                        ncols, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_levels(cls, A: list, P: list, R: list) -> "Hierarchy":
+        h = _vp()
+        check(lib.gs_hierarchy_create(C.byref(h)))
+        obj = cls(h)
+        for a in A:
+            ah = a.handle()
+            check(lib.gs_hierarchy_push_level(h, ah.h))
+        for p, r in zip(P, R):
+            ph, rh = p.handle(), r.handle()
+            check(lib.gs_hierarchy_push_transfer(h, ph.h, rh.h))
+        return obj
+
+    @property
+    def levels(self) -> int:
+        n = _i32()
+        check(lib.gs_hierarchy_levels(self.h, C.byref(n)))
+        return n.value
+
+    def get(self, which: str, k: int) -> Csr:
+        w = {"A": 0, "P": 1, "R": 2}[which]
+        return _from_handle_fn(lib.gs_hierarchy_get, self.h, w, k)
+
+    def aggregates(self, k: int) -> np.ndarray:
+        n = self.get("A", k).rows
+        out = np.empty(n, np.int32)
+        check(lib.gs_hierarchy_aggregates(self.h, k, out))
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Solver
+# ---------------------------------------------------------------------------
+@dataclass
+class SolveInfo:
+    iterations: int
+    relres2: float
+    true_relres: float
+    solve_ms: float
+    converged: bool
+    loop_iterations: int
+    breakdown: int
+    wall_ms: float
+
+
+class Solver:
+    """CGSolver + preconditioner on the GPU (gs_solver_* in include/gpusolver.h)."""
+
+    def __init__(self, precision: int = PREC_DOUBLE, **opts):
+        h = _vp()
+        check(lib.gs_solver_create(precision, C.byref(h)))
+        self.h = h
+        self.precision = precision
+        self.n = 0
+        for k, v in opts.items():
+            self.set(k, v)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gs_solver_destroy(self.h)
+            self.h = None
+
+    def set(self, key: str, value) -> None:
+        if key == "omega":
+            check(lib.gs_solver_set_real(self.h, key.encode(), float(value)))
+        else:
+            check(lib.gs_solver_set_int(self.h, key.encode(), int(value)))
+
+    def set_level_weights(self, omega=None, sweeps=None) -> None:
+        nl = len(omega) if omega is not None else len(sweeps)
+        om = np.ascontiguousarray(omega, np.float64) if omega is not None else None
+        sw = np.ascontiguousarray(sweeps, np.int32) if sweeps is not None else None
+        check(lib.gs_solver_set_level_weights(self.h, nl,
+                                              om.ctypes.data_as(_vp) if om is not None else None,
+                                              sw.ctypes.data_as(_vp) if sw is not None else None))
+
+    def setup(self, hier: Hierarchy, perms: list | None = None) -> None:
+        arr = None
+        keep = []
+        if perms is not None:
+            arr = (_vp * len(perms))()
+            for i, p in enumerate(perms):
+                if p is None:
+                    arr[i] = None
+                else:
+                    a = np.ascontiguousarray(p, np.int32)
+                    keep.append(a)
+                    arr[i] = a.ctypes.data_as(_vp)
+        check(lib.gs_solver_setup(self.h, hier.h, C.cast(arr, _vp) if arr is not None else None))
+        self.n = hier.get("A", 0).rows
+
+    def solve(self, b: np.ndarray, x0: np.ndarray | None = None, tol: float = 1e-8,
+              maxit: int = 1000):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True)
+        it = _i64()
+        rr, tr, ms = C.c_double(), C.c_double(), C.c_double()
+        check(lib.gs_solver_solve(self.h, b, x, tol, maxit, C.byref(it), C.byref(rr), C.byref(tr),
+                                  C.byref(ms)))
+        res = {}
+        for k in ("converged", "loop_iterations", "breakdown", "wall_ms"):
+            v = C.c_double()
+            check(lib.gs_solver_result(self.h, k.encode(), C.byref(v)))
+            res[k] = v.value
+        return x, SolveInfo(it.value, rr.value, tr.value, ms.value, bool(res["converged"]),
+                            int(res["loop_iterations"]), int(res["breakdown"]), res["wall_ms"])
+
+    def history(self) -> np.ndarray:
+        n = _i64()
+        check(lib.gs_solver_history(self.h, np.empty(1, np.float64), 0, C.byref(n)))
+        out = np.empty(max(n.value, 1), np.float64)
+        check(lib.gs_solver_history(self.h, out, n.value, C.byref(n)))
+        return out[:n.value]
+
+    def vcycle(self, r: np.ndarray) -> np.ndarray:
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty_like(r)
+        check(lib.gs_solver_vcycle(self.h, r, z))
+        return z
+
+    def spmv(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        check(lib.gs_solver_spmv(self.h, x, y))
+        return y
+
+    def time_kernel(self, which: int, reps: int = 20):
+        ms, by = C.c_double(), C.c_double()
+        check(lib.gs_solver_time_kernel(self.h, which, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def describe(self) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        check(lib.gs_solver_describe(self.h, buf, len(buf)))
+        return json.loads(buf.value.decode())
+
+
+def gpu_available() -> bool:
+    try:
+        import torch  # noqa: F401  (device query only)
+        return bool(torch.cuda.is_available())
+    except Exception:
+        return os.path.exists("/dev/kfd")

@@ -1,0 +1,182 @@
+/*
+ * gpusolver: C ABI of the MI355X (gfx950) AMG-PCG solve phase.
+ *
+ * Drop-in boundary for the solve path of the reference library
+ * (include/GPUSolver in the reference tree). The reference's own binding
+ * surface is its C++ API: CGSolver (include/GPUSolver/CGSolver.h:19-86),
+ * the Preconditioner / Smoother plugins (include/GPUSolver/Preconditioner.h:
+ * 13-20, include/GPUSolver/Smoother.h:12-22), the Backend plugin
+ * (include/GPUSolver/Backend.h:15-159) and the setup artifacts it consumes
+ * (HostCSRMatrix, HostELLMatrix, Permutation, GraphColoring, AMGHierarchy).
+ * Every entry point below replaces one of those, as cited on each
+ * declaration. Plain pointers and sizes only; no C++ or torch types.
+ *
+ * Conventions
+ *   - Every function returns GS_OK (0) or GS_ERR (-1). On GS_ERR,
+ *     gs_last_error() returns the message of the std::runtime_error the
+ *     reference would have thrown (thread-local, valid until the next call).
+ *   - Matrices are CSR with int32 row pointers / column indices and FP64
+ *     values; FP32 device copies are rounded from them once.
+ *   - Permutations are given as oldToNew arrays (src/Permutation.cc:8-35).
+ *   - Handles are opaque owning pointers; destroy each exactly once.
+ */
+#ifndef GPUSOLVER_H
+#define GPUSOLVER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_OK 0
+#define GS_ERR (-1)
+
+/* Precision modes of a solver: CG vectors / V-cycle. */
+#define GS_PREC_DOUBLE 0 /* FP64 / FP64 (plain)                        */
+#define GS_PREC_MIXED 1  /* FP64 / FP32 (mixed)                        */
+#define GS_PREC_SINGLE 2 /* FP32 / FP32 (the reference's precision)    */
+
+/* Smoother kinds, numbered as MultigridSmoother::SmootherType
+ * (include/GPUSolver/Smoothers/MultigridSmoother.h:18-23). */
+#define GS_SMOOTHER_DAMPED_JACOBI 0
+#define GS_SMOOTHER_SYMMETRIC_GAUSS_SEIDEL 1
+#define GS_SMOOTHER_BLOCK_JACOBI 2
+#define GS_SMOOTHER_BLOCK_GAUSS_SEIDEL 3
+
+const char *gs_last_error(void);
+int gs_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Host CSR matrices. Replaces HostCSRMatrix's raw-pointer constructor */
+/* (include/GPUSolver/HostSparseMatrix.h:34-35, src/HostSparseMatrix.cc */
+/* :9-55): the arrays are copied and validated.                        */
+/* ------------------------------------------------------------------ */
+typedef struct gs_csr_s *gs_csr_h;
+int gs_csr_create(int64_t rows, int64_t cols, int64_t nnz, const int32_t *row_ptr,
+                  const int32_t *col_idx, const double *values, gs_csr_h *out);
+int gs_csr_info(gs_csr_h A, int64_t *rows, int64_t *cols, int64_t *nnz);
+int gs_csr_export(gs_csr_h A, int32_t *row_ptr, int32_t *col_idx, double *values);
+int gs_csr_destroy(gs_csr_h A);
+
+/* HostCSRMatrix::applyPermutation, A' = P A P^T keeping each row's entry
+ * order (src/HostSparseMatrix.cc:118-249). */
+int gs_csr_permute_keep_order(gs_csr_h A, const int32_t *old_to_new, gs_csr_h *out);
+/* permuteMatrix: rows / columns relabelled, rows sorted, duplicates summed
+ * (src/SparseMatrixUtils.cc:8-31). */
+int gs_csr_permute_sorted(gs_csr_h A, const int32_t *row_old_to_new,
+                          const int32_t *col_old_to_new, gs_csr_h *out);
+
+/* HostELLMatrix (src/HostELLMatrix.cc:10-206): slot-major ELL with CSR
+ * overflow. Call once with ell_col == NULL to get *overflow_nnz. */
+int gs_ell_pack(gs_csr_h A, int32_t width, int32_t *ell_col, double *ell_val,
+                int32_t *ovf_row_ptr, int32_t *ovf_col, double *ovf_val, int64_t *overflow_nnz);
+
+/* GraphColoring::compute (src/GraphColoring.cc:9-72). offsets holds
+ * ncolors + 1 entries (size it n + 1). */
+int gs_graph_coloring(int64_t n, const int32_t *row_ptr, const int32_t *col_idx,
+                      int32_t *colors, int32_t *color_vertices, int32_t *color_offsets,
+                      int32_t *ncolors);
+/* RCMReordering::compute (src/ReorderingStrategies/RCMReordering.cc:11-110). */
+int gs_rcm(int64_t n, const int32_t *row_ptr, const int32_t *col_idx, int32_t *old_to_new,
+           int32_t *new_to_old);
+/* Permutation's validation (src/Permutation.cc:8-35). */
+int gs_permutation_check(int64_t n, const int32_t *old_to_new, const int32_t *new_to_old);
+/* buildBlockGraph (src/BlockUtils.cc:126-216). Call with g_ptr == NULL for *g_nnz. */
+int gs_block_graph(gs_csr_h A, int32_t block_size, int32_t *g_ptr, int32_t *g_col, int64_t *g_nnz);
+/* expandBlockPermutation (src/BlockUtils.cc:7-55). */
+int gs_expand_block_permutation(int64_t nodes, int32_t block_size, const int32_t *node_old_to_new,
+                                int32_t *old_to_new, int32_t *new_to_old);
+/* extractInverseDiagonalBlocks3x3 (src/BlockUtils.cc:218-277), 9 per block. */
+int gs_inverse_diag_blocks3(gs_csr_h A, double *inverse_blocks);
+
+/* ------------------------------------------------------------------ */
+/* Problem generators (replace the reference's mesh inputs).           */
+/* ------------------------------------------------------------------ */
+int gs_problem_poisson2d(int32_t nx, int32_t ny, gs_csr_h *out);
+int gs_problem_poisson3d(int32_t nx, int32_t ny, int32_t nz, gs_csr_h *out);
+int gs_problem_varcoef3d(int32_t n, uint64_t seed, double sigma, double scale, gs_csr_h *out);
+/* Kuhn-tet elasticity on nv^3 vertices, x = 0 clamped. rhs (rows) and
+ * nullspace (rows x 6, row-major) may be NULL. */
+int gs_problem_elasticity(int32_t nv, double young, double poisson, double hetero_sigma,
+                          double hetero_scale, uint64_t seed, gs_csr_h *out, double *rhs,
+                          double *nullspace);
+int gs_uniform_pm1(uint64_t seed, int64_t n, double *out);
+int gs_gaussian_field(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double sigma, double *out);
+
+/* ------------------------------------------------------------------ */
+/* AMG hierarchy. Replaces AMGHierarchy{A, P, R}                       */
+/* (include/GPUSolver/AMGHierarchy.h:13-21) and AMGHierarchyBuilder +  */
+/* SmoothedAggregationCoarsener (examples/amgSolver/src/               */
+/* AMGHierarchyBuilder.cc:58-148, SmoothedAggregationCoarsener.cc:     */
+/* 64-108).                                                            */
+/* ------------------------------------------------------------------ */
+typedef struct gs_hier_s *gs_hier_h;
+int gs_hierarchy_build_sa(gs_csr_h A, int32_t block_size, double eps_strong, int32_t max_levels,
+                          int64_t min_dofs, const double *nullspace, int32_t nullspace_cols,
+                          gs_hier_h *out);
+int gs_hierarchy_create(gs_hier_h *out);
+int gs_hierarchy_push_level(gs_hier_h h, gs_csr_h A);
+int gs_hierarchy_push_transfer(gs_hier_h h, gs_csr_h P, gs_csr_h R);
+int gs_hierarchy_levels(gs_hier_h h, int32_t *nlev);
+/* which: 0 = A[k], 1 = P[k], 2 = R[k]; returns a new host CSR copy. */
+int gs_hierarchy_get(gs_hier_h h, int32_t which, int32_t k, gs_csr_h *out);
+int gs_hierarchy_aggregates(gs_hier_h h, int32_t k, int32_t *aggregate_of_dof);
+int gs_hierarchy_destroy(gs_hier_h h);
+
+/* ------------------------------------------------------------------ */
+/* Solver: CGSolver (src/CGSolver.cc:18-308) with one of the reference's */
+/* preconditioners (src/Preconditioners/*.cc) on the GPU.              */
+/* ------------------------------------------------------------------ */
+typedef struct gs_solver_s *gs_solver_h;
+int gs_solver_create(int32_t precision, gs_solver_h *out);
+/* Integer options (set before gs_solver_setup):
+ *   "preconditioner"  0 identity, 1 multigrid (default)
+ *   "smoother"        fine-level GS_SMOOTHER_* (default SGS)
+ *   "coarse_smoother" levels > 0 (default: same as "smoother")
+ *   "pre", "post"     sweeps (default 1, 1)
+ *   "coarse_steps"    coarsest-level sweeps (default 2 = pre + post,
+ *                     src/Preconditioners/MultigridGaussSeidelPreconditioner.cc:21)
+ *   "vcycles"         V-cycles per application (default 1)
+ *   "coarse_direct"   1: FP64 Cholesky solve on the coarsest level
+ *   "graph_iters"     PCG iterations per captured hipGraph (0: no graph)
+ *   "cg_block3"       1: store the CG operator as 3x3 blocks
+ * Real options: "omega" (default 1.0). */
+int gs_solver_set_int(gs_solver_h s, const char *key, int64_t value);
+int gs_solver_set_real(gs_solver_h s, const char *key, double value);
+/* Per-level damping and pre = post sweep counts (learned weights); entries
+ * <= 0 keep the global setting. */
+int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega,
+                                const int32_t *sweeps);
+/* Upload the hierarchy. level_old_to_new[k] permutes level k
+ * (MultigridSmoother ctor, src/Smoothers/MultigridSmoother.cc:22-335); NULL
+ * (or a NULL entry) means identity. The CG operator is A[0] permuted by
+ * level 0's permutation (CGSolver ctor, src/CGSolver.cc:44-70). */
+int gs_solver_setup(gs_solver_h s, gs_hier_h h, const int32_t *const *level_old_to_new);
+/* CGSolver::solve (src/CGSolver.cc:87-306): b and x (initial guess in,
+ * solution out) in the caller's ordering. iters follows the reference's
+ * convention (getNbOfIterations, src/CGSolver.cc:257-268,308). */
+int gs_solver_solve(gs_solver_h s, const double *b, double *x, double tol, int64_t maxit,
+                    int64_t *iters, double *relres2, double *true_relres, double *solve_ms);
+/* Result fields of the last solve: "converged", "loop_iterations",
+ * "breakdown", "wall_ms", "zero_rhs". */
+int gs_solver_result(gs_solver_h s, const char *key, double *value);
+int gs_solver_history(gs_solver_h s, double *relres2, int64_t cap, int64_t *len);
+/* One preconditioner application z = M r (Preconditioner::apply,
+ * include/GPUSolver/Preconditioner.h:13-20), vectors in level-0 ordering. */
+int gs_solver_vcycle(gs_solver_h s, const double *r, double *z);
+/* y = A x with the CG operator (Backend::encodeSpmv, include/GPUSolver/
+ * Backend.h:68-69), vectors in level-0 ordering. */
+int gs_solver_spmv(gs_solver_h s, const double *x, double *y);
+/* Average ms per launch of a hot kernel timed with HIP events on the
+ * solver's stream: 0 CG SpMV, 1 V-cycle, 2 PCG iteration, 3 level-0 SGS /
+ * block-GS sweep. *bytes: algorithmic bytes per launch. */
+int gs_solver_time_kernel(gs_solver_h s, int32_t which, int32_t reps, double *ms, double *bytes);
+int gs_solver_describe(gs_solver_h s, char *buf, int64_t cap);
+int gs_solver_destroy(gs_solver_h s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUSOLVER_H */
