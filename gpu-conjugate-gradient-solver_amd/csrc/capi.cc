@@ -24,8 +24,13 @@ struct gs_solver_s {
   gse::SolveResult last;
 };
 
-namespace {
+namespace gsc {
 thread_local std::string g_error;
+void set_error(const char *msg) { g_error = msg; }
+}  // namespace gsc
+using gsc::g_error;
+
+namespace {
 
 template <class F>
 int guard(F &&f) {

@@ -396,10 +396,16 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
   Hierarchy H;
   H.A.push_back(A);
   std::vector<double> B = nullspace;
+  SAParams lp = prm;
   for (int32_t level = 0; level < prm.max_levels; ++level) {
     const Csr &cur = H.A.back();
     if (cur.rows <= prm.min_dofs) break;
-    CoarsenResult r = coarsen(cur, prm, B, ncols);
+    CoarsenResult r = coarsen(cur, lp, B, ncols);
+    // With a near-nullspace every aggregate becomes one coarse node of
+    // ncols unknowns, so coarser levels aggregate ncols-blocks (amgcl's
+    // smoothed_aggregation switches aggr.block_size to nullspace.cols the
+    // same way).
+    if (ncols > 0) lp.block_size = ncols;
     if (r.Ac.rows == 0 || r.Ac.rows >= cur.rows) break;  // no progress
     H.P.push_back(std::move(r.P));
     H.R.push_back(std::move(r.R));

@@ -93,6 +93,53 @@ _SIGS = {
     "gs_solver_destroy": (C.c_int, [_vp]),
 }
 
+_VP = C.POINTER(_vp)
+_SIGS.update({
+    "gs_backend_create": (C.c_int, [_i32, _VP]),
+    "gs_backend_destroy": (C.c_int, [_vp]),
+    "gs_backend_last_error": (C.c_char_p, []),
+    "gs_backend_create_encoder": (C.c_int, [_vp, _VP]),
+    "gs_backend_submit": (C.c_int, [_vp, _vp]),
+    "gs_backend_submit_and_wait": (C.c_int, [_vp, _vp]),
+    "gs_encoder_destroy": (C.c_int, [_vp]),
+    "gs_backend_create_vector": (C.c_int, [_vp, _i64, _vp, _VP]),
+    "gs_dvec_size": (C.c_int, [_vp, C.POINTER(_i64)]),
+    "gs_dvec_update": (C.c_int, [_vp, _vp, _i64]),
+    "gs_dvec_download": (C.c_int, [_vp, _vp]),
+    "gs_dvec_destroy": (C.c_int, [_vp]),
+    "gs_backend_create_index_vector": (C.c_int, [_vp, _i64, _i32p, _VP]),
+    "gs_divec_download": (C.c_int, [_vp, _i32p]),
+    "gs_divec_destroy": (C.c_int, [_vp]),
+    "gs_backend_create_scalar": (C.c_int, [_vp, C.c_double, _VP]),
+    "gs_dscalar_download": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "gs_dscalar_destroy": (C.c_int, [_vp]),
+    "gs_backend_create_csr": (C.c_int, [_vp, _vp, _VP]),
+    "gs_dcsr_update_values": (C.c_int, [_vp, _f64p, _i64]),
+    "gs_dcsr_destroy": (C.c_int, [_vp]),
+    "gs_backend_create_ell": (C.c_int, [_vp, _vp, _i32, _VP]),
+    "gs_dell_destroy": (C.c_int, [_vp]),
+    "gs_encode_scale": (C.c_int, [_vp, _vp, C.c_double]),
+    "gs_encode_scale_dev": (C.c_int, [_vp, _vp, _vp]),
+    "gs_encode_copy": (C.c_int, [_vp, _vp, _vp]),
+    "gs_encode_scale_vector": (C.c_int, [_vp, _vp, _vp]),
+    "gs_encode_set_zero": (C.c_int, [_vp, _vp]),
+    "gs_encode_axpy": (C.c_int, [_vp, C.c_double, _vp, _vp]),
+    "gs_encode_axpy_dev": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_spmv": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_spmv_ell": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_gauss_seidel_color": (C.c_int, [_vp, _vp, _vp, _i64, _i64, _vp, _vp, C.c_double]),
+    "gs_encode_dot": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_scalar_divide": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_scalar_multiply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_scalar_set": (C.c_int, [_vp, C.c_double, _vp]),
+    "gs_encode_scalar_copy": (C.c_int, [_vp, _vp, _vp]),
+    "gs_encode_scalar_negate": (C.c_int, [_vp, _vp, _vp]),
+    "gs_encode_scalar_sqrt": (C.c_int, [_vp, _vp, _vp]),
+    "gs_encode_apply_block_inverses3x3": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_encode_block_gauss_seidel_color3x3": (C.c_int, [_vp, _vp, _vp, _vp, _i64, _i64, _vp, _vp,
+                                                        C.c_double]),
+})
+
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
     _f.restype = _res
@@ -461,3 +508,234 @@ def gpu_available() -> bool:
         return bool(torch.cuda.is_available())
     except Exception:
         return os.path.exists("/dev/kfd")
+
+
+# ---------------------------------------------------------------------------
+# Backend plugin (include/GPUSolver/Backend.h) on HIP
+# ---------------------------------------------------------------------------
+class _Obj:
+    _destroy = None
+
+    def __init__(self, h, owner=None):
+        self.h = h
+        self.owner = owner  # keeps the backend alive
+
+    def __del__(self):
+        if getattr(self, "h", None) and self._destroy:
+            getattr(lib, self._destroy)(self.h)
+            self.h = None
+
+
+class DeviceVector(_Obj):
+    _destroy = "gs_dvec_destroy"
+
+    def size(self) -> int:
+        n = _i64()
+        check(lib.gs_dvec_size(self.h, C.byref(n)))
+        return n.value
+
+    def update_values(self, values) -> None:
+        a = np.ascontiguousarray(values, self.owner.np_dtype)
+        check(lib.gs_dvec_update(self.h, a.ctypes.data_as(_vp), a.size))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.size(), self.owner.np_dtype)
+        check(lib.gs_dvec_download(self.h, out.ctypes.data_as(_vp)))
+        return out
+
+
+class DeviceIndexVector(_Obj):
+    _destroy = "gs_divec_destroy"
+
+    def __init__(self, h, n, owner=None):
+        super().__init__(h, owner)
+        self.n = n
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, np.int32)
+        check(lib.gs_divec_download(self.h, out))
+        return out
+
+
+class DeviceScalar(_Obj):
+    _destroy = "gs_dscalar_destroy"
+
+    def download(self) -> float:
+        v = C.c_double()
+        check(lib.gs_dscalar_download(self.h, C.byref(v)))
+        return v.value
+
+
+class DeviceCSRMatrix(_Obj):
+    _destroy = "gs_dcsr_destroy"
+
+    def update_values(self, values) -> None:
+        a = np.ascontiguousarray(values, np.float64)
+        check(lib.gs_dcsr_update_values(self.h, a, a.size))
+
+
+class DeviceELLMatrix(_Obj):
+    _destroy = "gs_dell_destroy"
+
+
+class Encoder(_Obj):
+    _destroy = "gs_encoder_destroy"
+
+
+class Backend:
+    """gpuSolver::Backend on HIP: create*, encode*, submit / submitAndWait."""
+
+    def __init__(self, dtype=np.float32):
+        self.np_dtype = np.dtype(dtype)
+        code = 1 if self.np_dtype == np.float64 else 0
+        h = _vp()
+        check(lib.gs_backend_create(code, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gs_backend_destroy(self.h)
+            self.h = None
+
+    # factories (Backend.h:19-41)
+    def create_encoder(self) -> Encoder:
+        h = _vp()
+        check(lib.gs_backend_create_encoder(self.h, C.byref(h)))
+        return Encoder(h, self)
+
+    def submit(self, e: Encoder) -> None:
+        check(lib.gs_backend_submit(self.h, e.h))
+
+    def submit_and_wait(self, e: Encoder) -> None:
+        check(lib.gs_backend_submit_and_wait(self.h, e.h))
+
+    def create_vector(self, n_or_values) -> DeviceVector:
+        h = _vp()
+        if np.isscalar(n_or_values):
+            check(lib.gs_backend_create_vector(self.h, int(n_or_values), None, C.byref(h)))
+        else:
+            a = np.ascontiguousarray(n_or_values, self.np_dtype)
+            check(lib.gs_backend_create_vector(self.h, a.size, a.ctypes.data_as(_vp), C.byref(h)))
+        return DeviceVector(h, self)
+
+    def create_index_vector(self, values) -> DeviceIndexVector:
+        a = np.ascontiguousarray(values, np.int32)
+        h = _vp()
+        check(lib.gs_backend_create_index_vector(self.h, a.size, a, C.byref(h)))
+        return DeviceIndexVector(h, a.size, self)
+
+    def create_scalar(self, value: float = 0.0) -> DeviceScalar:
+        h = _vp()
+        check(lib.gs_backend_create_scalar(self.h, float(value), C.byref(h)))
+        return DeviceScalar(h, self)
+
+    def create_csr_matrix(self, A: Csr) -> DeviceCSRMatrix:
+        ah = A.handle()
+        h = _vp()
+        check(lib.gs_backend_create_csr(self.h, ah.h, C.byref(h)))
+        return DeviceCSRMatrix(h, self)
+
+    def create_ell_matrix(self, A: Csr, width: int) -> DeviceELLMatrix:
+        ah = A.handle()
+        h = _vp()
+        check(lib.gs_backend_create_ell(self.h, ah.h, width, C.byref(h)))
+        return DeviceELLMatrix(h, self)
+
+    # encode* (Backend.h:47-114)
+    def encode_scale(self, e, x, alpha):
+        if isinstance(alpha, DeviceScalar):
+            check(lib.gs_encode_scale_dev(e.h, x.h, alpha.h))
+        else:
+            check(lib.gs_encode_scale(e.h, x.h, float(alpha)))
+
+    def encode_copy(self, e, x, y):
+        check(lib.gs_encode_copy(e.h, x.h, y.h))
+
+    def encode_scale_vector(self, e, s, out):
+        check(lib.gs_encode_scale_vector(e.h, s.h, out.h))
+
+    def encode_set_zero(self, e, x):
+        check(lib.gs_encode_set_zero(e.h, x.h))
+
+    def encode_axpy(self, e, alpha, x, y):
+        if isinstance(alpha, DeviceScalar):
+            check(lib.gs_encode_axpy_dev(e.h, alpha.h, x.h, y.h))
+        else:
+            check(lib.gs_encode_axpy(e.h, float(alpha), x.h, y.h))
+
+    def encode_spmv(self, e, A, x, y):
+        if isinstance(A, DeviceELLMatrix):
+            check(lib.gs_encode_spmv_ell(e.h, A.h, x.h, y.h))
+        else:
+            check(lib.gs_encode_spmv(e.h, A.h, x.h, y.h))
+
+    def encode_gauss_seidel_color(self, e, A, colors, start, count, rhs, x, omega):
+        check(lib.gs_encode_gauss_seidel_color(e.h, A.h, colors.h, start, count, rhs.h, x.h,
+                                               float(omega)))
+
+    def encode_dot(self, e, x, y, out):
+        check(lib.gs_encode_dot(e.h, x.h, y.h, out.h))
+
+    def encode_scalar_divide(self, e, a, b, out):
+        check(lib.gs_encode_scalar_divide(e.h, a.h, b.h, out.h))
+
+    def encode_scalar_multiply(self, e, a, b, out):
+        check(lib.gs_encode_scalar_multiply(e.h, a.h, b.h, out.h))
+
+    def encode_scalar_set(self, e, value, out):
+        check(lib.gs_encode_scalar_set(e.h, float(value), out.h))
+
+    def encode_scalar_copy(self, e, a, out):
+        check(lib.gs_encode_scalar_copy(e.h, a.h, out.h))
+
+    def encode_scalar_negate(self, e, a, out):
+        check(lib.gs_encode_scalar_negate(e.h, a.h, out.h))
+
+    def encode_scalar_sqrt(self, e, a, out):
+        check(lib.gs_encode_scalar_sqrt(e.h, a.h, out.h))
+
+    def encode_apply_block_inverses3x3(self, e, inv, res, corr):
+        check(lib.gs_encode_apply_block_inverses3x3(e.h, inv.h, res.h, corr.h))
+
+    def encode_block_gauss_seidel_color3x3(self, e, A, blocks, dinv, begin, end, rhs, x, omega):
+        check(lib.gs_encode_block_gauss_seidel_color3x3(e.h, A.h, blocks.h, dinv.h, begin, end,
+                                                        rhs.h, x.h, float(omega)))
+
+    # blocking convenience methods (Backend.h:122-155)
+    def _blocking(self, fn, *args):
+        e = self.create_encoder()
+        fn(e, *args)
+        self.submit_and_wait(e)
+
+    def spmv(self, A, x, y):
+        self._blocking(self.encode_spmv, A, x, y)
+
+    def dot(self, x, y, out):
+        self._blocking(self.encode_dot, x, y, out)
+
+    def axpy(self, alpha, x, y):
+        self._blocking(self.encode_axpy, alpha, x, y)
+
+    def scale(self, x, alpha):
+        self._blocking(self.encode_scale, x, alpha)
+
+    def scale_vector_by_vector(self, s, out):
+        self._blocking(self.encode_scale_vector, s, out)
+
+    def scalar_set(self, value, out):
+        self._blocking(self.encode_scalar_set, value, out)
+
+    def scalar_copy(self, a, out):
+        self._blocking(self.encode_scalar_copy, a, out)
+
+    def scalar_divide(self, a, b, out):
+        self._blocking(self.encode_scalar_divide, a, b, out)
+
+    def scalar_multiply(self, a, b, out):
+        self._blocking(self.encode_scalar_multiply, a, b, out)
+
+    def scalar_negate(self, a, out):
+        self._blocking(self.encode_scalar_negate, a, out)
+
+    def scalar_sqrt(self, a, out):
+        self._blocking(self.encode_scalar_sqrt, a, out)

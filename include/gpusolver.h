@@ -175,6 +175,79 @@ int gs_solver_time_kernel(gs_solver_h s, int32_t which, int32_t reps, double *ms
 int gs_solver_describe(gs_solver_h s, char *buf, int64_t cap);
 int gs_solver_destroy(gs_solver_h s);
 
+/* ------------------------------------------------------------------ */
+/* Backend plugin (include/GPUSolver/Backend.h:15-159) on HIP.          */
+/* ------------------------------------------------------------------ */
+#define GS_DTYPE_F32 0 /* the reference's value type                 */
+#define GS_DTYPE_F64 1 /* FP64 extension of the interface            */
+
+typedef struct gs_backend_s *gs_backend_h;
+typedef struct gs_encoder_s *gs_encoder_h;
+typedef struct gs_dvec_s *gs_dvec_h;     /* DeviceVector      (DeviceVector.h:6-20)      */
+typedef struct gs_divec_s *gs_divec_h;   /* DeviceIndexVector (DeviceIndexVector.h:8-26) */
+typedef struct gs_dscalar_s *gs_dscalar_h; /* DeviceScalar    (DeviceScalar.h:18-20)     */
+typedef struct gs_dcsr_s *gs_dcsr_h;     /* DeviceCSRMatrix   (DeviceSparseMatrix.h:5-18) */
+typedef struct gs_dell_s *gs_dell_h;     /* DeviceELLMatrix   (DeviceELLMatrix.h:8-25)   */
+
+int gs_backend_create(int32_t dtype, gs_backend_h *out);
+int gs_backend_destroy(gs_backend_h b);
+const char *gs_backend_last_error(void);
+/* createEncoder / submit / submitAndWait (Backend.h:19-23). */
+int gs_backend_create_encoder(gs_backend_h b, gs_encoder_h *out);
+int gs_backend_submit(gs_backend_h b, gs_encoder_h e);
+int gs_backend_submit_and_wait(gs_backend_h b, gs_encoder_h e);
+int gs_encoder_destroy(gs_encoder_h e);
+/* createVector(size[, values]) (Backend.h:26-28); values in the backend's dtype. */
+int gs_backend_create_vector(gs_backend_h b, int64_t n, const void *values, gs_dvec_h *out);
+int gs_dvec_size(gs_dvec_h v, int64_t *n);
+int gs_dvec_update(gs_dvec_h v, const void *values, int64_t n);
+int gs_dvec_download(gs_dvec_h v, void *out);
+int gs_dvec_destroy(gs_dvec_h v);
+/* createIndexVector(size[, values]) (Backend.h:30-33). */
+int gs_backend_create_index_vector(gs_backend_h b, int64_t n, const int32_t *values,
+                                   gs_divec_h *out);
+int gs_divec_download(gs_divec_h v, int32_t *out);
+int gs_divec_destroy(gs_divec_h v);
+/* createScalar([value]) (Backend.h:35-37). */
+int gs_backend_create_scalar(gs_backend_h b, double value, gs_dscalar_h *out);
+int gs_dscalar_download(gs_dscalar_h s, double *value);
+int gs_dscalar_destroy(gs_dscalar_h s);
+/* createCSRMatrix(const HostCSRMatrix&) (Backend.h:39); updateValues. */
+int gs_backend_create_csr(gs_backend_h b, gs_csr_h host, gs_dcsr_h *out);
+int gs_dcsr_update_values(gs_dcsr_h m, const double *values, int64_t nnz);
+int gs_dcsr_destroy(gs_dcsr_h m);
+/* createELLMatrix(HostELLMatrix(csr, width)) (Backend.h:41). */
+int gs_backend_create_ell(gs_backend_h b, gs_csr_h host, int32_t width, gs_dell_h *out);
+int gs_dell_destroy(gs_dell_h m);
+/* encode* (Backend.h:47-114). */
+int gs_encode_scale(gs_encoder_h e, gs_dvec_h x, double alpha);
+int gs_encode_scale_dev(gs_encoder_h e, gs_dvec_h x, gs_dscalar_h alpha);
+int gs_encode_copy(gs_encoder_h e, gs_dvec_h x, gs_dvec_h y);
+int gs_encode_scale_vector(gs_encoder_h e, gs_dvec_h scale, gs_dvec_h out);
+int gs_encode_set_zero(gs_encoder_h e, gs_dvec_h x);
+int gs_encode_axpy(gs_encoder_h e, double alpha, gs_dvec_h x, gs_dvec_h y);
+int gs_encode_axpy_dev(gs_encoder_h e, gs_dscalar_h alpha, gs_dvec_h x, gs_dvec_h y);
+int gs_encode_spmv(gs_encoder_h e, gs_dcsr_h A, gs_dvec_h x, gs_dvec_h y);
+/* MetalBackend::encodeSpmvELLMetal (include/GPUSolver/metal/MetalBackend.h:229-236). */
+int gs_encode_spmv_ell(gs_encoder_h e, gs_dell_h A, gs_dvec_h x, gs_dvec_h y);
+int gs_encode_gauss_seidel_color(gs_encoder_h e, gs_dcsr_h A, gs_divec_h color_vertices,
+                                 int64_t color_start, int64_t color_count, gs_dvec_h rhs,
+                                 gs_dvec_h x, double omega);
+int gs_encode_dot(gs_encoder_h e, gs_dvec_h x, gs_dvec_h y, gs_dscalar_h out);
+int gs_encode_scalar_divide(gs_encoder_h e, gs_dscalar_h num, gs_dscalar_h den, gs_dscalar_h out);
+int gs_encode_scalar_multiply(gs_encoder_h e, gs_dscalar_h a, gs_dscalar_h b, gs_dscalar_h out);
+int gs_encode_scalar_set(gs_encoder_h e, double value, gs_dscalar_h out);
+int gs_encode_scalar_copy(gs_encoder_h e, gs_dscalar_h in, gs_dscalar_h out);
+int gs_encode_scalar_negate(gs_encoder_h e, gs_dscalar_h in, gs_dscalar_h out);
+int gs_encode_scalar_sqrt(gs_encoder_h e, gs_dscalar_h in, gs_dscalar_h out);
+int gs_encode_apply_block_inverses3x3(gs_encoder_h e, gs_dvec_h inverse_blocks, gs_dvec_h residual,
+                                      gs_dvec_h correction);
+/* (begin, end) as declared in Backend.h:110-114 and implemented by
+ * src/CPU/CPUBackend.cc:364. */
+int gs_encode_block_gauss_seidel_color3x3(gs_encoder_h e, gs_dcsr_h A, gs_divec_h color_blocks,
+                                          gs_dvec_h inverse_blocks, int64_t begin, int64_t end,
+                                          gs_dvec_h rhs, gs_dvec_h x, double omega);
+
 #ifdef __cplusplus
 }
 #endif

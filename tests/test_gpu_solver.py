@@ -1,0 +1,239 @@
+"""GPU parity of the solve engine against the CPU oracle, through the C ABI.
+
+Bars (written into each test):
+  - SpMV and one full V-cycle (every smoother, reference coarse mode): bit
+    for bit equal to the oracle in FP64 and FP32 -- same row orders, no FMA
+    contraction, correctly rounded division on both sides.
+  - PCG: FP64 solution within 1e-6 relative (2-norm) of the FP64 oracle at
+    a 1e-8 relative residual, iteration count within +-2 (reference
+    convention), FP64 true relative residual <= 1.5e-8.
+"""
+import numpy as np
+import pytest
+
+import gpusolver as gs
+import oracle_api as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def hier_view(H, perms=None):
+    L = H.levels
+    A = [H.get("A", k) for k in range(L)]
+    P = [H.get("P", k) for k in range(L - 1)]
+    R = [H.get("R", k) for k in range(L - 1)]
+    hv = orc.HierView([(a.ptr, a.col, a.val) for a in A], [(p.ptr, p.col, p.val) for p in P],
+                      [(r.ptr, r.col, r.val) for r in R], perms)
+    return hv, A
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    """2D 5-point Poisson, 256 x 256 interior (65 536 unknowns)."""
+    A = gs.poisson2d(256, 256)
+    H = gs.Hierarchy.smoothed_aggregation(A, block_size=1, eps_strong=0.1, max_levels=10,
+                                          min_dofs=1000)
+    return A, H
+
+
+@pytest.fixture(scope="module")
+def elast():
+    E, rhs, ns = gs.elasticity(12)
+    H = gs.Hierarchy.smoothed_aggregation(E, block_size=3, eps_strong=0.1, max_levels=10,
+                                          min_dofs=500, nullspace=ns)
+    return E, rhs, H
+
+
+def oracle_flags(prec):
+    return dict(cg_double=prec != gs.PREC_SINGLE, mg_double=prec == gs.PREC_DOUBLE)
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
+def test_spmv_bitexact(cfg1, prec):
+    A, H = cfg1
+    s = gs.Solver(prec)
+    s.setup(H)
+    x = gs.uniform_pm1(7, A.rows)
+    y = s.spmv(x)
+    if prec == gs.PREC_SINGLE:
+        yo = orc.spmv(A.ptr, A.col, A.val.astype(np.float32), x.astype(np.float32))
+    else:
+        yo = orc.spmv(A.ptr, A.col, A.val, x)
+    np.testing.assert_array_equal(y, yo.astype(np.float64))
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED, gs.PREC_SINGLE])
+@pytest.mark.parametrize("kind,omega", [(gs.SGS, 1.0), (gs.SGS, 1.3), (gs.DAMPED_JACOBI, 0.7)])
+def test_vcycle_bitexact_scalar(cfg1, prec, kind, omega):
+    A, H = cfg1
+    s = gs.Solver(prec, smoother=kind, omega=omega, coarse_steps=2)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(11, A.rows)
+    z = s.vcycle(r)
+    zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=kind, omegas=omega,
+                    coarse_steps=2)
+    np.testing.assert_array_equal(z, zo)
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
+@pytest.mark.parametrize("kind", [gs.BLOCK_GS, gs.BLOCK_JACOBI])
+def test_vcycle_bitexact_block(elast, prec, kind):
+    E, rhs, H = elast
+    s = gs.Solver(prec, smoother=kind, omega=0.7, coarse_steps=2)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(5, E.rows)
+    z = s.vcycle(r)
+    zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=kind, omegas=0.7,
+                    coarse_steps=2)
+    np.testing.assert_array_equal(z, zo)
+
+
+def test_vcycle_bitexact_hybrid(elast):
+    """Fine block GS, coarse block Jacobi (MultigridHybridBlockPreconditioner,
+    src/Preconditioners/MultigridHybridBlockPreconditioner.cc:9-61)."""
+    E, rhs, H = elast
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, coarse_smoother=gs.BLOCK_JACOBI,
+                  omega=0.7, coarse_steps=3)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(6, E.rows)
+    kinds = [gs.BLOCK_GS] + [gs.BLOCK_JACOBI] * (H.levels - 1)
+    zo = orc.vcycle(hv, r, kinds=kinds, omegas=0.7, coarse_steps=3)
+    np.testing.assert_array_equal(s.vcycle(r), zo)
+
+
+def _pcg_parity(s, hv, A, b, prec, tol=1e-8, **okw):
+    x, info = s.solve(b, tol=tol, maxit=1000)
+    xo, ito, rro, hist = orc.pcg(hv, b, tol=tol, maxit=1000, **oracle_flags(prec), **okw)
+    assert info.converged
+    assert abs(info.iterations - ito) <= 2, (info.iterations, ito)
+    S = A.to_scipy()
+    true = np.linalg.norm(b - S @ x) / np.linalg.norm(b)
+    return x, xo, info, true
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+def test_pcg_parity_cfg1(cfg1, prec):
+    A, H = cfg1
+    s = gs.Solver(prec, smoother=gs.SGS, coarse_steps=2)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    b = gs.uniform_pm1(1, A.rows)
+    x, xo, info, true = _pcg_parity(s, hv, A, b, prec)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert true <= 1.5e-8 and abs(info.true_relres - true) <= 1e-3 * true + 1e-16
+
+
+def test_pcg_single_precision_matches_reference_semantics(cfg1):
+    """FP32 throughout, the reference's precision, tol 1e-6 (its default)."""
+    A, H = cfg1
+    s = gs.Solver(gs.PREC_SINGLE, smoother=gs.SGS, coarse_steps=2)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    b = gs.uniform_pm1(2, A.rows)
+    x, info = s.solve(b, tol=1e-6, maxit=1000)
+    xo, ito, rro, _ = orc.pcg(hv, b, tol=1e-6, cg_double=False, mg_double=False)
+    assert abs(info.iterations - ito) <= 2
+    assert np.linalg.norm(x - xo) <= 1e-4 * np.linalg.norm(xo)
+
+
+def test_pcg_with_rcm_permutations(cfg1):
+    """Level permutations as the reference drivers pass them (RCM on every
+    level; the CG operator uses level 0's, order kept)."""
+    A, H = cfg1
+    perms = []
+    for k in range(H.levels):
+        Ak = H.get("A", k)
+        perms.append(gs.rcm(Ak)[0])
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS)
+    s.setup(H, perms)
+    hv, _ = hier_view(H, perms)
+    b = gs.uniform_pm1(3, A.rows)
+    x, xo, info, true = _pcg_parity(s, hv, A, b, gs.PREC_DOUBLE)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    # The V-cycle in the permuted level-0 ordering is still bit-exact.
+    r = gs.uniform_pm1(4, A.rows)
+    np.testing.assert_array_equal(s.vcycle(r), orc.vcycle(hv, r))
+
+
+def test_pcg_coarse_direct(cfg1):
+    A, H = cfg1
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_direct=1)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    b = gs.uniform_pm1(5, A.rows)
+    x, xo, info, true = _pcg_parity(s, hv, A, b, gs.PREC_DOUBLE, direct_coarse=True)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert true <= 1.5e-8
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+def test_pcg_parity_elasticity_block_gs(elast, prec):
+    E, rhs, H = elast
+    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    x, xo, info, true = _pcg_parity(s, hv, E, rhs, prec, kinds=gs.BLOCK_GS, omegas=0.7)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert true <= 1.5e-8
+
+
+def test_graph_and_eager_paths_agree(cfg1):
+    """The captured-graph loop and the eager loop run the same kernels."""
+    A, H = cfg1
+    b = gs.uniform_pm1(8, A.rows)
+    outs = []
+    for gi in (0, 1, 8):
+        s = gs.Solver(gs.PREC_DOUBLE, graph_iters=gi)
+        s.setup(H)
+        for _ in range(2):  # replays with eager work in between
+            x, info = s.solve(b, tol=1e-8)
+            outs.append((x, info.iterations))
+    for x, it in outs[1:]:
+        np.testing.assert_array_equal(x, outs[0][0])
+        assert it == outs[0][1]
+
+
+def test_history_and_iteration_convention(cfg1):
+    A, H = cfg1
+    s = gs.Solver(gs.PREC_DOUBLE)
+    s.setup(H)
+    b = gs.uniform_pm1(9, A.rows)
+    x, info = s.solve(b, tol=1e-8)
+    h = s.history()
+    assert len(h) == info.iterations + 2  # initial entry + (iterations + 1) updates
+    assert h[-1] < 1e-16 <= h[-2]
+    x, info3 = s.solve(b, tol=1e-8, maxit=3)
+    assert info3.iterations == 3 and not info3.converged
+
+
+def test_identity_preconditioner_cg_sanity():
+    """tests/metal/mainCGSanityCheck.cc:15-84 on the GPU."""
+    import scipy.sparse as sp
+    T = sp.diags([[-1.0] * 4, [4.0] * 5, [-1.0] * 4], [-1, 0, 1]).tocsr()
+    A = gs.Csr.from_scipy(T)
+    H = gs.Hierarchy.from_levels([A], [], [])
+    xs = np.arange(1, 6, dtype=np.float64)
+    b = T @ xs
+    for prec in (gs.PREC_SINGLE, gs.PREC_DOUBLE):
+        s = gs.Solver(prec, preconditioner=0)
+        s.setup(H)
+        x, info = s.solve(b, tol=1e-6, maxit=100)
+        assert np.linalg.norm(x - xs) < 1e-4
+        assert np.linalg.norm(T @ x - b) < 1e-4
+
+
+def test_zero_rhs_and_nan():
+    import scipy.sparse as sp
+    T = gs.Csr.from_scipy(sp.diags([[2.0] * 10], [0]).tocsr())
+    H = gs.Hierarchy.from_levels([T], [], [])
+    s = gs.Solver(gs.PREC_DOUBLE, preconditioner=0)
+    s.setup(H)
+    x, info = s.solve(np.zeros(10), x0=np.ones(10))
+    np.testing.assert_array_equal(x, np.zeros(10))
+    b = np.ones(10)
+    b[3] = np.nan
+    with pytest.raises(gs.GpuSolverError, match="NaN"):
+        s.solve(b)
