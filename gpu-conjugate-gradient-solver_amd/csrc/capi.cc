@@ -2,6 +2,8 @@
 // (declarations and reference citations in include/gpusolver.h).
 #include "../../include/gpusolver.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -22,6 +24,12 @@ struct gs_solver_s {
   bool coarse_kind_set = false;
   std::unique_ptr<gse::EngineBase> eng;
   gse::SolveResult last;
+  // Learned-weight guard: the plain solver restarts the solve when the
+  // learned one breaks down (p.Ap <= 0 or r.z <= 0) or lags the plain
+  // rate (more than lag * plain iterations).
+  gs_solver_s *fallback = nullptr;
+  double lag = 1.5;
+  bool fell_back = false;
 };
 
 namespace gsc {
@@ -336,6 +344,15 @@ int gs_hierarchy_aggregates(gs_hier_h h, int32_t k, int32_t *agg) {
   });
 }
 
+int gs_hierarchy_features(gs_hier_h h, int32_t block_size, double *features) {
+  return guard([&] {
+    need(h, "hierarchy");
+    need(features, "features");
+    std::vector<double> f = gsx::level_features(h->H, block_size);
+    std::memcpy(features, f.data(), f.size() * sizeof(double));
+  });
+}
+
 int gs_hierarchy_destroy(gs_hier_h h) {
   delete h;
   return GS_OK;
@@ -398,6 +415,17 @@ int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega
       if (omega) s->opt.level_omega[k] = omega[k];
       if (sweeps) s->opt.level_sweeps[k] = sweeps[k];
     }
+    if (s->eng) s->eng->set_level_weights(s->opt.level_omega, s->opt.level_sweeps);
+  });
+}
+
+int gs_solver_set_fallback(gs_solver_h s, gs_solver_h plain, double lag) {
+  return guard([&] {
+    need(s, "solver");
+    if (plain == s) throw std::runtime_error("fallback solver must be a different solver.");
+    if (plain && !(lag >= 1.0)) throw std::runtime_error("lag factor must be >= 1.");
+    s->fallback = plain;
+    s->lag = lag;
   });
 }
 
@@ -424,7 +452,31 @@ int gs_solver_solve(gs_solver_h s, const double *b, double *x, double tol, int64
     need(b, "b");
     need(x, "x");
     if (maxit < 0) throw std::runtime_error("maxIterations must be non-negative.");
-    s->last = s->eng->solve(b, x, tol, maxit);
+    s->fell_back = false;
+    if (s->fallback && s->fallback->eng) {
+      // Budget from the plain solver's last iteration count.
+      int64_t budget = maxit;
+      const int64_t ref = s->fallback->last.loop_iterations;
+      if (ref > 0)
+        budget = std::min<int64_t>(maxit, static_cast<int64_t>(std::ceil(s->lag * ref)));
+      const std::vector<double> x0(x, x + s->eng->size());
+      bool bad = false;
+      try {
+        s->last = s->eng->solve(b, x, tol, budget);
+        bad = !s->last.zero_rhs && (!s->last.converged || s->last.breakdown != 0);
+      } catch (const std::runtime_error &) {
+        bad = true;  // non-finite residual in the learned solve
+      }
+      if (bad) {
+        std::copy(x0.begin(), x0.end(), x);
+        const double first_ms = s->last.solve_ms;
+        s->last = s->fallback->eng->solve(b, x, tol, maxit);
+        s->last.solve_ms += first_ms;
+        s->fell_back = true;
+      }
+    } else {
+      s->last = s->eng->solve(b, x, tol, maxit);
+    }
     if (iters) *iters = s->last.iterations;
     if (relres2) *relres2 = s->last.relres2;
     if (true_relres) *true_relres = s->last.true_relres;
@@ -442,6 +494,7 @@ int gs_solver_result(gs_solver_h s, const char *key, double *value) {
     else if (k == "breakdown") *value = r.breakdown;
     else if (k == "wall_ms") *value = r.wall_ms;
     else if (k == "zero_rhs") *value = r.zero_rhs ? 1.0 : 0.0;
+    else if (k == "fell_back") *value = s->fell_back ? 1.0 : 0.0;
     else throw std::runtime_error("unknown result key: " + k);
   });
 }

@@ -123,6 +123,19 @@ class Engine final : public EngineBase {
   int64_t size() const override { return n_; }
   const std::vector<double> &history() const override { return hist_host_; }
   void level_info(std::vector<LevelInfo> &out) const override;
+  void set_level_weights(const std::vector<double> &omega, const std::vector<int> &sweeps) override {
+    for (int k = 0; k < nlev_ && k < static_cast<int>(L_.size()); ++k) {
+      Level<TM> &lv = L_[k];
+      if (k < static_cast<int>(omega.size()) && omega[k] > 0.0) {
+        if ((lv.kind == kSGS || lv.kind == kBlockGS) && !(omega[k] < 2.0))
+          throw std::runtime_error("Gauss-Seidel smoother: omega must be in (0, 2).");
+        lv.omega = static_cast<TM>(omega[k]);
+      }
+      if (k < static_cast<int>(sweeps.size()) && sweeps[k] > 0) lv.pre = lv.post = sweeps[k];
+    }
+    if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+  }
 
  private:
   void build_level(int k, const gsx::Csr &Ak_level, bool coarsest);

@@ -62,6 +62,10 @@ class EngineBase {
   virtual int64_t size() const = 0;
   virtual const std::vector<double> &history() const = 0;
   virtual void level_info(std::vector<LevelInfo> &out) const = 0;
+  // Per-level damping / pre = post sweeps after setup (entries <= 0 keep
+  // the current value); the captured graph is rebuilt on the next solve.
+  virtual void set_level_weights(const std::vector<double> &omega,
+                                 const std::vector<int> &sweeps) = 0;
 };
 
 std::unique_ptr<EngineBase> make_engine(int precision, const Options &o);

@@ -416,4 +416,35 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
   return H;
 }
 
+std::vector<double> level_features(const Hierarchy &H, int32_t block_size) {
+  std::vector<double> f;
+  for (size_t k = 0; k < H.A.size(); ++k) {
+    const Csr &A = H.A[k];
+    double dd = 0.0, dmax = 0.0, dmin = 1e300;
+    for (int64_t r = 0; r < A.rows; ++r) {
+      double d = 0.0, off = 0.0;
+      for (int32_t e = A.ptr[r]; e < A.ptr[r + 1]; ++e) {
+        if (A.col[e] == r) d += std::abs(A.val[e]);
+        else off += std::abs(A.val[e]);
+      }
+      dd += off > 0.0 ? std::min(10.0, d / off) : 10.0;
+      dmax = std::max(dmax, d);
+      if (d > 0.0) dmin = std::min(dmin, d);
+    }
+    int32_t colors = 0;
+    if (block_size == 3 && A.rows % 3 == 0) {
+      Graph g = block_graph(A, 3);
+      colors = greedy_coloring(A.rows / 3, g.ptr.data(), g.col.data()).ncolors;
+    } else {
+      colors = greedy_coloring(A.rows, A.ptr.data(), A.col.data()).ncolors;
+    }
+    f.push_back(static_cast<double>(A.nnz()) / static_cast<double>(A.rows));
+    f.push_back(dd / static_cast<double>(A.rows));
+    f.push_back(k == 0 ? 1.0 : static_cast<double>(A.rows) / static_cast<double>(H.A[k - 1].rows));
+    f.push_back(dmax > 0.0 && dmin < 1e300 ? std::log10(dmax / dmin) : 0.0);
+    f.push_back(static_cast<double>(colors));
+  }
+  return f;
+}
+
 }  // namespace gsx

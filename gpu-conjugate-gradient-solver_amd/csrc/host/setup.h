@@ -147,6 +147,13 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
                              const std::vector<double> &nullspace,
                              int32_t ncols);
 
+// Per-level features for the learned smoother weights, 5 per level:
+// mean row nnz, mean diagonal-dominance ratio |a_ii| / sum_{j!=i} |a_ij|
+// (capped at 10), coarsening ratio n_k / n_{k-1} (1 on level 0),
+// coefficient contrast log10(max |a_ii| / min |a_ii|), and the colour count
+// of the level's smoother colouring (block graph when block_size is 3).
+std::vector<double> level_features(const Hierarchy &H, int32_t block_size);
+
 // ---------------------------------------------------------------------------
 // Problem generators (counter-based SplitMix64 streams, reproducible in
 // numpy with uint64 arithmetic).

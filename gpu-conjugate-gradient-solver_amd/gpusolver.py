@@ -140,6 +140,21 @@ _SIGS.update({
                                                         C.c_double]),
 })
 
+_SIGS.update({
+    "gs_solver_set_fallback": (C.c_int, [_vp, _vp, C.c_double]),
+    "gs_hierarchy_features": (C.c_int, [_vp, _i32, _f64p]),
+    "gs_mlp_create": (C.c_int, [_i32, _i32, _i32, C.c_uint64, _VP]),
+    "gs_mlp_num_params": (C.c_int, [_vp, C.POINTER(_i64)]),
+    "gs_mlp_get_params": (C.c_int, [_vp, _f64p]),
+    "gs_mlp_set_params": (C.c_int, [_vp, _f64p]),
+    "gs_mlp_forward": (C.c_int, [_vp, _f64p, _i64, _f64p]),
+    "gs_mlp_loss_grad": (C.c_int, [_vp, _f64p, _f64p, _i64, C.POINTER(C.c_double), _f64p]),
+    "gs_mlp_adam_step": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "gs_mlp_train": (C.c_int, [_vp, _f64p, _f64p, _i64, _i64, C.c_double,
+                               C.POINTER(C.c_double)]),
+    "gs_mlp_destroy": (C.c_int, [_vp]),
+})
+
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
     _f.restype = _res
@@ -388,6 +403,12 @@ class Hierarchy:
         w = {"A": 0, "P": 1, "R": 2}[which]
         return _from_handle_fn(lib.gs_hierarchy_get, self.h, w, k)
 
+    def features(self, block_size: int = 1) -> np.ndarray:
+        """Per-level features (levels x 5) for the learned smoother weights."""
+        out = np.empty(5 * self.levels, np.float64)
+        check(lib.gs_hierarchy_features(self.h, block_size, out))
+        return out.reshape(self.levels, 5)
+
     def aggregates(self, k: int) -> np.ndarray:
         n = self.get("A", k).rows
         out = np.empty(n, np.int32)
@@ -471,6 +492,15 @@ class Solver:
             res[k] = v.value
         return x, SolveInfo(it.value, rr.value, tr.value, ms.value, bool(res["converged"]),
                             int(res["loop_iterations"]), int(res["breakdown"]), res["wall_ms"])
+
+    def set_fallback(self, plain: "Solver", lag: float = 1.5) -> None:
+        self._fallback = plain  # keep alive
+        check(lib.gs_solver_set_fallback(self.h, plain.h if plain is not None else None, lag))
+
+    def fell_back(self) -> bool:
+        v = C.c_double()
+        check(lib.gs_solver_result(self.h, b"fell_back", C.byref(v)))
+        return bool(v.value)
 
     def history(self) -> np.ndarray:
         n = _i64()
@@ -739,3 +769,58 @@ class Backend:
 
     def scalar_sqrt(self, a, out):
         self._blocking(self.encode_scalar_sqrt, a, out)
+
+
+class MLP:
+    """Smoother-weight MLP on the GPU (gs_mlp_* in include/gpusolver.h)."""
+
+    def __init__(self, inputs: int = 5, hidden: int = 64, outputs: int = 2, seed: int = 0):
+        h = _vp()
+        check(lib.gs_mlp_create(inputs, hidden, outputs, seed, C.byref(h)))
+        self.h = h
+        self.shape = (inputs, hidden, outputs)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.gs_mlp_destroy(self.h)
+            self.h = None
+
+    @property
+    def num_params(self) -> int:
+        n = _i64()
+        check(lib.gs_mlp_num_params(self.h, C.byref(n)))
+        return n.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.num_params, np.float64)
+        check(lib.gs_mlp_get_params(self.h, out))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, np.float64)
+        assert p.size == self.num_params
+        check(lib.gs_mlp_set_params(self.h, p))
+
+    def forward(self, X) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float64)
+        Y = np.empty((X.shape[0], self.shape[2]), np.float64)
+        check(lib.gs_mlp_forward(self.h, X, X.shape[0], Y))
+        return Y
+
+    def loss_grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        l = C.c_double()
+        g = np.empty(self.num_params, np.float64)
+        check(lib.gs_mlp_loss_grad(self.h, X, T, X.shape[0], C.byref(l), g))
+        return l.value, g
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8) -> None:
+        check(lib.gs_mlp_adam_step(self.h, lr, beta1, beta2, eps))
+
+    def train(self, X, T, epochs: int, lr: float = 3e-3) -> float:
+        X = np.ascontiguousarray(X, np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        l = C.c_double()
+        check(lib.gs_mlp_train(self.h, X, T, X.shape[0], epochs, lr, C.byref(l)))
+        return l.value

@@ -1,0 +1,239 @@
+"""Learned per-level smoother weights for the mixed-precision V-cycle.
+
+A 5 -> 64 -> 64 -> 2 MLP (HIP forward / backward / Adam, gs_mlp_* in
+include/gpusolver.h) maps five per-level features of an AMG hierarchy
+(mean row nnz, diagonal dominance, coarsening ratio, coefficient contrast,
+colour count) to that level's smoother damping w and its pre = post sweep
+count. Labels come from the solver itself: for each seeded training
+problem a coordinate search over (w, sweeps) per level keeps the setting
+with the lowest modelled solve work, iterations x per-iteration bytes, of
+a mixed-precision solve to 1e-8. The preconditioner stays a fixed
+symmetric operator within a solve (pre = post sweeps of a symmetric
+smoother, w in (0, 2)), so CG remains valid; the plain weights stay the
+fallback (gs_solver_set_fallback).
+
+Usage on a GPU box:
+  python learned.py train --kind sgs --seeds 200 --out learned_sgs.json
+  python learned.py evaluate --kind sgs --weights learned_weights.json
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE))
+import gpusolver as gs  # noqa: E402
+
+MODEL_PATH = HERE / "learned_weights.json"
+HIDDEN = 64
+
+# Search grids per smoother kind; the plain weight is the reference's
+# (MG-GS w = 1, examples/multigridPoisson/mainAMGtest.cc:697-698; block GS
+# w = 0.7, examples/elasticity/cantileverBeam.cc:572-573).
+KINDS = {
+    "sgs": dict(smoother=gs.SGS, plain_omega=1.0, omegas=[0.7, 0.85, 1.0, 1.15, 1.3, 1.45, 1.6],
+                sweeps=[1, 2], block=1, coarse_steps=2),
+    "bgs": dict(smoother=gs.BLOCK_GS, plain_omega=0.7,
+                omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2], block=3,
+                coarse_steps=0),
+}
+
+
+def training_problem(kind: str, seed: int, size: int):
+    """Seeded training / evaluation problems, at a reduced size with the
+    correlation length scaled to keep the same physical field."""
+    if kind == "sgs":
+        A = gs.varcoef3d(size, seed=seed, sigma=8.0 * size / 160.0, scale=2.0)
+        b = gs.uniform_pm1(seed + 1_000_000, A.rows)
+        H = gs.Hierarchy.smoothed_aggregation(A, block_size=1, eps_strong=0.1, max_levels=10,
+                                              min_dofs=1000)
+        return A, b, H
+    A, b, _ = gs.elasticity(size, young=1.0, poisson=0.3, hetero_sigma=6.0 * size / 96.0,
+                            hetero_scale=1.5, seed=seed)
+    min_dofs = max(300, int(5000 * (size / 96.0) ** 3))
+    H = gs.Hierarchy.smoothed_aggregation(A, block_size=3, eps_strong=0.1, max_levels=10,
+                                          min_dofs=min_dofs)
+    return A, b, H
+
+
+def normalize(F: np.ndarray) -> np.ndarray:
+    F = np.asarray(F, np.float64)
+    return np.stack([np.log2(np.maximum(F[:, 0], 1.0)) / 6.0,
+                     np.log(np.maximum(F[:, 1], 1e-3)),
+                     F[:, 2],
+                     F[:, 3] / 4.0,
+                     np.log2(np.maximum(F[:, 4], 1.0)) / 4.0], axis=1)
+
+
+def encode_targets(omega, sweeps) -> np.ndarray:
+    return np.stack([(np.asarray(omega) - 1.0) / 0.5, (np.asarray(sweeps) - 1.5) / 0.5], axis=1)
+
+
+def decode(Y: np.ndarray):
+    omega = np.clip(1.0 + 0.5 * Y[:, 0], 0.2, 1.9)
+    sweeps = np.clip(np.rint(1.5 + 0.5 * Y[:, 1]), 1, 3).astype(np.int32)
+    return omega, sweeps
+
+
+def level_nnz(H) -> np.ndarray:
+    return np.array([H.get("A", k).nnz for k in range(H.levels)], np.float64)
+
+
+def work_per_iteration(nnz: np.ndarray, sweeps) -> float:
+    """Modelled bytes-proportional work of one PCG iteration: the CG SpMV
+    plus, per smoothed level, 2 x sweeps forward/backward passes for pre and
+    post smoothing and one residual pass."""
+    w = nnz[0]
+    for k in range(len(nnz) - 1):
+        w += (4.0 * sweeps[k] + 1.0) * nnz[k]
+    return w
+
+
+def make_solver(H, spec, omega, sweeps):
+    s = gs.Solver(gs.PREC_MIXED, smoother=spec["smoother"], coarse_direct=1,
+                  coarse_steps=spec["coarse_steps"], omega=spec["plain_omega"], graph_iters=8,
+                  cg_block3=1 if spec["block"] == 3 else 0)
+    s.setup(H)
+    s.set_level_weights(omega=omega, sweeps=sweeps)
+    return s
+
+
+def sweep_labels(H, b, spec, maxit=400):
+    """Coordinate search over (w, sweeps) per smoothed level."""
+    L = H.levels
+    nnz = level_nnz(H)
+    omega = [spec["plain_omega"]] * L
+    sweeps = [1] * L
+    s = make_solver(H, spec, omega, sweeps)
+
+    def cost(om, sw):
+        s.set_level_weights(omega=om, sweeps=sw)
+        x, info = s.solve(b, tol=1e-8, maxit=maxit)
+        if not info.converged or info.breakdown:
+            return np.inf, info.iterations
+        return (info.iterations + 1) * work_per_iteration(nnz, sw), info.iterations
+
+    best, it0 = cost(omega, sweeps)
+    plain_cost = best
+    for k in range(L - 1):  # the coarsest level is solved directly
+        for om in spec["omegas"]:
+            for sw in spec["sweeps"]:
+                o2, s2 = list(omega), list(sweeps)
+                o2[k], s2[k] = om, sw
+                c, _ = cost(o2, s2)
+                if c < best:
+                    best, omega, sweeps = c, o2, s2
+    return np.array(omega[:L - 1]), np.array(sweeps[:L - 1]), plain_cost, best
+
+
+def load_model(kind: str, path: Path = MODEL_PATH):
+    d = json.loads(Path(path).read_text())
+    m = d[kind]
+    net = gs.MLP(5, m["hidden"], 2, seed=0)
+    net.set_params(np.array(m["params"], np.float64))
+    return net
+
+
+def predict(H, kind: str, path: Path = MODEL_PATH):
+    """Per-level (omega, sweeps) for hierarchy H from the trained model,
+    inference on the GPU."""
+    spec = KINDS[kind]
+    net = load_model(kind, path)
+    F = H.features(spec["block"])
+    omega, sweeps = decode(net.forward(normalize(F)))
+    return omega, sweeps
+
+
+def weights_for(config: str, H):
+    kind = "bgs" if "elasticity" in config else "sgs"
+    if not MODEL_PATH.exists():
+        return None
+    d = json.loads(MODEL_PATH.read_text())
+    if kind not in d:
+        return None
+    return predict(H, kind)
+
+
+def train(kind: str, seeds: int, size: int, epochs: int, out: Path) -> dict:
+    spec = KINDS[kind]
+    X, T, gains = [], [], []
+    t0 = time.time()
+    for seed in range(seeds):
+        A, b, H = training_problem(kind, seed, size)
+        F = H.features(spec["block"])[:H.levels - 1]
+        om, sw, pc, bc = sweep_labels(H, b, spec)
+        X.append(normalize(F))
+        T.append(encode_targets(om, sw))
+        gains.append(pc / bc)
+        if seed % 10 == 0:
+            print(f"[train {kind}] seed {seed}: levels {H.levels} omega {np.round(om, 2).tolist()} "
+                  f"sweeps {sw.tolist()} cost gain {pc / bc:.3f} ({time.time() - t0:.0f}s)",
+                  flush=True)
+    X = np.concatenate(X)
+    T = np.concatenate(T)
+    net = gs.MLP(5, HIDDEN, 2, seed=1234)
+    l0, _ = net.loss_grad(X, T)
+    loss = net.train(X, T, epochs=epochs, lr=3e-3)
+    print(f"[train {kind}] samples {X.shape[0]} loss {l0:.4f} -> {loss:.4f}", flush=True)
+    res = {"hidden": HIDDEN, "params": net.get_params().tolist(), "samples": int(X.shape[0]),
+           "seeds": seeds, "size": size, "epochs": epochs, "loss_initial": l0, "loss_final": loss,
+           "label_cost_gain_mean": float(np.mean(gains)), "features": F.tolist()}
+    doc = json.loads(out.read_text()) if out.exists() else {}
+    doc[kind] = res
+    out.write_text(json.dumps(doc))
+    return res
+
+
+def evaluate(kind: str, seeds, size: int, path: Path) -> dict:
+    """Held-out comparison: plain weights vs learned weights, both mixed."""
+    spec = KINDS[kind]
+    rows = []
+    for seed in seeds:
+        A, b, H = training_problem(kind, seed, size)
+        L = H.levels
+        plain = make_solver(H, spec, [spec["plain_omega"]] * L, [1] * L)
+        om, sw = predict(H, kind, path)
+        learned = make_solver(H, spec, om.tolist(), sw.tolist())
+        nnz = level_nnz(H)
+        res = {}
+        for name, s, swk in (("plain", plain, [1] * L), ("learned", learned, sw.tolist())):
+            s.solve(b, tol=1e-8, maxit=1000)
+            x, info = s.solve(b, tol=1e-8, maxit=1000)
+            res[name] = dict(iterations=info.iterations, ms=info.solve_ms,
+                             work=(info.iterations + 1) * work_per_iteration(nnz, swk),
+                             true_relres=info.true_relres)
+        rows.append(res)
+        print(f"[eval {kind}] seed {seed}: plain it {res['plain']['iterations']} "
+              f"{res['plain']['ms']:.2f} ms | learned it {res['learned']['iterations']} "
+              f"{res['learned']['ms']:.2f} ms", flush=True)
+    agg = {k: float(np.mean([r["plain"][k] / r["learned"][k] for r in rows]))
+           for k in ("iterations", "ms", "work")}
+    return {"per_seed": rows, "mean_ratio_plain_over_learned": agg}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train", "evaluate"])
+    ap.add_argument("--kind", choices=sorted(KINDS), default="sgs")
+    ap.add_argument("--seeds", type=int, default=200)
+    ap.add_argument("--size", type=int, default=0)
+    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--out", default=str(MODEL_PATH))
+    ap.add_argument("--weights", default=str(MODEL_PATH))
+    a = ap.parse_args()
+    size = a.size or (40 if a.kind == "sgs" else 24)
+    if a.cmd == "train":
+        train(a.kind, a.seeds, size, a.epochs, Path(a.out))
+    else:
+        r = evaluate(a.kind, range(200, 220), size, Path(a.weights))
+        print(json.dumps(r["mean_ratio_plain_over_learned"]))
+
+
+if __name__ == "__main__":
+    main()

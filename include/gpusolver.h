@@ -153,13 +153,19 @@ int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega
  * (or a NULL entry) means identity. The CG operator is A[0] permuted by
  * level 0's permutation (CGSolver ctor, src/CGSolver.cc:44-70). */
 int gs_solver_setup(gs_solver_h s, gs_hier_h h, const int32_t *const *level_old_to_new);
+/* Learned-weight guard (no counterpart in the reference): when set, a
+ * solve of s that breaks down (p.Ap <= 0 or r.z <= 0), fails to converge
+ * within lag times the plain solver's last iteration count, or produces a
+ * non-finite residual is restarted from the same initial guess with
+ * `plain`; "fell_back" in gs_solver_result reports it. */
+int gs_solver_set_fallback(gs_solver_h s, gs_solver_h plain, double lag);
 /* CGSolver::solve (src/CGSolver.cc:87-306): b and x (initial guess in,
  * solution out) in the caller's ordering. iters follows the reference's
  * convention (getNbOfIterations, src/CGSolver.cc:257-268,308). */
 int gs_solver_solve(gs_solver_h s, const double *b, double *x, double tol, int64_t maxit,
                     int64_t *iters, double *relres2, double *true_relres, double *solve_ms);
 /* Result fields of the last solve: "converged", "loop_iterations",
- * "breakdown", "wall_ms", "zero_rhs". */
+ * "breakdown", "wall_ms", "zero_rhs", "fell_back". */
 int gs_solver_result(gs_solver_h s, const char *key, double *value);
 int gs_solver_history(gs_solver_h s, double *relres2, int64_t cap, int64_t *len);
 /* One preconditioner application z = M r (Preconditioner::apply,
@@ -247,6 +253,30 @@ int gs_encode_apply_block_inverses3x3(gs_encoder_h e, gs_dvec_h inverse_blocks, 
 int gs_encode_block_gauss_seidel_color3x3(gs_encoder_h e, gs_dcsr_h A, gs_divec_h color_blocks,
                                           gs_dvec_h inverse_blocks, int64_t begin, int64_t end,
                                           gs_dvec_h rhs, gs_dvec_h x, double omega);
+
+/* ------------------------------------------------------------------ */
+/* Learned per-level smoother weights. No counterpart in the reference: */
+/* a small MLP (in -> hidden -> hidden -> out, tanh) whose forward,     */
+/* backward (MSE) and Adam steps run as HIP kernels in FP64.            */
+/* ------------------------------------------------------------------ */
+/* Five features per level (mean row nnz, diagonal dominance, coarsening
+ * ratio, coefficient contrast, colour count); features holds 5 * levels. */
+int gs_hierarchy_features(gs_hier_h h, int32_t block_size, double *features);
+typedef struct gs_mlp_s *gs_mlp_h;
+int gs_mlp_create(int32_t inputs, int32_t hidden, int32_t outputs, uint64_t seed, gs_mlp_h *out);
+int gs_mlp_num_params(gs_mlp_h h, int64_t *n);
+int gs_mlp_get_params(gs_mlp_h h, double *params);
+int gs_mlp_set_params(gs_mlp_h h, const double *params);
+int gs_mlp_forward(gs_mlp_h h, const double *X, int64_t batch, double *Y);
+/* Loss 0.5 / B * sum (y - t)^2 and its gradient with respect to the
+ * parameters (flat layout W1, b1, W2, b2, W3, b3, row-major). */
+int gs_mlp_loss_grad(gs_mlp_h h, const double *X, const double *T, int64_t batch, double *loss,
+                     double *grad);
+/* One Adam update with the gradient of the last gs_mlp_loss_grad. */
+int gs_mlp_adam_step(gs_mlp_h h, double lr, double beta1, double beta2, double eps);
+int gs_mlp_train(gs_mlp_h h, const double *X, const double *T, int64_t batch, int64_t epochs,
+                 double lr, double *final_loss);
+int gs_mlp_destroy(gs_mlp_h h);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,86 @@
+"""The learned-weight MLP on the GPU against its FP64 oracle, the level
+features, and the guarded learned solve. GPU."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gpusolver as gs
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    net = gs.MLP(5, 64, 2, seed=3)
+    p = net.get_params()
+    rng = np.random.default_rng(0)
+    X = rng.normal(0, 1, (37, 5))
+    T = rng.normal(0, 1, (37, 2))
+    Y = net.forward(X)
+    Yo, _, _ = mo.forward(p, X, 5, 64, 2)
+    np.testing.assert_allclose(Y, Yo, rtol=1e-12, atol=1e-13)
+    loss, g = net.loss_grad(X, T)
+    assert abs(loss - mo.loss(p, X, T, 5, 64, 2)) <= 1e-12 * loss
+    go = mo.grad(p, X, T, 5, 64, 2)
+    np.testing.assert_allclose(g, go, rtol=1e-10, atol=1e-14)
+    # and against finite differences on a sample of parameters
+    idx = rng.choice(p.size, 40, replace=False)
+    np.testing.assert_allclose(g[idx], mo.fd_grad(p, X, T, 5, 64, 2, idx), rtol=1e-5, atol=1e-8)
+
+
+def test_adam_steps_match_oracle():
+    net = gs.MLP(5, 16, 2, seed=5)
+    p = net.get_params()
+    rng = np.random.default_rng(1)
+    X = rng.normal(0, 1, (11, 5))
+    T = rng.normal(0, 1, (11, 2))
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t in range(1, 6):
+        _, g = net.loss_grad(X, T)
+        net.adam_step(lr=1e-2)
+        p, m, v = mo.adam(p, mo.grad(p, X, T, 5, 16, 2), m, v, t, lr=1e-2)
+        np.testing.assert_allclose(net.get_params(), p, rtol=1e-10, atol=1e-13)
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (64, 5))
+    T = np.stack([np.sin(X[:, 0]), X[:, 1] * X[:, 2]], 1)
+    net = gs.MLP(5, 64, 2, seed=7)
+    l0, _ = net.loss_grad(X, T)
+    l1 = net.train(X, T, epochs=500, lr=3e-3)
+    assert l1 < 0.2 * l0
+
+
+def test_guarded_solve_falls_back_on_bad_weights():
+    """A deliberately poor learned weight (w = 0.05) lags the plain rate;
+    the plain solver restarts the solve and still reaches 1e-8."""
+    A = gs.poisson2d(128, 128)
+    H = gs.Hierarchy.smoothed_aggregation(A, min_dofs=500)
+    b = gs.uniform_pm1(4, A.rows)
+    plain = gs.Solver(gs.PREC_MIXED, coarse_direct=1)
+    plain.setup(H)
+    xp, ip = plain.solve(b, tol=1e-8)
+    bad = gs.Solver(gs.PREC_MIXED, coarse_direct=1)
+    bad.setup(H)
+    bad.set_level_weights(omega=[0.05] * H.levels)
+    bad.set_fallback(plain, lag=1.5)
+    x, info = bad.solve(b, tol=1e-8)
+    assert bad.fell_back()
+    assert info.converged and info.true_relres <= 1.5e-8
+    np.testing.assert_array_equal(x, xp)
+
+
+def test_learned_weights_gain_over_plain():
+    """Held-out seeds: the trained network's weights need no more modelled
+    work (iterations x per-iteration bytes) than the plain weights."""
+    import learned
+    if not learned.MODEL_PATH.exists():
+        pytest.skip("no trained weights committed yet")
+    doc = json.loads(learned.MODEL_PATH.read_text())
+    for kind in doc:
+        r = learned.evaluate(kind, range(200, 203), doc[kind]["size"], learned.MODEL_PATH)
+        assert r["mean_ratio_plain_over_learned"]["work"] >= 1.0, r
