@@ -161,11 +161,15 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_csr_gs_color(const int32_t *ptr, const int32_t *col,
                                                          const T *val, const int32_t *order,
                                                          int64_t start, int64_t count,
-                                                         const T *rhs, T *x, T omega,
-                                                         int32_t *zero_diag) {
+                                                         int64_t rows, const T *rhs, T *x,
+                                                         T omega, int32_t *zero_diag) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k >= count) return;
   const int32_t row = order[start + k];
+  if (row < 0 || row >= rows) {  // invalid colorVertices entry
+    *zero_diag = 2;
+    return;
+  }
   T diag = T(0), sum = T(0);
   for (int32_t e = ptr[row]; e < ptr[row + 1]; ++e) {
     const int32_t c = col[e];
@@ -187,10 +191,14 @@ __global__ __launch_bounds__(kBlock) void k_csr_gs_color(const int32_t *ptr, con
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_csr_block_gs_color(
     const int32_t *ptr, const int32_t *col, const T *val, const int32_t *order, const T *dinv,
-    int64_t begin, int64_t end, const T *rhs, T *x, T omega) {
+    int64_t begin, int64_t end, int64_t nb, const T *rhs, T *x, T omega, int32_t *bad) {
   const int64_t k = begin + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (k >= end) return;
   const int64_t b = order[k];
+  if (b < 0 || b >= nb) {  // invalid colorBlocks entry
+    *bad = 2;
+    return;
+  }
   const int64_t r0 = 3 * b;
   T lr[3];
   for (int l = 0; l < 3; ++l) {
@@ -343,6 +351,8 @@ int gs_backend_submit_and_wait(gs_backend_h, gs_encoder_h e) {
     if (flag) {
       k_zero<int32_t><<<1, kBlock, 0, e->stream>>>(zero_diag_flag(e), 1);
       GS_HIP(hipStreamSynchronize(e->stream));
+      if (flag == 2)
+        throw std::runtime_error("HipBackend: color vertex index out of range.");
       throw std::runtime_error("HipBackend::encodeGaussSeidelColor: zero diagonal entry.");
     }
   });
@@ -687,7 +697,7 @@ int gs_encode_gauss_seidel_color(gs_encoder_h e, gs_dcsr_h A, gs_divec_h colors,
       using T = decltype(t);
       k_csr_gs_color<T><<<grid_for(count), kBlock, 0, e->stream>>>(
           dp<int32_t>(A->ptr), dp<int32_t>(A->col), dp<T>(A->val), dp<int32_t>(colors->d), start,
-          count, dp<T>(rhs->d), dp<T>(x->d), static_cast<T>(omega), flag);
+          count, A->rows, dp<T>(rhs->d), dp<T>(x->d), static_cast<T>(omega), flag);
     });
   });
 }
@@ -709,6 +719,8 @@ int gs_encode_block_gauss_seidel_color3x3(gs_encoder_h e, gs_dcsr_h A, gs_divec_
           "HipBackend::encodeBlockGaussSeidelColor3x3: color range exceeds colorBlocks size.");
     if (rhs->n != x->n)
       throw std::runtime_error("HipBackend::encodeBlockGaussSeidelColor3x3: rhs/solution size mismatch.");
+    if (A->rows != rhs->n || A->cols != rhs->n)
+      throw std::runtime_error("HipBackend::encodeBlockGaussSeidelColor3x3: matrix/vector size mismatch.");
     if (rhs->n % 3 != 0)
       throw std::runtime_error(
           "HipBackend::encodeBlockGaussSeidelColor3x3: vector size must be divisible by 3.");
@@ -720,7 +732,8 @@ int gs_encode_block_gauss_seidel_color3x3(gs_encoder_h e, gs_dcsr_h A, gs_divec_
       using T = decltype(t);
       k_csr_block_gs_color<T><<<grid_for(end - begin), kBlock, 0, e->stream>>>(
           dp<int32_t>(A->ptr), dp<int32_t>(A->col), dp<T>(A->val), dp<int32_t>(blocks->d),
-          dp<T>(dinv->d), begin, end, dp<T>(rhs->d), dp<T>(x->d), static_cast<T>(omega));
+          dp<T>(dinv->d), begin, end, rhs->n / 3, dp<T>(rhs->d), dp<T>(x->d),
+          static_cast<T>(omega), zero_diag_flag(e));
     });
   });
 }

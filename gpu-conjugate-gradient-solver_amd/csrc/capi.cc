@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <climits>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -163,6 +164,21 @@ int gs_ell_pack(gs_csr_h A, int32_t width, int32_t *ell_col, double *ell_val, in
       std::memcpy(ovf_col, E.ovf_col.data(), E.ovf_col.size() * sizeof(int32_t));
       std::memcpy(ovf_val, E.ovf_val.data(), E.ovf_val.size() * sizeof(double));
     }
+  });
+}
+
+int gs_csr_sell_layout(gs_csr_h A, int64_t *slots, int32_t *min_col, int32_t *max_col) {
+  return guard([&] {
+    need(A, "matrix");
+    const gsx::Sell64 P = gsx::sell64_pack(A->A);
+    if (slots) *slots = P.slots;
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int32_t c : P.col) {
+      lo = std::min(lo, c);
+      hi = std::max(hi, c);
+    }
+    if (min_col) *min_col = P.col.empty() ? 0 : lo;
+    if (max_col) *max_col = P.col.empty() ? 0 : hi;
   });
 }
 

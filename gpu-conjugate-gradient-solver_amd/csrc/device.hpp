@@ -80,7 +80,7 @@ std::vector<T> cast_values(const std::vector<double> &v) {
   return out;
 }
 
-// SELL-64 device matrix. Rows keep their CSR entry order.
+// SELL-64 device matrix (layout and validation in gsx::sell64_pack).
 template <class T>
 struct DSell {
   int64_t rows = 0, cols = 0;
@@ -89,41 +89,15 @@ struct DSell {
   DevBuf slice_ptr, slice_w, col, val;
 
   void build(const gsx::Csr &A) {
-    rows = A.rows;
-    cols = A.cols;
-    nnz = A.nnz();
-    const int64_t ns = (rows + gsk::kSlice - 1) / gsk::kSlice;
-    std::vector<int64_t> sp(ns + 1, 0);
-    std::vector<int32_t> sw(ns, 0);
-    for (int64_t s = 0; s < ns; ++s) {
-      int32_t w = 0;
-      for (int64_t r = s * gsk::kSlice; r < std::min(rows, (s + 1) * gsk::kSlice); ++r)
-        w = std::max(w, A.ptr[r + 1] - A.ptr[r]);
-      sw[s] = w;
-      sp[s + 1] = sp[s] + static_cast<int64_t>(w) * gsk::kSlice;
-    }
-    slots = sp[ns];
-    std::vector<int32_t> c(slots, 0);
-    std::vector<T> v(slots, T(0));
-    for (int64_t s = 0; s < ns; ++s)
-      for (int64_t l = 0; l < gsk::kSlice; ++l) {
-        const int64_t r = s * gsk::kSlice + l;
-        int32_t pad = 0;
-        int32_t len = 0;
-        if (r < rows) {
-          len = A.ptr[r + 1] - A.ptr[r];
-          pad = len > 0 ? A.col[A.ptr[r + 1] - 1] : static_cast<int32_t>(std::min<int64_t>(r, cols - 1));
-          for (int32_t j = 0; j < len; ++j) {
-            c[sp[s] + j * gsk::kSlice + l] = A.col[A.ptr[r] + j];
-            v[sp[s] + j * gsk::kSlice + l] = static_cast<T>(A.val[A.ptr[r] + j]);
-          }
-        }
-        for (int32_t j = len; j < sw[s]; ++j) c[sp[s] + j * gsk::kSlice + l] = pad;
-      }
-    slice_ptr.upload(sp);
-    slice_w.upload(sw);
-    col.upload(c);
-    val.upload(v);
+    const gsx::Sell64 P = gsx::sell64_pack(A);
+    rows = P.rows;
+    cols = P.cols;
+    nnz = P.nnz;
+    slots = P.slots;
+    slice_ptr.upload(P.slice_ptr);
+    slice_w.upload(P.slice_w);
+    col.upload(P.col);
+    val.upload(cast_values<T>(P.val));
   }
   gsk::Sell<T> view() const {
     return gsk::Sell<T>{rows, cols, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(),
@@ -134,7 +108,7 @@ struct DSell {
   }
 };
 
-// BSR3-SELL-64 device matrix built from a host BSR3-ELL.
+// BSR3-SELL-64 device matrix (layout in gsx::bsr3_sell64_pack).
 template <class T>
 struct DBsr3 {
   int64_t nb = 0;
@@ -143,40 +117,14 @@ struct DBsr3 {
   DevBuf slice_ptr, slice_w, bcol, val;
 
   void build(const gsx::Bsr3Ell &B) {
-    nb = B.nb;
-    nblocks = B.nblocks;
-    const int64_t ns = (nb + gsk::kSlice - 1) / gsk::kSlice;
-    std::vector<int64_t> sp(ns + 1, 0);
-    std::vector<int32_t> sw(ns, 0);
-    for (int64_t s = 0; s < ns; ++s) {
-      int32_t w = 0;
-      for (int64_t b = s * gsk::kSlice; b < std::min(nb, (s + 1) * gsk::kSlice); ++b)
-        w = std::max(w, B.nblk[b]);
-      sw[s] = w;
-      sp[s + 1] = sp[s] + static_cast<int64_t>(w) * gsk::kSlice;
-    }
-    slots = sp[ns];
-    std::vector<int32_t> c(slots, 0);
-    std::vector<T> v(slots * 9, T(0));
-    for (int64_t s = 0; s < ns; ++s)
-      for (int64_t l = 0; l < gsk::kSlice; ++l) {
-        const int64_t b = s * gsk::kSlice + l;
-        for (int32_t j = 0; j < sw[s]; ++j) {
-          const int64_t slot = sp[s] + static_cast<int64_t>(j) * gsk::kSlice;
-          if (b < nb && j < B.nblk[b]) {
-            c[slot + l] = B.bcol[static_cast<int64_t>(j) * nb + b];
-            for (int e = 0; e < 9; ++e)
-              v[slot * 9 + e * gsk::kSlice + l] =
-                  static_cast<T>(B.val[(static_cast<int64_t>(j) * 9 + e) * nb + b]);
-          } else {
-            c[slot + l] = static_cast<int32_t>(b < nb ? b : 0);
-          }
-        }
-      }
-    slice_ptr.upload(sp);
-    slice_w.upload(sw);
-    bcol.upload(c);
-    val.upload(v);
+    const gsx::Bsr3Sell64 P = gsx::bsr3_sell64_pack(B);
+    nb = P.nb;
+    nblocks = P.nblocks;
+    slots = P.slots;
+    slice_ptr.upload(P.slice_ptr);
+    slice_w.upload(P.slice_w);
+    bcol.upload(P.bcol);
+    val.upload(cast_values<T>(P.val));
   }
   gsk::Bsr3<T> view() const {
     return gsk::Bsr3<T>{nb, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(), bcol.as<int32_t>(),

@@ -361,57 +361,55 @@ void Engine<TC, TM>::encode_residual(Level<TM> &lv, const int32_t *stop) {
                                                              lv.res.template as<TM>(), stop);
 }
 
+// The colour sequence of `iters` symmetric sweeps (forward 0..C-1, backward
+// C-1..0 each), with back-to-back repeats of one colour merged into a
+// single launch that applies the update twice.
+static std::vector<std::pair<int, int>> sweep_sequence(int ncolors, int iters) {
+  std::vector<int> seq;
+  for (int it = 0; it < iters; ++it) {
+    for (int c = 0; c < ncolors; ++c) seq.push_back(c);
+    for (int c = ncolors - 1; c >= 0; --c) seq.push_back(c);
+  }
+  std::vector<std::pair<int, int>> out;
+  for (int c : seq) {
+    if (!out.empty() && out.back().first == c && out.back().second == 1) out.back().second = 2;
+    else out.emplace_back(c, 1);
+  }
+  return out;
+}
+
 template <class TC, class TM>
 void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop) {
   const int64_t n = lv.n;
   bool x_ready = !zero_x;
-  if (lv.kind == kSGS) {
-    for (int it = 0; it < iters; ++it) {
-      for (int c = 0; c < lv.ncolors; ++c) {
-        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
-        if (!x_ready && c == 0) {
-          gsk::k_gs_color_zero<TM><<<grid_for(n), kBlock, 0, stream_>>>(
-              lv.diag.template as<TM>(), e, n, lv.rhs, lv.x, lv.omega, stop);
-          x_ready = true;
-          continue;
-        }
-        if (e > b)
-          gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-              lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, stop);
-      }
-      for (int c = lv.ncolors - 1; c >= 0; --c) {
-        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
-        if (e > b)
-          gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-              lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, stop);
-      }
-    }
-  } else if (lv.kind == kBlockGS) {
-    const int64_t nb = n / 3;
-    for (int it = 0; it < iters; ++it) {
-      for (int c = 0; c < lv.ncolors; ++c) {
-        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
-        if (!x_ready && c == 0) {
-          gsk::k_block_gs_color_zero<TM><<<grid_for(nb), kBlock, 0, stream_>>>(
-              lv.dinv.template as<TM>(), e, nb, lv.rhs, lv.x, lv.omega, stop);
-          x_ready = true;
-          continue;
-        }
-        if (e <= b) continue;
-        if (lv.A.b3)
-          gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-              lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, stop);
+  if ((lv.kind == kSGS || lv.kind == kBlockGS) && iters > 0) {
+    const bool block = lv.kind == kBlockGS;
+    if (block && !lv.A.b3)
+      throw std::runtime_error("block Gauss-Seidel level is not 3x3-block representable.");
+    const int64_t units = block ? n / 3 : n;
+    for (auto [c, reps] : sweep_sequence(lv.ncolors, iters)) {
+      const int64_t b = lv.coff[c], e = lv.coff[c + 1];
+      if (!x_ready && c == 0) {
+        // First update from x = 0: closed form for colour 0, zero elsewhere.
+        if (block)
+          gsk::k_block_gs_color_zero<TM><<<grid_for(units), kBlock, 0, stream_>>>(
+              lv.dinv.template as<TM>(), e, units, lv.rhs, lv.x, lv.omega, stop);
         else
-          throw std::runtime_error("block Gauss-Seidel level is not 3x3-block representable.");
+          gsk::k_gs_color_zero<TM><<<grid_for(units), kBlock, 0, stream_>>>(
+              lv.diag.template as<TM>(), e, units, lv.rhs, lv.x, lv.omega, stop);
+        x_ready = true;
+        if (reps == 1) continue;
+        reps = 1;  // the second update of the pair reads the fresh values
       }
-      for (int c = lv.ncolors - 1; c >= 0; --c) {
-        const int64_t b = lv.coff[c], e = lv.coff[c + 1];
-        if (e <= b) continue;
+      if (e <= b) continue;
+      if (block)
         gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-            lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, stop);
-      }
+            lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+      else
+        gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
+            lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
     }
-  } else {
+  } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {
     if (!x_ready && iters > 0) {
       launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
       x_ready = true;

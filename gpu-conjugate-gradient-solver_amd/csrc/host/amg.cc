@@ -27,6 +27,13 @@
 namespace gsx {
 namespace {
 
+// No strong connection anywhere on a level: nothing to aggregate. The
+// hierarchy stops there (amgcl's amg builder ends coarsening on its
+// empty_level error the same way).
+struct EmptyLevel : std::runtime_error {
+  EmptyLevel() : std::runtime_error("smoothed aggregation: empty level.") {}
+};
+
 constexpr int32_t kUndefined = -1;
 constexpr int32_t kRemoved = -2;
 
@@ -83,7 +90,7 @@ Aggregates plain_aggregates(const Csr &A, double eps) {
         if (ag.strong[k] && ag.id[cc] == kUndefined) ag.id[cc] = cur;
       }
   }
-  if (count == 0) throw std::runtime_error("smoothed aggregation: empty level.");
+  if (count == 0) throw EmptyLevel();
   std::vector<int32_t> keep(count, 0);
   for (int32_t v : ag.id)
     if (v >= 0) keep[v] = 1;
@@ -267,7 +274,7 @@ CoarsenResult coarsen(const Csr &A, const SAParams &prm,
       }
     }
   }
-  if (point_count == 0) throw std::runtime_error("smoothed aggregation: empty level.");
+  if (point_count == 0) throw EmptyLevel();
 
   CoarsenResult res;
   res.agg.assign(n, -1);
@@ -400,7 +407,12 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
   for (int32_t level = 0; level < prm.max_levels; ++level) {
     const Csr &cur = H.A.back();
     if (cur.rows <= prm.min_dofs) break;
-    CoarsenResult r = coarsen(cur, lp, B, ncols);
+    CoarsenResult r;
+    try {
+      r = coarsen(cur, lp, B, ncols);
+    } catch (const EmptyLevel &) {
+      break;
+    }
     // With a near-nullspace every aggregate becomes one coarse node of
     // ncols unknowns, so coarser levels aggregate ncols-blocks (amgcl's
     // smoothed_aggregation switches aggr.block_size to nullspace.cols the

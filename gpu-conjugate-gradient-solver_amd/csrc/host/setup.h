@@ -92,6 +92,36 @@ struct Bsr3Ell {
 // 3x3 blocks with sorted, block-aligned rows.
 bool bsr3_pack(const Csr &A, Bsr3Ell &out);
 
+// SELL-64: the device layout of every scalar operator. Rows are cut into
+// slices of 64; slice s stores its rows slot-major with width slice_w[s]
+// starting at slice_ptr[s]; entry j of row r sits at
+// slice_ptr[r / 64] + 64 * j + r % 64. Each row keeps its CSR order. Padding
+// slots hold value 0 and a column inside [0, cols) that, for square
+// operators, is never the row itself (a Gauss-Seidel row kernel takes the
+// entry on column == row as the diagonal). sell64_pack validates all of
+// this and throws otherwise.
+struct Sell64 {
+  int64_t rows = 0, cols = 0, nnz = 0, slots = 0;
+  std::vector<int64_t> slice_ptr;  // slices + 1
+  std::vector<int32_t> slice_w;    // slices
+  std::vector<int32_t> col;        // slots
+  std::vector<double> val;         // slots
+};
+Sell64 sell64_pack(const Csr &A);
+
+// BSR3-SELL-64: the device layout of 3x3-block operators. Block rows in
+// slices of 64; block slot j of block row b: bcol at slice_ptr[s] + 64 j +
+// lane, value e (= 3 r + c) at (slice_ptr[s] + 64 j) * 9 + 64 e + lane.
+// Padding blocks are zero with bcol = own block row.
+struct Bsr3Sell64 {
+  int64_t nb = 0, nblocks = 0, slots = 0;
+  std::vector<int64_t> slice_ptr;
+  std::vector<int32_t> slice_w;
+  std::vector<int32_t> bcol;
+  std::vector<double> val;
+};
+Bsr3Sell64 bsr3_sell64_pack(const Bsr3Ell &B);
+
 // ---------------------------------------------------------------------------
 // Colouring and orderings.
 // ---------------------------------------------------------------------------

@@ -367,6 +367,107 @@ bool bsr3_pack(const Csr &Ain, Bsr3Ell &out) {
   return true;
 }
 
+Sell64 sell64_pack(const Csr &A) {
+  constexpr int64_t S = 64;
+  Sell64 out;
+  out.rows = A.rows;
+  out.cols = A.cols;
+  out.nnz = A.nnz();
+  const int64_t ns = (A.rows + S - 1) / S;
+  out.slice_ptr.assign(ns + 1, 0);
+  out.slice_w.assign(ns, 0);
+  for (int64_t s = 0; s < ns; ++s) {
+    int32_t w = 0;
+    for (int64_t r = s * S; r < std::min(A.rows, (s + 1) * S); ++r)
+      w = std::max(w, A.ptr[r + 1] - A.ptr[r]);
+    out.slice_w[s] = w;
+    out.slice_ptr[s + 1] = out.slice_ptr[s] + static_cast<int64_t>(w) * S;
+  }
+  out.slots = out.slice_ptr[ns];
+  out.col.assign(out.slots, 0);
+  out.val.assign(out.slots, 0.0);
+  const bool square = A.rows == A.cols;
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < ns; ++s)
+    for (int64_t l = 0; l < S; ++l) {
+      const int64_t r = s * S + l;
+      const int64_t base = out.slice_ptr[s] + l;
+      int32_t len = 0;
+      int32_t pad = 0;
+      if (r < A.rows) {
+        len = A.ptr[r + 1] - A.ptr[r];
+        for (int32_t j = 0; j < len; ++j) {
+          out.col[base + j * S] = A.col[A.ptr[r] + j];
+          out.val[base + j * S] = A.val[A.ptr[r] + j];
+        }
+        // A column of the row other than the diagonal, else a valid
+        // neighbouring column.
+        pad = -1;
+        for (int32_t j = len - 1; j >= 0 && pad < 0; --j)
+          if (A.col[A.ptr[r] + j] != r) pad = A.col[A.ptr[r] + j];
+        if (pad < 0) {
+          if (A.cols == 1) pad = 0;
+          else if (r + 1 < A.cols) pad = static_cast<int32_t>(r + 1);
+          else if (r < A.cols) pad = static_cast<int32_t>(r - 1);
+          else pad = 0;
+        }
+      }
+      for (int32_t j = len; j < out.slice_w[s]; ++j) out.col[base + j * S] = pad;
+    }
+  // Validation of what the kernels assume.
+  for (int64_t s = 0; s < ns; ++s)
+    for (int64_t l = 0; l < S; ++l) {
+      const int64_t r = s * S + l;
+      if (r >= A.rows) break;
+      const int32_t len = A.ptr[r + 1] - A.ptr[r];
+      for (int32_t j = 0; j < out.slice_w[s]; ++j) {
+        const int32_t c = out.col[out.slice_ptr[s] + l + j * S];
+        if (c < 0 || c >= A.cols)
+          throw std::runtime_error("SELL-64 packing: column index out of range.");
+        if (square && j >= len && c == r && A.cols > 1)
+          throw std::runtime_error("SELL-64 packing: padding on the diagonal.");
+      }
+    }
+  return out;
+}
+
+Bsr3Sell64 bsr3_sell64_pack(const Bsr3Ell &B) {
+  constexpr int64_t S = 64;
+  Bsr3Sell64 out;
+  out.nb = B.nb;
+  out.nblocks = B.nblocks;
+  const int64_t ns = (B.nb + S - 1) / S;
+  out.slice_ptr.assign(ns + 1, 0);
+  out.slice_w.assign(ns, 0);
+  for (int64_t s = 0; s < ns; ++s) {
+    int32_t w = 0;
+    for (int64_t b = s * S; b < std::min(B.nb, (s + 1) * S); ++b) w = std::max(w, B.nblk[b]);
+    out.slice_w[s] = w;
+    out.slice_ptr[s + 1] = out.slice_ptr[s] + static_cast<int64_t>(w) * S;
+  }
+  out.slots = out.slice_ptr[ns];
+  out.bcol.assign(out.slots, 0);
+  out.val.assign(out.slots * 9, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t s = 0; s < ns; ++s)
+    for (int64_t l = 0; l < S; ++l) {
+      const int64_t b = s * S + l;
+      for (int32_t j = 0; j < out.slice_w[s]; ++j) {
+        const int64_t slot = out.slice_ptr[s] + static_cast<int64_t>(j) * S;
+        if (b < B.nb && j < B.nblk[b]) {
+          out.bcol[slot + l] = B.bcol[static_cast<int64_t>(j) * B.nb + b];
+          for (int e = 0; e < 9; ++e)
+            out.val[slot * 9 + e * S + l] = B.val[(static_cast<int64_t>(j) * 9 + e) * B.nb + b];
+        } else {
+          out.bcol[slot + l] = static_cast<int32_t>(b < B.nb ? b : 0);
+        }
+      }
+    }
+  for (int32_t c : out.bcol)
+    if (c < 0 || c >= B.nb) throw std::runtime_error("BSR3-SELL-64 packing: block column out of range.");
+  return out;
+}
+
 static void inverse3x3_spd(const double a[9], double inv[9], bool &ok) {
   // Cholesky a = L L^T, then inv = L^-T L^-1.
   double l00 = a[0];

@@ -291,17 +291,26 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__re
 // One colour of multicolour Gauss-Seidel over contiguous (colour-permuted)
 // rows [begin, end): x_i = (1 - w) x_i + w (b_i - sum_{j != i} a_ij x_j) / a_ii
 // (src/CPU/CPUBackend.cc:238-268).
+//
+// `reps` applies the update 1 or 2 times in a row: a symmetric sweep visits
+// its last colour twice back to back (end of the forward pass, start of the
+// backward pass) with no other colour in between, so the off-diagonal sum
+// and b_i / a_ii are the same both times and one read of the row serves
+// both updates, with the reference's operations in the reference's order.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, int64_t begin, int64_t end,
                                                      const T *__restrict__ rhs, T *x, T omega,
-                                                     const int32_t *stop) {
+                                                     int reps, const int32_t *stop) {
   if (stopped(stop)) return;
   const int64_t row = begin + xcd_block() * kBlock + threadIdx.x;
   if (row >= end) return;
   T diag;
   const T off = sell_row_offdiag(A, row, x, diag);
   const T nv = (rhs[row] - off) / diag;
-  x[row] = (T(1) - omega) * x[row] + omega * nv;
+  T xi = x[row];
+  xi = (T(1) - omega) * xi + omega * nv;
+  if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
+  x[row] = xi;
 }
 
 // First colour of a forward sweep started from x = 0: the off-diagonal sum
@@ -327,11 +336,12 @@ __global__ __launch_bounds__(kBlock) void k_gs_color_zero(const T *__restrict__ 
 // One colour of 3x3 block Gauss-Seidel over contiguous block rows
 // [begin, end) (src/CPU/CPUBackend.cc:350-387): r_l = b_l - sum over
 // off-diagonal blocks, x_b = (1 - w) x_b + w D_b^-1 r.
+// `reps` as for k_gs_color.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_block_gs_color(Bsr3<T> A, const T *__restrict__ dinv,
                                                            int64_t begin, int64_t end,
                                                            const T *__restrict__ rhs, T *x,
-                                                           T omega, const int32_t *stop) {
+                                                           T omega, int reps, const int32_t *stop) {
   if (stopped(stop)) return;
   const int64_t b = begin + xcd_block() * kBlock + threadIdx.x;
   if (b >= end) return;
@@ -347,7 +357,10 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color(Bsr3<T> A, const T *_
     nv += bi[3 * l + 0] * lr[0];
     nv += bi[3 * l + 1] * lr[1];
     nv += bi[3 * l + 2] * lr[2];
-    x[3 * b + l] = (T(1) - omega) * x[3 * b + l] + omega * nv;
+    T xi = x[3 * b + l];
+    xi = (T(1) - omega) * xi + omega * nv;
+    if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
+    x[3 * b + l] = xi;
   }
 }
 

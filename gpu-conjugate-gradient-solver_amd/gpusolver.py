@@ -142,6 +142,7 @@ _SIGS.update({
 
 _SIGS.update({
     "gs_solver_set_fallback": (C.c_int, [_vp, _vp, C.c_double]),
+    "gs_csr_sell_layout": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i32), C.POINTER(_i32)]),
     "gs_hierarchy_features": (C.c_int, [_vp, _i32, _f64p]),
     "gs_mlp_create": (C.c_int, [_i32, _i32, _i32, C.c_uint64, _VP]),
     "gs_mlp_num_params": (C.c_int, [_vp, C.POINTER(_i64)]),
@@ -275,6 +276,15 @@ def gaussian_field(nx: int, ny: int, nz: int, seed: int, sigma: float) -> np.nda
 # ---------------------------------------------------------------------------
 # Setup artifacts
 # ---------------------------------------------------------------------------
+def sell_layout(A: Csr):
+    """(slots, min column, max column) of A packed as SELL-64; raises when
+    the packed layout would break a kernel's assumptions."""
+    h = A.handle()
+    slots, lo, hi = _i64(), _i32(), _i32()
+    check(lib.gs_csr_sell_layout(h.h, C.byref(slots), C.byref(lo), C.byref(hi)))
+    return slots.value, lo.value, hi.value
+
+
 def graph_coloring(A: Csr):
     n = A.rows
     colors = np.empty(n, np.int32)

@@ -72,6 +72,12 @@ int gs_csr_permute_sorted(gs_csr_h A, const int32_t *row_old_to_new,
 int gs_ell_pack(gs_csr_h A, int32_t width, int32_t *ell_col, double *ell_val,
                 int32_t *ovf_row_ptr, int32_t *ovf_col, double *ovf_val, int64_t *overflow_nnz);
 
+/* Device layout of a scalar operator (SELL-64, no counterpart in the
+ * reference): packs A as the solver uploads it, validates it (every stored
+ * column inside [0, cols), no padding slot on the diagonal) and reports the
+ * slot count and the smallest / largest stored column. */
+int gs_csr_sell_layout(gs_csr_h A, int64_t *slots, int32_t *min_col, int32_t *max_col);
+
 /* GraphColoring::compute (src/GraphColoring.cc:9-72). offsets holds
  * ncolors + 1 entries (size it n + 1). */
 int gs_graph_coloring(int64_t n, const int32_t *row_ptr, const int32_t *col_idx,

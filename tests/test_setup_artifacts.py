@@ -230,3 +230,33 @@ def test_inverse_diag_blocks3_not_spd_raises():
         np.diag([1.0, -1.0, 1.0])))
     with pytest.raises(gs.GpuSolverError, match="not SPD"):
         gs.inverse_diag_blocks3(A)
+
+
+def test_sell_layout_in_bounds_for_every_operator():
+    """Every stored column of every device operator (padding included) lies
+    inside [0, cols): the prolongators of a Poisson hierarchy have empty
+    rows (strength-isolated coarse nodes) and rectangular shapes."""
+    A = gs.poisson2d(256, 256)
+    H = gs.Hierarchy.smoothed_aggregation(A, min_dofs=1000)
+    ops = [H.get("A", k) for k in range(H.levels)]
+    ops += [H.get(w, k) for w in ("P", "R") for k in range(H.levels - 1)]
+    assert any((np.diff(H.get("P", k).ptr) == 0).any() for k in range(H.levels - 1))
+    for M in ops:
+        slots, lo, hi = gs.sell_layout(M)
+        assert slots >= M.nnz and 0 <= lo and hi < M.shape[1]
+
+
+def test_sell_layout_diagonal_only_rows():
+    import scipy.sparse as sp
+    D = gs.Csr.from_scipy(sp.csr_matrix(np.diag(np.arange(1.0, 70.0))))
+    slots, lo, hi = gs.sell_layout(D)
+    assert slots >= D.nnz and 0 <= lo and hi < 69
+    # Mixed lengths force padding; a row whose only entry is its diagonal
+    # must be padded with another column.
+    M = sp.lil_matrix((70, 70))
+    for i in range(70):
+        M[i, i] = 2.0
+        if i % 3 == 0 and i + 1 < 70:
+            M[i, i + 1] = -1.0
+    slots, lo, hi = gs.sell_layout(gs.Csr.from_scipy(M.tocsr()))
+    assert slots > M.nnz and hi < 70
