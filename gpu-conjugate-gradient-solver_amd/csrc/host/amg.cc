@@ -418,6 +418,11 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
     // smoothed_aggregation switches aggr.block_size to nullspace.cols the
     // same way).
     if (ncols > 0) lp.block_size = ncols;
+    // amgcl halves the strength threshold after every level
+    // (aggr.eps_strong *= 0.5 in transfer_operators, carried across levels
+    // by its multilevel driver): Galerkin operators spread their couplings
+    // over more neighbours, each weaker than on the level above.
+    lp.eps_strong *= 0.5;
     if (r.Ac.rows == 0 || r.Ac.rows >= cur.rows) break;  // no progress
     H.P.push_back(std::move(r.P));
     H.R.push_back(std::move(r.R));
