@@ -404,6 +404,8 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "coarse_direct") o.coarse_direct = v != 0;
     else if (k == "graph_iters") o.graph_iters = static_cast<int>(v);
     else if (k == "cg_block3") o.cg_block3 = v != 0;
+    else if (k == "reference_vcycle") o.reference_vcycle = v != 0;
+    else if (k == "max_direct") o.max_direct = v;
     else throw std::runtime_error("unknown integer option: " + k);
     if (o.pre < 0 || o.post < 0 || o.coarse_steps < 0 || o.vcycles < 0 || o.graph_iters < 0)
       throw std::runtime_error("option values must be non-negative.");

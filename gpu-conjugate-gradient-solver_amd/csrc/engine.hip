@@ -217,6 +217,10 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   const std::vector<int32_t> qinv = inverse_of(lv.q);
   gsx::Csr Ai = relabel(Ak, lv.q, qinv);
   if (lv.direct) {
+    if (lv.n > opt_.max_direct)
+      throw std::runtime_error("coarsest level has " + std::to_string(lv.n) +
+                               " unknowns, above the dense direct-solve limit of " +
+                               std::to_string(opt_.max_direct) + "; coarsen further.");
     std::vector<double> dense(static_cast<size_t>(lv.n) * lv.n, 0.0);
     for (int64_t r = 0; r < lv.n; ++r)
       for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e) dense[r * lv.n + Ai.col[e]] += Ai.val[e];
@@ -454,7 +458,10 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
   gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(
       lv.R.view(), lv.res.template as<TM>(), cl.rhs, coarse_self_zero ? nullptr : cl.x, stop);
   encode_vcycle(k + 1, stop, true);
-  gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.view(), cl.x, lv.x, stop);
+  if (opt_.reference_vcycle && k > 0)
+    gsk::k_prolong_alias<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.view(), cl.x, lv.x, stop);
+  else
+    gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.view(), cl.x, lv.x, stop);
   encode_smooth(lv, lv.post, false, stop);
 }
 

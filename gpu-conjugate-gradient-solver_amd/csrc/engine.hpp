@@ -26,6 +26,11 @@ struct Options {
   int precond = kPrecMultigrid;
   int graph_iters = 8;
   bool cg_block3 = false;  // store the CG operator as 3x3 blocks when possible
+  // Reproduce the reference's coarse-level correction aliasing
+  // (solution = 2 P e below the finest level); off: x += P e.
+  bool reference_vcycle = false;
+  // Largest coarsest level accepted for the dense FP64 inverse.
+  int64_t max_direct = 8192;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (> 0 applies)
 };

@@ -285,6 +285,21 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__re
   x[row] = x[row] + T(1) * c;
 }
 
+// The reference's coarse-level correction storage: on levels below the
+// finest, MultigridSmoother::vCycle prolongates into the level's own
+// `correction` vector, which is also the `solution` it was called with
+// (src/Smoothers/MultigridSmoother.cc:401-404), and then adds that vector to
+// itself: x = P e, x = x + 1 * x.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong_alias(Sell<T> P, const T *__restrict__ e,
+                                                          T *__restrict__ x, const int32_t *stop) {
+  if (stopped(stop)) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  if (row >= P.rows) return;
+  const T c = sell_row_dot(P, row, e);
+  x[row] = c + T(1) * c;
+}
+
 // ---------------------------------------------------------------------------
 // Smoothers.
 // ---------------------------------------------------------------------------

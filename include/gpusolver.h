@@ -147,6 +147,12 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *   "coarse_direct"   1: FP64 Cholesky solve on the coarsest level
  *   "graph_iters"     PCG iterations per captured hipGraph (0: no graph)
  *   "cg_block3"       1: store the CG operator as 3x3 blocks
+ *   "reference_vcycle" 1: reproduce the reference's coarse-level correction
+ *                     aliasing (src/Smoothers/MultigridSmoother.cc:401-404:
+ *                     below the finest level the prolongated correction
+ *                     overwrites the level's solution and is then added to
+ *                     itself); 0 (default): x += P e
+ *   "max_direct"      largest coarsest level for "coarse_direct" (8192)
  * Real options: "omega" (default 1.0). */
 int gs_solver_set_int(gs_solver_h s, const char *key, int64_t value);
 int gs_solver_set_real(gs_solver_h s, const char *key, double value);

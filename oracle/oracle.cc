@@ -257,6 +257,7 @@ struct Level {
   Coloring col;            // SGS: scalar colouring; block GS: block colouring
   std::vector<T> dinv;     // Jacobi: 1/a_ii; block: 9 per block
   std::vector<T> rhs, res, corr, az, tmp;
+  std::vector<T> jaz, jcorr;  // the Jacobi smoothers' own work vectors
 };
 
 template <class T>
@@ -267,6 +268,13 @@ struct MG {
   bool direct_coarse = false;
   std::vector<double> coarse_inv;  // dense SPD inverse (row-major)
   int vcycles = 1;
+  // Reference storage semantics: on every level below the finest, the
+  // prolongated correction is written into the level's `correction` vector,
+  // which is also the `solution` the level was called with
+  // (src/Smoothers/MultigridSmoother.cc:394-404 with :475-476 of the
+  // caller), so the level ends with solution = P e + P e. Off: the
+  // textbook x += P e with a separate buffer.
+  bool reference_aliasing = true;
 };
 
 template <class T>
@@ -298,11 +306,12 @@ void smooth(Level<T> &lv, const T *rhs, T *x, int iters) {
     }
   } else {
     // src/Smoothers/DampedJacobiSmoother.cc:79-87 and
-    // src/Smoothers/BlockJacobiSmoother.cc:17-87.
+    // src/Smoothers/BlockJacobiSmoother.cc:17-87; the smoothers own their
+    // work vectors (Az_, correction_), separate from the level's.
     for (int it = 0; it < iters; ++it) {
-      spmv(lv.A, x, lv.tmp.data());
-      std::vector<T> &c = lv.corr;
-      for (int64_t i = 0; i < n; ++i) c[i] = rhs[i] + T(-1) * lv.tmp[i];
+      spmv(lv.A, x, lv.jaz.data());
+      std::vector<T> &c = lv.jcorr;
+      for (int64_t i = 0; i < n; ++i) c[i] = rhs[i] + T(-1) * lv.jaz[i];
       if (lv.kind == kDampedJacobi) {
         for (int64_t i = 0; i < n; ++i) c[i] *= lv.dinv[i];
         axpy(lv.omega, c.data(), x, n);
@@ -339,8 +348,14 @@ void vcycle(MG<T> &mg, size_t k, const T *rhs, T *x) {
   spmv(mg.R[k], lv.res.data(), cl.rhs.data());
   std::fill(cl.corr.begin(), cl.corr.end(), T(0));
   vcycle(mg, k + 1, cl.rhs.data(), cl.corr.data());
-  spmv(mg.P[k], cl.corr.data(), lv.corr.data());
-  axpy(T(1), lv.corr.data(), x, n);
+  if (mg.reference_aliasing && k > 0) {
+    // x is lv.corr here: spmv overwrites it, axpy doubles it.
+    spmv(mg.P[k], cl.corr.data(), x);
+    axpy(T(1), x, x, n);
+  } else {
+    spmv(mg.P[k], cl.corr.data(), lv.tmp.data());
+    axpy(T(1), lv.tmp.data(), x, n);
+  }
   smooth(lv, rhs, x, lv.post);
 }
 
@@ -666,9 +681,10 @@ static CsrT<T> csr_cast(const CsrT<double> &A) {
 template <class TM>
 static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const double *omegas,
                      const int32_t *pres, const int32_t *posts, int32_t coarse_steps,
-                     int32_t direct_coarse, int32_t vcycles) {
+                     int32_t direct_coarse, int32_t vcycles, int32_t ref_alias) {
   const int nl = h.nlev;
   mg.L.resize(nl);
+  mg.reference_aliasing = ref_alias != 0;
   mg.coarse_steps = coarse_steps;
   mg.direct_coarse = direct_coarse != 0;
   mg.vcycles = vcycles;
@@ -739,6 +755,8 @@ static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const d
     lv.corr.assign(n, TM(0));
     lv.az.assign(n, TM(0));
     lv.tmp.assign(n, TM(0));
+    lv.jaz.assign(n, TM(0));
+    lv.jcorr.assign(n, TM(0));
     if (k > 0) {
       const int64_t nf = h.n[k - 1];
       CsrT<double> Pd = permute_sorted(nf, n, h.pptr[k - 1], h.pcol[k - 1], h.pval[k - 1],
@@ -761,12 +779,13 @@ static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const d
 template <class TC, class TM>
 static void pcg(const OrcHier &h, int prec, const int32_t *kinds, const double *omegas,
                 const int32_t *pres, const int32_t *posts, int32_t coarse_steps,
-                int32_t direct_coarse, int32_t vcycles, const double *b_in, double *x_io,
-                double tol, int64_t maxit, int64_t *iters, double *relres2,
+                int32_t direct_coarse, int32_t vcycles, int32_t ref_alias, const double *b_in,
+                double *x_io, double tol, int64_t maxit, int64_t *iters, double *relres2,
                 double *history, int64_t *hist_len) {
   const int64_t n = h.n[0];
   MG<TM> mg;
-  if (prec == 1) build_mg(mg, h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles);
+  if (prec == 1)
+    build_mg(mg, h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias);
   // CG matrix: level-0 input with the order-keeping permutation.
   std::vector<int32_t> ident;
   const int32_t *o2n = nullptr, *n2o = nullptr;
@@ -880,18 +899,19 @@ static void pcg(const OrcHier &h, int prec, const int32_t *kinds, const double *
 extern "C" int orc_pcg(const OrcHier *h, int32_t prec, int32_t cg_double, int32_t mg_double,
             const int32_t *kinds, const double *omegas, const int32_t *pres,
             const int32_t *posts, int32_t coarse_steps, int32_t direct_coarse,
-            int32_t vcycles, const double *b, double *x, double tol, int64_t maxit,
-            int64_t *iters, double *relres2, double *history, int64_t *hist_len) {
+            int32_t vcycles, int32_t ref_alias, const double *b, double *x, double tol,
+            int64_t maxit, int64_t *iters, double *relres2, double *history,
+            int64_t *hist_len) {
   ORC_TRY({
     if (cg_double && mg_double)
       pcg<double, double>(*h, prec, kinds, omegas, pres, posts, coarse_steps, direct_coarse,
-                          vcycles, b, x, tol, maxit, iters, relres2, history, hist_len);
+                          vcycles, ref_alias, b, x, tol, maxit, iters, relres2, history, hist_len);
     else if (cg_double && !mg_double)
       pcg<double, float>(*h, prec, kinds, omegas, pres, posts, coarse_steps, direct_coarse,
-                         vcycles, b, x, tol, maxit, iters, relres2, history, hist_len);
+                         vcycles, ref_alias, b, x, tol, maxit, iters, relres2, history, hist_len);
     else if (!cg_double && !mg_double)
       pcg<float, float>(*h, prec, kinds, omegas, pres, posts, coarse_steps, direct_coarse,
-                        vcycles, b, x, tol, maxit, iters, relres2, history, hist_len);
+                        vcycles, ref_alias, b, x, tol, maxit, iters, relres2, history, hist_len);
     else
       throw std::runtime_error("orc_pcg: FP32 CG with an FP64 V-cycle is not a supported mode.");
   });
@@ -902,18 +922,18 @@ extern "C" int orc_pcg(const OrcHier *h, int32_t prec, int32_t cg_double, int32_
 extern "C" int orc_vcycle(const OrcHier *h, int32_t mg_double, const int32_t *kinds,
                const double *omegas, const int32_t *pres, const int32_t *posts,
                int32_t coarse_steps, int32_t direct_coarse, int32_t vcycles,
-               const double *r, double *z) {
+               int32_t ref_alias, const double *r, double *z) {
   ORC_TRY({
     const int64_t n = h->n[0];
     if (mg_double) {
       MG<double> mg;
-      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles);
+      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias);
       std::vector<double> zz(n, 0.0);
       for (int v = 0; v < vcycles; ++v) vcycle(mg, 0, r, zz.data());
       std::memcpy(z, zz.data(), n * sizeof(double));
     } else {
       MG<float> mg;
-      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles);
+      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias);
       std::vector<float> rr(n), zz(n, 0.0f);
       for (int64_t i = 0; i < n; ++i) rr[i] = static_cast<float>(r[i]);
       for (int v = 0; v < vcycles; ++v) vcycle(mg, 0, rr.data(), zz.data());

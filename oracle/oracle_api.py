@@ -64,10 +64,11 @@ def _lib() -> C.CDLL:
                                          _i32p, _f64p, C.POINTER(_i64)]),
         "orc_block_graph": (C.c_int, [_i64, _i32p, _i32p, C.c_int32, _vp, _vp, C.POINTER(_i64)]),
         "orc_pcg": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _i32p, _f64p, _i32p, _i32p,
-                              C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double, _i64,
-                              C.POINTER(_i64), C.POINTER(C.c_double), _f64p, C.POINTER(_i64)]),
+                              C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double,
+                              _i64, C.POINTER(_i64), C.POINTER(C.c_double), _f64p,
+                              C.POINTER(_i64)]),
         "orc_vcycle": (C.c_int, [_vp, C.c_int32, _i32p, _f64p, _i32p, _i32p, C.c_int32, C.c_int32,
-                                 C.c_int32, _f64p, _f64p]),
+                                 C.c_int32, C.c_int32, _f64p, _f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -245,9 +246,11 @@ def _level_params(nlev, kinds, omegas, pres, posts):
 
 def pcg(hier: HierView, b, x0=None, tol=1e-8, maxit=1000, prec=1, cg_double=True,
         mg_double=True, kinds=1, omegas=1.0, pres=1, posts=1, coarse_steps=2,
-        direct_coarse=False, vcycles=1):
+        direct_coarse=False, vcycles=1, reference_vcycle=False):
     """Restated CGSolver::solve with the multigrid (prec=1) or identity (0)
-    preconditioner. Returns x, iterations, relres2, history."""
+    preconditioner. reference_vcycle reproduces the reference's correction
+    aliasing on coarse levels (solution = 2 P e); off, the textbook
+    x += P e. Returns x, iterations, relres2, history."""
     n = len(b)
     k, o, pr, po = _level_params(hier.nlev, kinds, omegas, pres, posts)
     x = np.zeros(n) if x0 is None else np.array(x0, np.float64, copy=True)
@@ -256,18 +259,19 @@ def pcg(hier: HierView, b, x0=None, tol=1e-8, maxit=1000, prec=1, cg_double=True
     hist = np.empty(maxit + 1, np.float64)
     hl = _i64()
     _check(lib.orc_pcg(hier.ptr(), prec, int(cg_double), int(mg_double), k, o, pr, po,
-                       coarse_steps, int(direct_coarse), vcycles,
+                       coarse_steps, int(direct_coarse), vcycles, int(reference_vcycle),
                        np.ascontiguousarray(b, np.float64), x, tol, maxit, C.byref(it),
                        C.byref(rr), hist, C.byref(hl)))
     return x, it.value, rr.value, hist[:hl.value].copy()
 
 
 def vcycle(hier: HierView, r, mg_double=True, kinds=1, omegas=1.0, pres=1, posts=1,
-           coarse_steps=2, direct_coarse=False, vcycles=1):
+           coarse_steps=2, direct_coarse=False, vcycles=1, reference_vcycle=False):
     k, o, pr, po = _level_params(hier.nlev, kinds, omegas, pres, posts)
     z = np.empty(len(r), np.float64)
     _check(lib.orc_vcycle(hier.ptr(), int(mg_double), k, o, pr, po, coarse_steps,
-                          int(direct_coarse), vcycles, np.ascontiguousarray(r, np.float64), z))
+                          int(direct_coarse), vcycles, int(reference_vcycle),
+                          np.ascontiguousarray(r, np.float64), z))
     return z
 
 

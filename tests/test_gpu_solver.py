@@ -62,31 +62,34 @@ def test_spmv_bitexact(cfg1, prec):
     np.testing.assert_array_equal(y, yo.astype(np.float64))
 
 
+@pytest.mark.parametrize("ref", [0, 1])
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED, gs.PREC_SINGLE])
 @pytest.mark.parametrize("kind,omega", [(gs.SGS, 1.0), (gs.SGS, 1.3), (gs.DAMPED_JACOBI, 0.7)])
-def test_vcycle_bitexact_scalar(cfg1, prec, kind, omega):
+def test_vcycle_bitexact_scalar(cfg1, prec, kind, omega, ref):
+    """ref = 1: the reference's coarse-level correction aliasing."""
     A, H = cfg1
-    s = gs.Solver(prec, smoother=kind, omega=omega, coarse_steps=2)
+    s = gs.Solver(prec, smoother=kind, omega=omega, coarse_steps=2, reference_vcycle=ref)
     s.setup(H)
     hv, _ = hier_view(H)
     r = gs.uniform_pm1(11, A.rows)
     z = s.vcycle(r)
     zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=kind, omegas=omega,
-                    coarse_steps=2)
+                    coarse_steps=2, reference_vcycle=bool(ref))
     np.testing.assert_array_equal(z, zo)
 
 
+@pytest.mark.parametrize("ref", [0, 1])
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
 @pytest.mark.parametrize("kind", [gs.BLOCK_GS, gs.BLOCK_JACOBI])
-def test_vcycle_bitexact_block(elast, prec, kind):
+def test_vcycle_bitexact_block(elast, prec, kind, ref):
     E, rhs, H = elast
-    s = gs.Solver(prec, smoother=kind, omega=0.7, coarse_steps=2)
+    s = gs.Solver(prec, smoother=kind, omega=0.7, coarse_steps=2, reference_vcycle=ref)
     s.setup(H)
     hv, _ = hier_view(H)
     r = gs.uniform_pm1(5, E.rows)
     z = s.vcycle(r)
     zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=kind, omegas=0.7,
-                    coarse_steps=2)
+                    coarse_steps=2, reference_vcycle=bool(ref))
     np.testing.assert_array_equal(z, zo)
 
 
@@ -102,6 +105,19 @@ def test_vcycle_bitexact_hybrid(elast):
     kinds = [gs.BLOCK_GS] + [gs.BLOCK_JACOBI] * (H.levels - 1)
     zo = orc.vcycle(hv, r, kinds=kinds, omegas=0.7, coarse_steps=3)
     np.testing.assert_array_equal(s.vcycle(r), zo)
+
+
+def test_pcg_parity_reference_semantics(cfg1):
+    """The reference's own V-cycle (correction aliasing included) in FP64
+    against the oracle's restatement of it."""
+    A, H = cfg1
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_steps=2, reference_vcycle=1)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    b = gs.uniform_pm1(12, A.rows)
+    x, xo, info, true = _pcg_parity(s, hv, A, b, gs.PREC_DOUBLE, reference_vcycle=True)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert true <= 1.5e-8
 
 
 def _pcg_parity(s, hv, A, b, prec, tol=1e-8, **okw):

@@ -229,3 +229,59 @@ def test_sgs_matches_sequential_gs_on_bipartite_grid():
     xr = np.empty(n)
     xr[perm] = x
     np.testing.assert_allclose(z, xr, rtol=1e-12, atol=1e-14)
+
+
+def _hier(A, levels_min_dofs=100, block=1):
+    import gpusolver as gs
+    H = gs.Hierarchy.smoothed_aggregation(A, block_size=block, min_dofs=levels_min_dofs)
+    L = H.levels
+    Al = [H.get("A", k) for k in range(L)]
+    return orc.HierView([(a.ptr, a.col, a.val) for a in Al],
+                        [(p.ptr, p.col, p.val) for p in (H.get("P", k) for k in range(L - 1))],
+                        [(r.ptr, r.col, r.val) for r in (H.get("R", k) for k in range(L - 1))]), L
+
+
+@pytest.mark.parametrize("kind,omega", [(1, 1.0), (1, 1.3), (0, 0.7)])
+def test_standard_vcycle_is_symmetric(kind, omega):
+    """Textbook V-cycle with pre = post symmetric smoothing is a symmetric
+    operator: u^T M v = v^T M u in FP64 (pins sweep order, R = P^T use,
+    x += P e)."""
+    import gpusolver as gs
+    A = gs.poisson2d(64, 64)
+    hv, L = _hier(A)
+    assert L >= 3
+    rng = np.random.default_rng(9)
+    u, v = rng.uniform(-1, 1, A.rows), rng.uniform(-1, 1, A.rows)
+    kw = dict(kinds=kind, omegas=omega, coarse_steps=2)
+    Mu, Mv = orc.vcycle(hv, u, **kw), orc.vcycle(hv, v, **kw)
+    assert abs(u @ Mv - v @ Mu) <= 1e-11 * abs(u @ Mv)
+    assert u @ Mu > 0
+
+
+def test_reference_vcycle_aliasing_only_below_the_finest_level():
+    """The reference prolongates into the coarse level's own `solution`
+    vector (src/Smoothers/MultigridSmoother.cc:401-404): with two levels
+    nothing is aliased and both variants agree bit for bit; with three or
+    more they differ. With symmetric sweeps both stay symmetric operators:
+    the reference variant is 2 (I - S A) C (I - A S) + S with the sweep S
+    symmetric, the standard one S + S - S A S + (I - S A) C (I - A S)."""
+    import gpusolver as gs
+    A = gs.poisson2d(48, 48)
+    H2 = gs.Hierarchy.smoothed_aggregation(A, min_dofs=100, max_levels=1)
+    assert H2.levels == 2
+    hv2 = orc.HierView([(a.ptr, a.col, a.val) for a in (H2.get("A", k) for k in range(2))],
+                       [(H2.get("P", 0).ptr, H2.get("P", 0).col, H2.get("P", 0).val)],
+                       [(H2.get("R", 0).ptr, H2.get("R", 0).col, H2.get("R", 0).val)])
+    r = np.sin(0.3 * np.arange(A.rows))
+    np.testing.assert_array_equal(orc.vcycle(hv2, r, reference_vcycle=True),
+                                  orc.vcycle(hv2, r, reference_vcycle=False))
+    hv, L = _hier(A, 50)
+    assert L >= 3
+    za = orc.vcycle(hv, r, reference_vcycle=True)
+    zb = orc.vcycle(hv, r, reference_vcycle=False)
+    assert np.abs(za - zb).max() > 1e-6
+    rng = np.random.default_rng(1)
+    u, v = rng.uniform(-1, 1, A.rows), rng.uniform(-1, 1, A.rows)
+    Mu = orc.vcycle(hv, u, reference_vcycle=True)
+    Mv = orc.vcycle(hv, v, reference_vcycle=True)
+    assert abs(u @ Mv - v @ Mu) <= 1e-11 * abs(u @ Mv)
