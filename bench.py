@@ -122,10 +122,13 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="hetero_elasticity_96")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph-iters", type=int, default=8,
+                    help="PCG iterations per captured HIP graph (0: eager launches of the same kernels)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus above 1 is not supported "
-                                   "(the path runs on one GPU's share, see DESIGN.md)"}))
+        print(json.dumps({"error": "the solve runs on one GPU; --gpus above 1 is not supported "
+                                   "(a multi-GPU run would shard rows by contiguous colour "
+                                   "blocks and exchange halos, which is not built)"}))
         sys.exit(2)
     import gpusolver as gs
 
@@ -137,17 +140,26 @@ def main() -> None:
     modes = {}
     # Plain FP64 AMG-PCG, the reference's method in FP64.
     t0 = time.perf_counter()
-    plain = gs.Solver(gs.PREC_DOUBLE, graph_iters=8, **sk)
+    plain = gs.Solver(gs.PREC_DOUBLE, graph_iters=a.graph_iters, **sk)
     plain.setup(H)
     t_setup_plain = time.perf_counter() - t0
     xp, ip, ms_plain = time_solves(plain, b, a.steps, a.warmup)
     modes["plain_fp64"] = dict(ms=ms_plain, iterations=ip[-1].iterations,
                                true_relres=ip[-1].true_relres,
                                device_ms=float(np.mean([i.solve_ms for i in ip])))
-    # Mixed: FP32 V-cycle in FP64 CG, FP64 coarsest Cholesky.
     mixed_kw = dict(sk)
     mixed_kw["coarse_direct"] = 1
-    mixed = gs.Solver(gs.PREC_MIXED, graph_iters=8, **mixed_kw)
+    # FP64 throughout with the FP64 coarsest Cholesky: separates the gain of
+    # the direct coarsest solve from that of the FP32 V-cycle.
+    fd = gs.Solver(gs.PREC_DOUBLE, graph_iters=a.graph_iters, **mixed_kw)
+    fd.setup(H)
+    _, idd, ms_fd = time_solves(fd, b, a.steps, a.warmup)
+    modes["fp64_direct"] = dict(ms=ms_fd, iterations=idd[-1].iterations,
+                                true_relres=idd[-1].true_relres,
+                                device_ms=float(np.mean([i.solve_ms for i in idd])))
+    del fd
+    # Mixed: FP32 V-cycle in FP64 CG, FP64 coarsest Cholesky.
+    mixed = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
     mixed.setup(H)
     xm, im, ms_mixed = time_solves(mixed, b, a.steps, a.warmup)
     modes["mixed"] = dict(ms=ms_mixed, iterations=im[-1].iterations,
@@ -156,7 +168,7 @@ def main() -> None:
     head = "mixed"
     lw = learned_weights(a.config, H)
     if lw is not None:
-        ml = gs.Solver(gs.PREC_MIXED, graph_iters=8, **mixed_kw)
+        ml = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
         ml.setup(H)
         ml.set_level_weights(omega=lw[0], sweeps=lw[1])
         ml.set_fallback(plain, lag=1.5)
@@ -206,7 +218,8 @@ def main() -> None:
         "data": "synthetic (seeded generators; no external data)",
         "config": {"workload": a.config, "description": desc, "n": int(A.rows), "nnz": int(A.nnz),
                    "levels": H.levels, "tol": 1e-8, "headline_mode": head},
-        "modes": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+        "modes": {k: {kk: (float(f"{vv:.6g}") if isinstance(vv, float) else vv)
+                      for kk, vv in v.items()}
                   for k, v in modes.items()},
         "speedup_vs_plain": round(modes["plain_fp64"]["device_ms"] / value, 3),
         "rel_err_vs_plain_fp64": err_vs_plain,

@@ -54,7 +54,7 @@ struct gs_backend_s {
 struct gs_encoder_s {
   gs_backend_s *be = nullptr;
   hipStream_t stream = nullptr;
-  DevBuf scratch;  // dot-product partials and ticket
+  DevBuf scratch;  // dot-product partials, then the zero-diagonal flag
   bool submitted = false;
   ~gs_encoder_s() {
     if (stream) {
@@ -231,18 +231,20 @@ __global__ __launch_bounds__(kBlock) void k_block_inv(const T *inv, const T *res
   }
 }
 
-// Deterministic dot: fixed-tree block sums, last block reduces the
-// partials in index order and writes the device scalar.
+// Deterministic dot: fixed-tree block sums into partials, then one block
+// (k_dot_final) sums the partials in index order into the device scalar.
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_dot(const T *x, const T *y, int64_t n, T *out,
-                                                T *partials, uint32_t *ticket) {
+__global__ __launch_bounds__(kBlock) void k_dot(const T *x, const T *y, int64_t n, T *partials) {
   T v = T(0);
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * kBlock)
     v += x[i] * y[i];
-  v = gsk::block_sum(v);
-  if (!gsk::publish_partial(v, partials, ticket)) return;
-  const T s = gsk::final_sum(partials, gridDim.x, ticket);
+  gsk::block_partial(v, partials);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot_final(const T *partials, uint32_t G, T *out) {
+  const T s = gsk::fixed_sum(partials, G);
   if (threadIdx.x == 0) *out = s;
 }
 
@@ -325,8 +327,8 @@ int gs_backend_create_encoder(gs_backend_h b, gs_encoder_h *out) {
     e->be = b;
     GS_HIP(hipStreamCreate(&e->stream));
     e->scratch.alloc((2048 + 64) * sizeof(double));
-    // Ticket and zero-diagonal flag start at zero (a fill kernel, so the
-    // stream stays capturable).
+    // The zero-diagonal flag starts at zero (a fill kernel, so the stream
+    // stays capturable).
     k_zero<double><<<1, kBlock, 0, e->stream>>>(
         reinterpret_cast<double *>(static_cast<char *>(e->scratch.get()) + 2048 * sizeof(double)),
         64);
@@ -765,12 +767,13 @@ int gs_encode_dot(gs_encoder_h e, gs_dvec_h x, gs_dvec_h y, gs_dscalar_h out) {
     need(out, "scalar");
     same_size(x, y, "encodeDot");
     char *base = static_cast<char *>(e->scratch.get());
-    uint32_t *ticket = reinterpret_cast<uint32_t *>(base + 2048 * sizeof(double));
+    const uint32_t G = dot_grid(x->n);
     dispatch(e->be, [&](auto t) {
       using T = decltype(t);
-      k_dot<T><<<dot_grid(x->n), kBlock, 0, e->stream>>>(dp<T>(x->d), dp<T>(y->d), x->n,
-                                                         dp<T>(out->d),
-                                                         reinterpret_cast<T *>(base), ticket);
+      k_dot<T><<<G, kBlock, 0, e->stream>>>(dp<T>(x->d), dp<T>(y->d), x->n,
+                                            reinterpret_cast<T *>(base));
+      k_dot_final<T><<<1, kBlock, 0, e->stream>>>(reinterpret_cast<const T *>(base), G,
+                                                  dp<T>(out->d));
     });
   });
 }

@@ -418,7 +418,10 @@ int gs_solver_set_real(gs_solver_h s, const char *key, double v) {
   return guard([&] {
     need(s, "solver");
     need(key, "key");
-    if (std::string(key) == "omega") s->opt.omega = v;
+    const std::string k(key);
+    if (k == "omega") s->opt.omega = v;
+    else if (k == "group_min_scalar") s->opt.group_min_scalar = v;
+    else if (k == "group_min_block") s->opt.group_min_block = v;
     else throw std::runtime_error(std::string("unknown real option: ") + key);
   });
 }

@@ -14,6 +14,7 @@
 
 #include "host/setup.h"
 #include "kernels.hpp"
+#include "kernels_wide.hpp"
 
 namespace gsd {
 
@@ -135,32 +136,128 @@ struct DBsr3 {
   }
 };
 
-// Square operator stored either as scalar SELL or as BSR3-SELL.
+// Row-contiguous CSR for the group-per-row kernels.
+template <class T>
+struct DRows {
+  int64_t rows = 0, cols = 0, nnz = 0;
+  DevBuf ptr, col, val;
+  void build(const gsx::Csr &A) {
+    rows = A.rows;
+    cols = A.cols;
+    nnz = A.nnz();
+    std::vector<int64_t> p(A.ptr.begin(), A.ptr.end());
+    for (int32_t c : A.col)
+      if (c < 0 || c >= A.cols) throw std::runtime_error("CSR upload: column index out of range.");
+    ptr.upload(p);
+    col.upload(A.col);
+    val.upload(cast_values<T>(A.val));
+  }
+  gsk::Rows<T> view() const {
+    return gsk::Rows<T>{rows, cols, ptr.as<int64_t>(), col.as<int32_t>(), val.as<T>()};
+  }
+};
+
+// Row-contiguous 3x3 block CSR (blocks of a row in the packer's order).
+template <class T>
+struct DBRows {
+  int64_t nb = 0, nblocks = 0;
+  DevBuf ptr, bcol, val;
+  void build(const gsx::Bsr3Ell &B) {
+    nb = B.nb;
+    nblocks = B.nblocks;
+    std::vector<int64_t> p(nb + 1, 0);
+    for (int64_t b = 0; b < nb; ++b) p[b + 1] = p[b] + B.nblk[b];
+    std::vector<int32_t> bc(nblocks);
+    std::vector<double> v(9 * nblocks);
+    for (int64_t b = 0; b < nb; ++b)
+      for (int32_t s = 0; s < B.nblk[b]; ++s) {
+        const int64_t k = p[b] + s;
+        bc[k] = B.bcol[static_cast<int64_t>(s) * nb + b];
+        if (bc[k] < 0 || bc[k] >= nb) throw std::runtime_error("BSR upload: block column out of range.");
+        for (int e = 0; e < 9; ++e) v[9 * k + e] = B.val[(static_cast<int64_t>(s) * 9 + e) * nb + b];
+      }
+    ptr.upload(p);
+    bcol.upload(bc);
+    val.upload(cast_values<T>(v));
+  }
+  gsk::BRows<T> view() const {
+    return gsk::BRows<T>{nb, ptr.as<int64_t>(), bcol.as<int32_t>(), val.as<T>()};
+  }
+};
+
+// Lanes per row for the group-per-row kernels given the average row length
+// (entries, or blocks for block rows); 0 keeps the thread-per-row SELL
+// layout. min_avg < 0 never groups.
+inline int pick_group(double avg, double min_avg) {
+  if (min_avg < 0.0 || avg < min_avg) return 0;
+  if (avg <= 16.0) return 16;
+  if (avg <= 32.0) return 32;
+  return 64;
+}
+
+// Scalar operator (transfers): SELL-64 or row-contiguous CSR.
+template <class T>
+struct DMat {
+  int G = 0;  // lanes per row; 0: SELL-64, one thread per row
+  int64_t rows = 0, cols = 0, nnz = 0;
+  DSell<T> s;
+  DRows<T> w;
+  void build(const gsx::Csr &A, double group_min) {
+    rows = A.rows;
+    cols = A.cols;
+    nnz = A.nnz();
+    G = pick_group(rows ? static_cast<double>(nnz) / rows : 0.0, group_min);
+    if (G) w.build(A);
+    else s.build(A);
+  }
+};
+
+// Square operator: scalar or 3x3 blocks, each either SELL-64 (thread per
+// row) or row-contiguous (G lanes per row).
 template <class T>
 struct DOp {
   bool b3 = false;
+  int G = 0;
   int64_t n = 0;
+  int64_t nnz = 0;      // scalar entries (blocks: 9 per block)
+  int64_t nblocks = 0;  // 3x3 blocks (0 for scalar storage)
   DSell<T> s;
   DBsr3<T> b;
-  void build(const gsx::Csr &A, bool prefer_b3) {
+  DRows<T> w;
+  DBRows<T> bw;
+  // block_key: optional rank of each block column, the order in which a
+  // block row's blocks are stored and summed. group_min: average row length
+  // (entries; blocks for block rows) from which rows are processed by lane
+  // groups (< 0: never).
+  void build(const gsx::Csr &A, bool prefer_b3, const int32_t *block_key = nullptr,
+             double group_min_scalar = -1.0, double group_min_block = -1.0) {
     n = A.rows;
     b3 = false;
+    G = 0;
+    nblocks = 0;
     if (prefer_b3) {
       gsx::Bsr3Ell B;
-      if (gsx::bsr3_pack(A, B)) {
-        b.build(B);
+      if (gsx::bsr3_pack(A, B, block_key)) {
         b3 = true;
+        nblocks = B.nblocks;
+        nnz = 9 * B.nblocks;
+        G = pick_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min_block);
+        if (G) bw.build(B);
+        else b.build(B);
         return;
       }
     }
-    s.build(A);
+    nnz = A.nnz();
+    G = pick_group(n ? static_cast<double>(nnz) / n : 0.0, group_min_scalar);
+    if (G) w.build(A);
+    else s.build(A);
   }
   // Algorithmic bytes of one y = A x (values + indices + x + y), the
   // roofline unit: scalar nnz (s + 4) + 2 n s, blocks nb (9 s + 4) + 2 n s.
   double spmv_bytes() const {
     const double sz = sizeof(T);
-    if (b3) return b.nblocks * (9 * sz + 4) + 2.0 * n * sz;
-    return s.nnz * (sz + 4) + 2.0 * n * sz;
+    if (b3) return nblocks * (9 * sz + 4) + 2.0 * n * sz;
+    return nnz * (sz + 4) + 2.0 * n * sz;
   }
 };
 

@@ -30,6 +30,7 @@ namespace gse {
 
 using gsd::DBsr3;
 using gsd::DevBuf;
+using gsd::DMat;
 using gsd::DOp;
 using gsd::DSell;
 using gsk::grid_for;
@@ -77,6 +78,17 @@ void launch_fill(T *p, T v, int64_t n, hipStream_t s, const int32_t *stop) {
   gsk::k_fill<T><<<grid_for(n), kBlock, 0, s>>>(p, v, n, stop);
 }
 
+// Calls f with the lane-group size G as a compile-time constant.
+template <class F>
+void with_group(int G, F &&f) {
+  switch (G) {
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    default: throw std::runtime_error("lane-group size must be 16, 32 or 64.");
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -95,7 +107,7 @@ struct Level {
   DevBuf diag, dinv, inv;
   DevBuf rhs_buf, x_buf, res, tmp;
   TM *rhs = nullptr, *x = nullptr;  // bound to own buffers or the CG vectors
-  DSell<TM> R, P;                    // transfer to level k + 1
+  DMat<TM> R, P;                     // transfer to level k + 1
   std::vector<int32_t> q;            // internal index -> level index
 };
 
@@ -109,6 +121,7 @@ class Engine final : public EngineBase {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (host_ctl_) (void)hipHostFree(host_ctl_);
+    if (stage_) (void)hipHostFree(stage_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     (void)hipStreamDestroy(stream_);
@@ -161,6 +174,8 @@ class Engine final : public EngineBase {
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   gsk::Ctl<TC> *host_ctl_ = nullptr;
+  TC *stage_ = nullptr;  // pinned staging for b and x (2 n values)
+  int64_t stage_n_ = 0;
   int64_t hist_cap_ = 0;
   std::vector<double> hist_host_;
   bool setup_done_ = false;
@@ -227,7 +242,15 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     gsx::spd_inverse(lv.n, dense);
     lv.inv.upload(dense);
   }
-  lv.A.build(Ai, blocky);
+  // Blocks summed in the level's own (pre-relabelling) column order, the
+  // order of the CSR rows the reference walks.
+  std::vector<int32_t> key;
+  if (blocky && lv.n % 3 == 0) {
+    key.resize(lv.n / 3);
+    for (int64_t j = 0; j < lv.n / 3; ++j) key[j] = lv.q[3 * j] / 3;
+  }
+  lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
+             opt_.group_min_block);
   if (!lv.direct) {
     if (lv.kind == kSGS || lv.kind == kDampedJacobi) {
       std::vector<TM> d(lv.n);
@@ -307,8 +330,8 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
         throw std::runtime_error("MultigridSmoother: transfer operator shape mismatch.");
       const std::vector<int32_t> qf_inv = inverse_of(L_[k].q);
       const std::vector<int32_t> qc_inv = inverse_of(L_[k + 1].q);
-      L_[k].P.build(relabel(Pk, L_[k].q, qc_inv));
-      L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv));
+      L_[k].P.build(relabel(Pk, L_[k].q, qc_inv), opt_.group_min_scalar);
+      L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv), opt_.group_min_scalar);
     }
   }
   // CG operator: level-0 input with the order-keeping permutation, then the
@@ -319,11 +342,18 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
     gsx::Csr Ai = relabel(Acg, L_[0].q, q0inv);
     const bool blocky = opt_.fine_kind == kBlockGS || opt_.fine_kind == kBlockJacobi ||
                         opt_.cg_block3;
-    Acg_.build(Ai, blocky && n_ % 3 == 0);
+    int2lvl0_ = L_[0].q;
+    int2orig_.resize(n_);
+    for (int64_t i = 0; i < n_; ++i) int2orig_[i] = perms[0].new_to_old[L_[0].q[i]];
+    // The keep-order permutation leaves each row in the input's column
+    // order: rank block columns by their input index.
+    std::vector<int32_t> key;
+    if (blocky && n_ % 3 == 0) {
+      key.resize(n_ / 3);
+      for (int64_t j = 0; j < n_ / 3; ++j) key[j] = int2orig_[3 * j] / 3;
+    }
+    Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data());
   }
-  int2lvl0_ = L_[0].q;
-  int2orig_.resize(n_);
-  for (int64_t i = 0; i < n_; ++i) int2orig_[i] = perms[0].new_to_old[L_[0].q[i]];
   // CG vectors and control block.
   b_.alloc(n_ * sizeof(TC));
   x_.alloc(n_ * sizeof(TC));
@@ -344,6 +374,13 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   tr_.alloc(2 * sizeof(double));
   if (!host_ctl_) GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_ctl_), sizeof(gsk::Ctl<TC>)));
   std::memset(host_ctl_, 0, sizeof(gsk::Ctl<TC>));
+  if (stage_n_ != n_) {
+    if (stage_) GS_HIP(hipHostFree(stage_));
+    stage_ = nullptr;
+    stage_n_ = 0;
+    GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&stage_), 2 * n_ * sizeof(TC)));
+    stage_n_ = n_;
+  }
   if (!ev0_) GS_HIP(hipEventCreate(&ev0_));
   if (!ev1_) GS_HIP(hipEventCreate(&ev1_));
   if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
@@ -357,12 +394,24 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
 template <class TC, class TM>
 void Engine<TC, TM>::encode_residual(Level<TM> &lv, const int32_t *stop) {
   const int64_t n = lv.n;
-  if (lv.A.b3)
+  TM *res = lv.res.template as<TM>();
+  if (lv.A.G) {
+    with_group(lv.A.G, [&](auto g) {
+      constexpr int G = decltype(g)::value;
+      if (lv.A.b3)
+        gsk::k_residual_bw<TM, G><<<gsk::grid_rows<G>(n / 3), kBlock, 0, stream_>>>(
+            lv.A.bw.view(), lv.rhs, lv.x, res, stop);
+      else
+        gsk::k_residual_w<TM, G><<<gsk::grid_rows<G>(n), kBlock, 0, stream_>>>(
+            lv.A.w.view(), lv.rhs, lv.x, res, stop);
+    });
+  } else if (lv.A.b3) {
     gsk::k_residual_b3<TM><<<grid_for(n / 3), kBlock, 0, stream_>>>(lv.A.b.view(), lv.rhs, lv.x,
-                                                                    lv.res.template as<TM>(), stop);
-  else
-    gsk::k_residual<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.A.s.view(), lv.rhs, lv.x,
-                                                             lv.res.template as<TM>(), stop);
+                                                                    res, stop);
+  } else {
+    gsk::k_residual<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.A.s.view(), lv.rhs, lv.x, res,
+                                                             stop);
+  }
 }
 
 // The colour sequence of `iters` symmetric sweeps (forward 0..C-1, backward
@@ -391,7 +440,9 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
     if (block && !lv.A.b3)
       throw std::runtime_error("block Gauss-Seidel level is not 3x3-block representable.");
     const int64_t units = block ? n / 3 : n;
-    for (auto [c, reps] : sweep_sequence(lv.ncolors, iters)) {
+    for (const auto &step : sweep_sequence(lv.ncolors, iters)) {
+      const int c = step.first;
+      int reps = step.second;
       const int64_t b = lv.coff[c], e = lv.coff[c + 1];
       if (!x_ready && c == 0) {
         // First update from x = 0: closed form for colour 0, zero elsewhere.
@@ -406,20 +457,45 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
         reps = 1;  // the second update of the pair reads the fresh values
       }
       if (e <= b) continue;
-      if (block)
+      if (lv.A.G) {
+        with_group(lv.A.G, [&](auto g) {
+          constexpr int G = decltype(g)::value;
+          if (block)
+            gsk::k_block_gs_color_bw<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
+                lv.A.bw.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
+                stop);
+          else
+            gsk::k_gs_color_w<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
+                lv.A.w.view(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+        });
+      } else if (block) {
         gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
             lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
-      else
+      } else {
         gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
             lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+      }
     }
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {
     if (!x_ready && iters > 0) {
       launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
       x_ready = true;
     }
+    TM *dinv = lv.dinv.template as<TM>(), *tmp = lv.tmp.template as<TM>();
     for (int it = 0; it < iters; ++it) {
-      if (lv.kind == kDampedJacobi) {
+      if (lv.A.G) {
+        with_group(lv.A.G, [&](auto g) {
+          constexpr int G = decltype(g)::value;
+          if (lv.kind == kDampedJacobi && !lv.A.b3)
+            gsk::k_jacobi_w<TM, G><<<gsk::grid_rows<G>(n), kBlock, 0, stream_>>>(
+                lv.A.w.view(), dinv, lv.rhs, lv.x, tmp, lv.omega, stop);
+          else if (lv.kind == kBlockJacobi && lv.A.b3)
+            gsk::k_block_jacobi_bw<TM, G><<<gsk::grid_rows<G>(n / 3), kBlock, 0, stream_>>>(
+                lv.A.bw.view(), dinv, lv.rhs, lv.x, tmp, lv.omega, stop);
+          else
+            throw std::runtime_error("Jacobi smoother: operator storage does not match the smoother.");
+        });
+      } else if (lv.kind == kDampedJacobi) {
         gsk::k_jacobi<TM><<<grid_for(n), kBlock, 0, stream_>>>(
             lv.A.s.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
       } else if (lv.A.b3) {
@@ -455,13 +531,28 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
   const bool coarse_self_zero =
       !cl.direct && ((k + 1 == nlev_ - 1) ? opt_.coarse_steps > 0 : cl.pre > 0) &&
       (cl.kind == kSGS || cl.kind == kBlockGS);
-  gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(
-      lv.R.view(), lv.res.template as<TM>(), cl.rhs, coarse_self_zero ? nullptr : cl.x, stop);
-  encode_vcycle(k + 1, stop, true);
-  if (opt_.reference_vcycle && k > 0)
-    gsk::k_prolong_alias<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.view(), cl.x, lv.x, stop);
+  TM *corr_c = coarse_self_zero ? nullptr : cl.x;
+  if (lv.R.G)
+    with_group(lv.R.G, [&](auto g) {
+      constexpr int G = decltype(g)::value;
+      gsk::k_restrict_w<TM, G><<<gsk::grid_rows<G>(cl.n), kBlock, 0, stream_>>>(
+          lv.R.w.view(), lv.res.template as<TM>(), cl.rhs, corr_c, stop);
+    });
   else
-    gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.view(), cl.x, lv.x, stop);
+    gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(
+        lv.R.s.view(), lv.res.template as<TM>(), cl.rhs, corr_c, stop);
+  encode_vcycle(k + 1, stop, true);
+  const bool alias = opt_.reference_vcycle && k > 0;
+  if (lv.P.G)
+    with_group(lv.P.G, [&](auto g) {
+      constexpr int G = decltype(g)::value;
+      gsk::k_prolong_w<TM, G><<<gsk::grid_rows<G>(lv.n), kBlock, 0, stream_>>>(
+          lv.P.w.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+    });
+  else if (alias)
+    gsk::k_prolong_alias<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), cl.x, lv.x, stop);
+  else
+    gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), cl.x, lv.x, stop);
   encode_smooth(lv, lv.post, false, stop);
 }
 
@@ -486,12 +577,16 @@ void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   else
     gsk::k_cg_spmv_pap<TC><<<g, kBlock, 0, stream_>>>(Acg_.s.view(), p_.template as<TC>(), q_.template as<TC>(),
                                                        ctl, parts);
+  gsk::k_cg_alpha<TC><<<1, kBlock, 0, stream_>>>(ctl, parts, g);
   TM *rm = std::is_same<TC, TM>::value ? nullptr : rm_.template as<TM>();
-  gsk::k_cg_update_xr<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(
+  const uint32_t gn = grid_for(n_);
+  gsk::k_cg_update_xr<TC, TM><<<gn, kBlock, 0, stream_>>>(
       x_.template as<TC>(), r_.template as<TC>(), rm, p_.template as<TC>(), q_.template as<TC>(), n_, ctl, parts);
+  gsk::k_cg_rr_final<TC><<<1, kBlock, 0, stream_>>>(ctl, parts, gn);
   apply_preconditioner(stop);
-  gsk::k_cg_rz<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(r_.template as<TC>(), z_.template as<TM>(), n_, ctl,
-                                                             parts);
+  gsk::k_cg_rz<TC, TM><<<gn, kBlock, 0, stream_>>>(r_.template as<TC>(), z_.template as<TM>(), n_, ctl,
+                                                   parts);
+  gsk::k_cg_beta<TC><<<1, kBlock, 0, stream_>>>(ctl, parts, gn);
   gsk::k_cg_update_p<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(p_.template as<TC>(), z_.template as<TM>(), n_,
                                                                    ctl);
 }
@@ -521,13 +616,14 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
     res.zero_rhs = true;
     return res;
   }
-  std::vector<TC> bi(n_), xi(n_);
+  // Gather into pinned staging (internal order), one DMA each way.
+  TC *bi = stage_, *xi = stage_ + n_;
   for (int64_t i = 0; i < n_; ++i) {
     bi[i] = static_cast<TC>(b[int2orig_[i]]);
     xi[i] = static_cast<TC>(x[int2orig_[i]]);
   }
-  GS_HIP(hipMemcpyAsync(b_.get(), bi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
-  GS_HIP(hipMemcpyAsync(x_.get(), xi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
+  GS_HIP(hipMemcpyAsync(b_.get(), bi, n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
+  GS_HIP(hipMemcpyAsync(x_.get(), xi, n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
   if (hist_cap_ < maxit + 1) {
     hist_.alloc((maxit + 1) * sizeof(TC));
     hist_cap_ = maxit + 1;
@@ -552,6 +648,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   apply_preconditioner(nullptr);
   gsk::k_cg_init_dots<TC, TM><<<grid_for(n_), kBlock, 0, stream_>>>(
       b_.template as<TC>(), r_.template as<TC>(), z_.template as<TM>(), p_.template as<TC>(), n_, ctl, partials_.template as<TC>());
+  gsk::k_cg_init_final<TC><<<1, kBlock, 0, stream_>>>(ctl, partials_.template as<TC>(), grid_for(n_));
   const int32_t *stop = &ctl->stop;
   int64_t launched = 0;
   if (maxit > 0) {
@@ -594,17 +691,19 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   GS_HIP(hipMemcpy(h.data(), hist_.get(), hl * sizeof(TC), hipMemcpyDeviceToHost));
   hist_host_.assign(h.begin(), h.end());
   // Solution back to the caller's ordering.
-  GS_HIP(hipMemcpy(xi.data(), x_.get(), n_ * sizeof(TC), hipMemcpyDeviceToHost));
+  GS_HIP(hipMemcpyAsync(xi, x_.get(), n_ * sizeof(TC), hipMemcpyDeviceToHost, stream_));
+  GS_HIP(hipStreamSynchronize(stream_));
   for (int64_t i = 0; i < n_; ++i) x[int2orig_[i]] = static_cast<double>(xi[i]);
   // True residual of the CG operator in FP64 arithmetic.
+  const uint32_t gt = grid_for(Acg_.b3 ? n_ / 3 : n_);
   if (Acg_.b3)
-    gsk::k_true_residual_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(
-        Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), tr_.template as<double>(), partials_.template as<double>(),
-        &ctl->ticket[0]);
+    gsk::k_true_residual_b3<TC><<<gt, kBlock, 0, stream_>>>(
+        Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), partials_.template as<double>());
   else
-    gsk::k_true_residual<TC><<<grid_for(n_), kBlock, 0, stream_>>>(
-        Acg_.s.view(), b_.template as<TC>(), x_.template as<TC>(), tr_.template as<double>(), partials_.template as<double>(),
-        &ctl->ticket[0]);
+    gsk::k_true_residual<TC><<<gt, kBlock, 0, stream_>>>(
+        Acg_.s.view(), b_.template as<TC>(), x_.template as<TC>(), partials_.template as<double>());
+  gsk::k_true_final<double><<<1, kBlock, 0, stream_>>>(partials_.template as<double>(), gt,
+                                                        tr_.template as<double>());
   double tr[2];
   GS_HIP(hipMemcpyAsync(tr, tr_.get(), sizeof(tr), hipMemcpyDeviceToHost, stream_));
   GS_HIP(hipStreamSynchronize(stream_));
@@ -673,8 +772,8 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
   if (which == 3 && !L_.empty()) {
     const double s = sizeof(TM);
     const auto &lv = L_[0];
-    *bytes = lv.A.b3 ? 2 * (lv.A.b.nblocks * (9 * s + 4) + 3.0 * lv.n * s + 9.0 * (lv.n / 3) * s)
-                     : 2 * (lv.A.s.nnz * (s + 4) + 3.0 * lv.n * s);
+    *bytes = lv.A.b3 ? 2 * (lv.A.nblocks * (9 * s + 4) + 3.0 * lv.n * s + 9.0 * (lv.n / 3) * s)
+                     : 2 * (lv.A.nnz * (s + 4) + 3.0 * lv.n * s);
   }
   // Keep the iteration kernels live: a zeroed control block with stop = 0.
   if (which == 2) {
@@ -704,8 +803,11 @@ void Engine<TC, TM>::level_info(std::vector<LevelInfo> &out) const {
     const auto &lv = L_[k];
     LevelInfo li;
     li.n = lv.n;
-    li.nnz = lv.A.b3 ? lv.A.b.nblocks * 9 : lv.A.s.nnz;
-    li.blocks3 = lv.A.b3 ? lv.A.b.nblocks : 0;
+    li.nnz = lv.A.nnz;
+    li.blocks3 = lv.A.nblocks;
+    li.group = lv.A.G;
+    li.r_group = k + 1 < nlev_ ? lv.R.G : 0;
+    li.p_group = k + 1 < nlev_ ? lv.P.G : 0;
     li.ncolors = lv.ncolors;
     li.kind = lv.kind;
     li.direct = lv.direct;
@@ -727,7 +829,9 @@ std::string Engine<TC, TM>::describe() const {
     os << "{\"n\": " << li[k].n << ", \"nnz\": " << li[k].nnz << ", \"blocks3\": " << li[k].blocks3
        << ", \"colors\": " << li[k].ncolors << ", \"kind\": " << li[k].kind
        << ", \"direct\": " << (li[k].direct ? "true" : "false") << ", \"omega\": " << li[k].omega
-       << ", \"pre\": " << li[k].pre << ", \"post\": " << li[k].post << "}";
+       << ", \"pre\": " << li[k].pre << ", \"post\": " << li[k].post
+       << ", \"group\": " << li[k].group << ", \"r_group\": " << li[k].r_group
+       << ", \"p_group\": " << li[k].p_group << "}";
   }
   os << "], \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
      << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters

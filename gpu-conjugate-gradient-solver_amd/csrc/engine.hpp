@@ -31,6 +31,11 @@ struct Options {
   bool reference_vcycle = false;
   // Largest coarsest level accepted for the dense FP64 inverse.
   int64_t max_direct = 8192;
+  // Operators whose average row length reaches these (scalar entries per
+  // row; 3x3 blocks per block row) run on lane-group kernels (same
+  // arithmetic, same order); negative: never.
+  double group_min_scalar = 24.0;
+  double group_min_block = 20.0;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (> 0 applies)
 };
@@ -53,6 +58,9 @@ struct LevelInfo {
   bool direct = false;
   double omega = 1.0;
   int pre = 1, post = 1;
+  int group = 0;    // lanes per row of the level operator (0: one thread)
+  int r_group = 0;  // same for the restriction to the next level
+  int p_group = 0;  // and the prolongation from it
 };
 
 class EngineBase {

@@ -76,10 +76,8 @@ int32_t max_row_length(const Csr &A);
 // major; the 9 values of block slot s of block row b sit at
 // val[(s * 9 + e) * nb + b] for e = 3 * r + c, so a wavefront reading 64
 // consecutive block rows issues 64-wide unit-stride loads per element.
-// Requires every scalar row of a block row to list the same block columns
-// in the same order (true for the assembled elasticity operator and every
-// Galerkin level built from it); blocks missing a scalar entry are padded
-// with exact zeros.
+// A block row stores the union of the block columns of its three scalar
+// rows; blocks missing a scalar entry are padded with exact zeros.
 struct Bsr3Ell {
   int64_t nb = 0;            // block rows
   int32_t width = 0;         // max blocks per block row
@@ -88,9 +86,11 @@ struct Bsr3Ell {
   std::vector<int32_t> nblk; // blocks per block row
   int64_t nblocks = 0;       // stored (non-padding) blocks
 };
-// Returns false (and leaves out untouched) when A is not representable as
-// 3x3 blocks with sorted, block-aligned rows.
-bool bsr3_pack(const Csr &A, Bsr3Ell &out);
+// Returns false (and leaves out untouched) when A is not square with a
+// multiple of 3 rows. Blocks of a block row are stored in increasing
+// key[block column] (block column itself when key is null); duplicate
+// entries are summed.
+bool bsr3_pack(const Csr &A, Bsr3Ell &out, const int32_t *key = nullptr);
 
 // SELL-64: the device layout of every scalar operator. Rows are cut into
 // slices of 64; slice s stores its rows slot-major with width slice_w[s]

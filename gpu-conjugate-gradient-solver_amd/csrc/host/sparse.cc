@@ -309,22 +309,13 @@ Ell ell_pack(const Csr &A, int32_t width) {
   return E;
 }
 
-bool bsr3_pack(const Csr &Ain, Bsr3Ell &out) {
-  if (Ain.rows != Ain.cols || Ain.rows % 3 != 0) return false;
-  const Csr *Ap = &Ain;
-  Csr sorted;
-  bool is_sorted = true;
-  for (int64_t r = 0; r < Ain.rows && is_sorted; ++r)
-    for (int32_t k = Ain.ptr[r] + 1; k < Ain.ptr[r + 1]; ++k)
-      if (Ain.col[k] <= Ain.col[k - 1]) { is_sorted = false; break; }
-  if (!is_sorted) {
-    sorted = Ain;
-    sort_rows(sorted);
-    Ap = &sorted;
-  }
-  const Csr &A = *Ap;
+bool bsr3_pack(const Csr &A, Bsr3Ell &out, const int32_t *key) {
+  if (A.rows != A.cols || A.rows % 3 != 0) return false;
   const int64_t nb = A.rows / 3;
-  // Block columns of each block row: union over its three scalar rows.
+  auto rank = [key](int32_t cb) { return key ? key[cb] : cb; };
+  // Block columns of each block row: union over its three scalar rows,
+  // ordered by rank, so that a row whose entries come in rank order is
+  // summed in that same order by the block kernels.
   std::vector<std::vector<int32_t>> bcols(nb);
   int32_t width = 0;
   int64_t nblocks = 0;
@@ -335,7 +326,9 @@ bool bsr3_pack(const Csr &Ain, Bsr3Ell &out) {
       const int64_t r = 3 * b + l;
       for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) bc.push_back(A.col[k] / 3);
     }
-    std::sort(bc.begin(), bc.end());
+    std::sort(bc.begin(), bc.end(), [&](int32_t u, int32_t v) {
+      return rank(u) != rank(v) ? rank(u) < rank(v) : u < v;
+    });
     bc.erase(std::unique(bc.begin(), bc.end()), bc.end());
     width = std::max(width, static_cast<int32_t>(bc.size()));
     nblocks += static_cast<int64_t>(bc.size());
@@ -353,14 +346,19 @@ bool bsr3_pack(const Csr &Ain, Bsr3Ell &out) {
     for (int32_t s = 0; s < width; ++s)
       out.bcol[static_cast<int64_t>(s) * nb + b] =
           s < static_cast<int32_t>(bc.size()) ? bc[s] : static_cast<int32_t>(b);
+    // Slot of each block column (bc holds distinct columns).
+    std::vector<std::pair<int32_t, int32_t>> slot_of(bc.size());
+    for (size_t s = 0; s < bc.size(); ++s) slot_of[s] = {bc[s], static_cast<int32_t>(s)};
+    std::sort(slot_of.begin(), slot_of.end());
     for (int l = 0; l < 3; ++l) {
       const int64_t r = 3 * b + l;
-      size_t s = 0;
       for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
         const int32_t cb = A.col[k] / 3;
-        while (bc[s] != cb) ++s;
+        const auto it = std::lower_bound(slot_of.begin(), slot_of.end(),
+                                         std::make_pair(cb, static_cast<int32_t>(-1)));
+        const int64_t s = it->second;
         const int e = 3 * l + (A.col[k] % 3);
-        out.val[(static_cast<int64_t>(s) * 9 + e) * nb + b] = A.val[k];
+        out.val[(s * 9 + e) * nb + b] += A.val[k];
       }
     }
   }

@@ -14,10 +14,11 @@
 // nine value planes of 64 lanes each.
 //
 // Reductions are deterministic: each workgroup reduces its chunk with wave
-// shuffles and LDS in a fixed tree, writes one partial, and the last
-// workgroup to arrive (device-scope atomic ticket) reduces the partials in
-// index order and applies the scalar epilogue. The ticket resets itself,
-// so the kernels can be replayed from a captured graph.
+// shuffles and LDS in a fixed tree and stores one partial; a one-workgroup
+// finalize kernel launched next on the same stream sums the partials in a
+// fixed order and applies the scalar epilogue. The kernel boundary makes
+// the partials visible across XCDs; a last-block-arrives scheme would need
+// a device-scope release (an L2 writeback on this part) in every block.
 //
 // Every kernel that runs inside the PCG loop takes a pointer to the
 // solver's stop word and returns at once when it is set, so iterations
@@ -69,7 +70,6 @@ struct Ctl {
   int64_t iters_out;  // reported iteration count (reference convention)
   int32_t stop;       // 0 run, 1 converged, 2 maxit, 3 non-finite, 4 breakdown
   int32_t breakdown;  // 1: p.Ap <= 0, 2: r.z <= 0 (recorded, not fatal)
-  uint32_t ticket[4];
   T *hist;            // relres^2 per iteration (length maxit + 1) or null
 };
 
@@ -105,29 +105,20 @@ __device__ __forceinline__ T block_sum(T v) {
   return r;
 }
 
-// Writes this block's partial and reports whether it arrived last. The
-// last block must then call final_sum over `nparts` partials.
+// Stores this block's sum as partials[blockIdx.x].
 template <class T>
-__device__ __forceinline__ bool publish_partial(T v, T *partials, uint32_t *ticket) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = v;
-    __threadfence();
-    const uint32_t t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
+__device__ __forceinline__ void block_partial(T v, T *partials) {
+  v = block_sum(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
+// Sum of nparts partials in a fixed order, for a one-block launch of kBlock
+// threads; result valid in thread 0.
 template <class T>
-__device__ __forceinline__ T final_sum(const T *partials, uint32_t nparts, uint32_t *ticket) {
+__device__ __forceinline__ T fixed_sum(const T *partials, uint32_t nparts) {
   T s = T(0);
   for (uint32_t i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
-  s = block_sum(s);
-  if (threadIdx.x == 0) *ticket = 0u;
-  return s;
+  return block_sum(s);
 }
 
 template <class T>
@@ -555,32 +546,20 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_dots(const TC *__restrict__ 
     srr = ri * ri;
     p[i] = zi;
   }
-  sbb = block_sum(sbb);
-  srz = block_sum(srz);
-  srr = block_sum(srr);
   const uint32_t G = gridDim.x;
-  __shared__ bool last;
+  block_partial(sbb, partials);
+  block_partial(srz, partials + G);
+  block_partial(srr, partials + 2 * G);
+}
+
+// One block: bb, rho, rr from the partials of k_cg_init_dots (G each).
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_cg_init_final(Ctl<TC> *ctl, const TC *partials,
+                                                          uint32_t G) {
+  const TC a = fixed_sum(partials, G);
+  const TC c = fixed_sum(partials + G, G);
+  const TC d = fixed_sum(partials + 2 * G, G);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = sbb;
-    partials[G + blockIdx.x] = srz;
-    partials[2 * G + blockIdx.x] = srr;
-    __threadfence();
-    last = atomicAdd(&ctl->ticket[0], 1u) == G - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  TC a = TC(0), c = TC(0), d = TC(0);
-  for (uint32_t k = threadIdx.x; k < G; k += kBlock) {
-    a += partials[k];
-    c += partials[G + k];
-    d += partials[2 * G + k];
-  }
-  a = block_sum(a);
-  c = block_sum(c);
-  d = block_sum(d);
-  if (threadIdx.x == 0) {
-    ctl->ticket[0] = 0u;
     ctl->bb = a;
     ctl->rho = c;
     ctl->rr = d;
@@ -601,13 +580,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_dots(const TC *__restrict__ 
   }
 }
 
-// q = A p and p.q; the last block forms alpha = rho / pAp
-// (src/CGSolver.cc:210-215).
+// q = A p with per-block partials of p.q; k_cg_alpha then forms
+// alpha = rho / pAp (src/CGSolver.cc:210-215).
 template <class TC>
-__device__ __forceinline__ void cg_pap_epilogue(TC v, Ctl<TC> *ctl, TC *partials) {
-  v = block_sum(v);
-  if (!publish_partial(v, partials, &ctl->ticket[1])) return;
-  const TC pap = final_sum(partials, gridDim.x, &ctl->ticket[1]);
+__global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl<TC> *ctl, const TC *partials,
+                                                     uint32_t G) {
+  if (ctl->stop) return;
+  const TC pap = fixed_sum(partials, G);
   if (threadIdx.x == 0) {
     ctl->pap = pap;
     ctl->alpha = ctl->rho / pap;
@@ -627,7 +606,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(Sell<TC> A, const TC *__
     q[row] = qi;
     v = p[row] * qi;
   }
-  cg_pap_epilogue(v, ctl, partials);
+  block_partial(v, partials);
 }
 
 template <class TC>
@@ -647,11 +626,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3(Bsr3<TC> A, const TC 
     v += p[3 * b + 1] * y[1];
     v += p[3 * b + 2] * y[2];
   }
-  cg_pap_epilogue(v, ctl, partials);
+  block_partial(v, partials);
 }
 
-// x += alpha p; r = r + (-alpha) q; rr = r.r; rel = rr / bb; stop test
-// (src/CGSolver.cc:218-230, 252-268). Writes the V-cycle-type copy of r.
+// x += alpha p; r = r + (-alpha) q; partials of r.r (k_cg_rr_final then
+// forms rel = rr / bb and the stop test, src/CGSolver.cc:218-230, 252-268).
+// Writes the V-cycle-type copy of r.
 template <class TC, class TM>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(TC *__restrict__ x, TC *__restrict__ r,
                                                          TM *__restrict__ rm,
@@ -670,9 +650,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(TC *__restrict__ x, TC 
     if (rm) rm[i] = static_cast<TM>(ri);
     v = ri * ri;
   }
-  v = block_sum(v);
-  if (!publish_partial(v, partials, &ctl->ticket[2])) return;
-  const TC rr = final_sum(partials, gridDim.x, &ctl->ticket[2]);
+  block_partial(v, partials);
+}
+
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_cg_rr_final(Ctl<TC> *ctl, const TC *partials,
+                                                        uint32_t G) {
+  if (ctl->stop) return;
+  const TC rr = fixed_sum(partials, G);
   if (threadIdx.x == 0) {
     const TC rel = rr / ctl->bb;
     ctl->rr = rr;
@@ -692,7 +677,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(TC *__restrict__ x, TC 
   }
 }
 
-// rho_new = r.z, beta = rho_new / rho (src/CGSolver.cc:236-239).
+// Partials of r.z; k_cg_beta forms rho_new and beta = rho_new / rho
+// (src/CGSolver.cc:236-239).
 template <class TC, class TM>
 __global__ __launch_bounds__(kBlock) void k_cg_rz(const TC *__restrict__ r,
                                                   const TM *__restrict__ z, int64_t n,
@@ -701,9 +687,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_rz(const TC *__restrict__ r,
   const int64_t i = xcd_block() * kBlock + threadIdx.x;
   TC v = TC(0);
   if (i < n) v = r[i] * static_cast<TC>(z[i]);
-  v = block_sum(v);
-  if (!publish_partial(v, partials, &ctl->ticket[3])) return;
-  const TC rz = final_sum(partials, gridDim.x, &ctl->ticket[3]);
+  block_partial(v, partials);
+}
+
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_cg_beta(Ctl<TC> *ctl, const TC *partials, uint32_t G) {
+  if (ctl->stop) return;
+  const TC rz = fixed_sum(partials, G);
   if (threadIdx.x == 0) {
     ctl->rho_new = rz;
     ctl->beta = rz / ctl->rho;
@@ -729,8 +719,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(TC *__restrict__ p,
 template <class TC>
 __global__ __launch_bounds__(kBlock) void k_true_residual(Sell<TC> A, const TC *__restrict__ b,
                                                           const TC *__restrict__ x,
-                                                          double *out2, double *partials,
-                                                          uint32_t *ticket) {
+                                                          double *partials) {
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   double rr = 0.0, bb = 0.0;
   if (row < A.rows) {
@@ -747,38 +736,14 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(Sell<TC> A, const TC *
     rr = d * d;
     bb = bi * bi;
   }
-  rr = block_sum(rr);
-  bb = block_sum(bb);
-  const uint32_t G = gridDim.x;
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = rr;
-    partials[G + blockIdx.x] = bb;
-    __threadfence();
-    last = atomicAdd(ticket, 1u) == G - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double a = 0.0, c = 0.0;
-  for (uint32_t k = threadIdx.x; k < G; k += kBlock) {
-    a += partials[k];
-    c += partials[G + k];
-  }
-  a = block_sum(a);
-  c = block_sum(c);
-  if (threadIdx.x == 0) {
-    *ticket = 0u;
-    out2[0] = a;
-    out2[1] = c;
-  }
+  block_partial(rr, partials);
+  block_partial(bb, partials + gridDim.x);
 }
 
 template <class TC>
 __global__ __launch_bounds__(kBlock) void k_true_residual_b3(Bsr3<TC> A, const TC *__restrict__ b,
                                                              const TC *__restrict__ x,
-                                                             double *out2, double *partials,
-                                                             uint32_t *ticket) {
+                                                             double *partials) {
   const int64_t blk = xcd_block() * kBlock + threadIdx.x;
   double rr = 0.0, bb = 0.0;
   if (blk < A.nb) {
@@ -801,28 +766,16 @@ __global__ __launch_bounds__(kBlock) void k_true_residual_b3(Bsr3<TC> A, const T
       bb += bi * bi;
     }
   }
-  rr = block_sum(rr);
-  bb = block_sum(bb);
-  const uint32_t G = gridDim.x;
-  __shared__ bool last;
+  block_partial(rr, partials);
+  block_partial(bb, partials + gridDim.x);
+}
+
+// One block: out2 = {||b - A x||^2, ||b||^2} from the partials (G each).
+template <class T = double>
+__global__ __launch_bounds__(kBlock) void k_true_final(const T *partials, uint32_t G, T *out2) {
+  const T a = fixed_sum(partials, G);
+  const T c = fixed_sum(partials + G, G);
   if (threadIdx.x == 0) {
-    partials[blockIdx.x] = rr;
-    partials[G + blockIdx.x] = bb;
-    __threadfence();
-    last = atomicAdd(ticket, 1u) == G - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double a = 0.0, c = 0.0;
-  for (uint32_t k = threadIdx.x; k < G; k += kBlock) {
-    a += partials[k];
-    c += partials[G + k];
-  }
-  a = block_sum(a);
-  c = block_sum(c);
-  if (threadIdx.x == 0) {
-    *ticket = 0u;
     out2[0] = a;
     out2[1] = c;
   }

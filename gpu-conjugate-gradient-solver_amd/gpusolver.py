@@ -441,6 +441,9 @@ class SolveInfo:
     wall_ms: float
 
 
+_REAL_OPTIONS = ("omega", "group_min_scalar", "group_min_block")
+
+
 class Solver:
     """CGSolver + preconditioner on the GPU (gs_solver_* in include/gpusolver.h)."""
 
@@ -459,7 +462,7 @@ class Solver:
             self.h = None
 
     def set(self, key: str, value) -> None:
-        if key == "omega":
+        if key in _REAL_OPTIONS:
             check(lib.gs_solver_set_real(self.h, key.encode(), float(value)))
         else:
             check(lib.gs_solver_set_int(self.h, key.encode(), int(value)))

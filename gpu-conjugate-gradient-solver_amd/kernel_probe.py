@@ -28,11 +28,12 @@ def main() -> None:
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--precision", type=int, default=gs.PREC_DOUBLE)
     ap.add_argument("--copy-mib", type=int, default=512)
+    ap.add_argument("--graph-iters", type=int, default=0)
     a = ap.parse_args()
     import bench
     A, b, hk, sk, desc = bench.build_workload(a.config)
     H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
-    s = gs.Solver(a.precision, graph_iters=8, **sk)
+    s = gs.Solver(a.precision, graph_iters=a.graph_iters, **sk)
     s.setup(H)
     s.solve(b, tol=1e-8, maxit=3)  # leaves valid vectors in p
     ms, by = s.time_kernel(0, a.reps)

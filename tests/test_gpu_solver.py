@@ -62,13 +62,24 @@ def test_spmv_bitexact(cfg1, prec):
     np.testing.assert_array_equal(y, yo.astype(np.float64))
 
 
+# Operator storage: -1 = one thread per row everywhere (SELL-64), 0 = lane
+# groups for every operator, None = the default selection by row length.
+GROUPS = [-1.0, 0.0, None]
+
+
+def group_opts(grp):
+    return {} if grp is None else dict(group_min_scalar=grp, group_min_block=grp)
+
+
+@pytest.mark.parametrize("grp", GROUPS)
 @pytest.mark.parametrize("ref", [0, 1])
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED, gs.PREC_SINGLE])
 @pytest.mark.parametrize("kind,omega", [(gs.SGS, 1.0), (gs.SGS, 1.3), (gs.DAMPED_JACOBI, 0.7)])
-def test_vcycle_bitexact_scalar(cfg1, prec, kind, omega, ref):
+def test_vcycle_bitexact_scalar(cfg1, prec, kind, omega, ref, grp):
     """ref = 1: the reference's coarse-level correction aliasing."""
     A, H = cfg1
-    s = gs.Solver(prec, smoother=kind, omega=omega, coarse_steps=2, reference_vcycle=ref)
+    s = gs.Solver(prec, smoother=kind, omega=omega, coarse_steps=2, reference_vcycle=ref,
+                  **group_opts(grp))
     s.setup(H)
     hv, _ = hier_view(H)
     r = gs.uniform_pm1(11, A.rows)
@@ -78,12 +89,14 @@ def test_vcycle_bitexact_scalar(cfg1, prec, kind, omega, ref):
     np.testing.assert_array_equal(z, zo)
 
 
+@pytest.mark.parametrize("grp", GROUPS)
 @pytest.mark.parametrize("ref", [0, 1])
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
 @pytest.mark.parametrize("kind", [gs.BLOCK_GS, gs.BLOCK_JACOBI])
-def test_vcycle_bitexact_block(elast, prec, kind, ref):
+def test_vcycle_bitexact_block(elast, prec, kind, ref, grp):
     E, rhs, H = elast
-    s = gs.Solver(prec, smoother=kind, omega=0.7, coarse_steps=2, reference_vcycle=ref)
+    s = gs.Solver(prec, smoother=kind, omega=0.7, coarse_steps=2, reference_vcycle=ref,
+                  **group_opts(grp))
     s.setup(H)
     hv, _ = hier_view(H)
     r = gs.uniform_pm1(5, E.rows)
@@ -98,7 +111,7 @@ def test_vcycle_bitexact_hybrid(elast):
     src/Preconditioners/MultigridHybridBlockPreconditioner.cc:9-61)."""
     E, rhs, H = elast
     s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, coarse_smoother=gs.BLOCK_JACOBI,
-                  omega=0.7, coarse_steps=3)
+                  omega=0.7, coarse_steps=3, group_min_scalar=0, group_min_block=0)
     s.setup(H)
     hv, _ = hier_view(H)
     r = gs.uniform_pm1(6, E.rows)
