@@ -201,6 +201,80 @@ __device__ __forceinline__ void bsr3_row_dot(const Bsr3<T> &A, int64_t b, const 
 }
 
 __device__ __forceinline__ bool stopped(const int32_t *stop) { return stop && *stop; }
+__device__ __forceinline__ int32_t stop_word(const int32_t *stop) { return stop ? *stop : 0; }
+
+// Slice head of a row: the offset of its first slot and the slice width.
+// V-cycle kernels load it together with their other row-independent
+// operands and the stop word, then test the stop word, so the stop test
+// costs no extra memory round trip.
+struct SliceHead {
+  int64_t base;
+  int w;
+};
+template <class T>
+__device__ __forceinline__ SliceHead sell_head(const Sell<T> &A, int64_t row) {
+  return SliceHead{A.slice_ptr[row >> 6] + (row & 63), A.slice_w[row >> 6]};
+}
+template <class T>
+__device__ __forceinline__ SliceHead bsr3_head(const Bsr3<T> &A, int64_t b) {
+  return SliceHead{A.slice_ptr[b >> 6], A.slice_w[b >> 6]};
+}
+template <class T>
+__device__ __forceinline__ T sell_dot_h(const Sell<T> &A, SliceHead h, const T *x) {
+  T sum = T(0);
+#pragma unroll 4
+  for (int j = 0; j < h.w; ++j) {
+    const int64_t e = h.base + static_cast<int64_t>(j) * kSlice;
+    sum += A.val[e] * x[A.col[e]];
+  }
+  return sum;
+}
+template <class T>
+__device__ __forceinline__ T sell_offdiag_h(const Sell<T> &A, SliceHead h, int64_t row,
+                                            const T *x, T &diag) {
+  T sum = T(0);
+  diag = T(0);
+#pragma unroll 4
+  for (int j = 0; j < h.w; ++j) {
+    const int64_t e = h.base + static_cast<int64_t>(j) * kSlice;
+    const int32_t c = A.col[e];
+    const T v = A.val[e];
+    if (c == row) {
+      diag = v;
+    } else {
+      sum += v * x[c];
+    }
+  }
+  return sum;
+}
+template <class T>
+__device__ __forceinline__ void bsr3_dot_h(const Bsr3<T> &A, SliceHead h, int64_t b, const T *x,
+                                           T y[3], bool skip_diag) {
+  const int64_t lane = b & 63;
+  T y0 = T(0), y1 = T(0), y2 = T(0);
+#pragma unroll 2
+  for (int j = 0; j < h.w; ++j) {
+    const int64_t slot = h.base + static_cast<int64_t>(j) * kSlice;
+    const int32_t bc = A.bcol[slot + lane];
+    if (skip_diag && bc == b) continue;
+    const T *v = A.val + slot * 9 + lane;
+    const T x0 = x[3 * static_cast<int64_t>(bc)];
+    const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
+    const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
+    y0 += v[0 * kSlice] * x0;
+    y0 += v[1 * kSlice] * x1;
+    y0 += v[2 * kSlice] * x2;
+    y1 += v[3 * kSlice] * x0;
+    y1 += v[4 * kSlice] * x1;
+    y1 += v[5 * kSlice] * x2;
+    y2 += v[6 * kSlice] * x0;
+    y2 += v[7 * kSlice] * x1;
+    y2 += v[8 * kSlice] * x2;
+  }
+  y[0] = y0;
+  y[1] = y1;
+  y[2] = y2;
+}
 
 // ---------------------------------------------------------------------------
 // SpMV family.
@@ -232,24 +306,28 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_residual(Sell<T> A, const T *__restrict__ rhs,
                                                      const T *__restrict__ x,
                                                      T *__restrict__ res, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   if (row >= A.rows) return;
-  res[row] = rhs[row] - sell_row_dot(A, row, x);
+  const SliceHead h = sell_head(A, row);
+  const T bi = rhs[row];
+  if (stop_word(stop)) return;
+  res[row] = bi - sell_dot_h(A, h, x);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_residual_b3(Bsr3<T> A, const T *__restrict__ rhs,
                                                         const T *__restrict__ x,
                                                         T *__restrict__ res, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t b = xcd_block() * kBlock + threadIdx.x;
   if (b >= A.nb) return;
+  const SliceHead h = bsr3_head(A, b);
+  const T b0 = rhs[3 * b], b1 = rhs[3 * b + 1], b2 = rhs[3 * b + 2];
+  if (stop_word(stop)) return;
   T v[3];
-  bsr3_row_dot(A, b, x, v, false);
-  res[3 * b] = rhs[3 * b] - v[0];
-  res[3 * b + 1] = rhs[3 * b + 1] - v[1];
-  res[3 * b + 2] = rhs[3 * b + 2] - v[2];
+  bsr3_dot_h(A, h, b, x, v, false);
+  res[3 * b] = b0 - v[0];
+  res[3 * b + 1] = b1 - v[1];
+  res[3 * b + 2] = b2 - v[2];
 }
 
 // rhs_c = R res and corr_c = 0 (the coarse correction is zeroed in the same
@@ -258,10 +336,11 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict(Sell<T> R, const T *__restrict__ res,
                                                      T *__restrict__ rhs_c, T *__restrict__ corr_c,
                                                      const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   if (row >= R.rows) return;
-  rhs_c[row] = sell_row_dot(R, row, res);
+  const SliceHead h = sell_head(R, row);
+  if (stop_word(stop)) return;
+  rhs_c[row] = sell_dot_h(R, h, res);
   if (corr_c) corr_c[row] = T(0);
 }
 
@@ -269,11 +348,13 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Sell<T> R, const T *__restr
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__restrict__ e,
                                                         T *__restrict__ x, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   if (row >= P.rows) return;
-  const T c = sell_row_dot(P, row, e);
-  x[row] = x[row] + T(1) * c;
+  const SliceHead h = sell_head(P, row);
+  const T xi = x[row];
+  if (stop_word(stop)) return;
+  const T c = sell_dot_h(P, h, e);
+  x[row] = xi + T(1) * c;
 }
 
 // The reference's coarse-level correction storage: on levels below the
@@ -284,10 +365,11 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__re
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong_alias(Sell<T> P, const T *__restrict__ e,
                                                           T *__restrict__ x, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   if (row >= P.rows) return;
-  const T c = sell_row_dot(P, row, e);
+  const SliceHead h = sell_head(P, row);
+  if (stop_word(stop)) return;
+  const T c = sell_dot_h(P, h, e);
   x[row] = c + T(1) * c;
 }
 
@@ -307,13 +389,15 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, int64_t begin, int64_t end,
                                                      const T *__restrict__ rhs, T *x, T omega,
                                                      int reps, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = begin + xcd_block() * kBlock + threadIdx.x;
   if (row >= end) return;
+  const SliceHead h = sell_head(A, row);
+  const T bi = rhs[row];
+  T xi = x[row];  // not read by the off-diagonal sum
+  if (stop_word(stop)) return;
   T diag;
-  const T off = sell_row_offdiag(A, row, x, diag);
-  const T nv = (rhs[row] - off) / diag;
-  T xi = x[row];
+  const T off = sell_offdiag_h(A, h, row, x, diag);
+  const T nv = (bi - off) / diag;
   xi = (T(1) - omega) * xi + omega * nv;
   if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
   x[row] = xi;
@@ -328,11 +412,16 @@ __global__ __launch_bounds__(kBlock) void k_gs_color_zero(const T *__restrict__ 
                                                           int64_t end0, int64_t n,
                                                           const T *__restrict__ rhs, T *x,
                                                           T omega, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   if (row >= n) return;
+  T bi = T(0), di = T(1);
   if (row < end0) {
-    const T nv = rhs[row] / diag[row];
+    bi = rhs[row];
+    di = diag[row];
+  }
+  if (stop_word(stop)) return;
+  if (row < end0) {
+    const T nv = bi / di;
     x[row] = omega * nv;
   } else {
     x[row] = T(0);
@@ -348,22 +437,30 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color(Bsr3<T> A, const T *_
                                                            int64_t begin, int64_t end,
                                                            const T *__restrict__ rhs, T *x,
                                                            T omega, int reps, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t b = begin + xcd_block() * kBlock + threadIdx.x;
   if (b >= end) return;
+  const SliceHead h = bsr3_head(A, b);
+  T bl[3], xl[3], bi[9];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    bl[l] = rhs[3 * b + l];
+    xl[l] = x[3 * b + l];  // not read by the off-block sum
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) bi[e] = dinv[9 * b + e];
+  if (stop_word(stop)) return;
   T off[3];
-  bsr3_row_dot(A, b, x, off, true);
+  bsr3_dot_h(A, h, b, x, off, true);
   T lr[3];
 #pragma unroll
-  for (int l = 0; l < 3; ++l) lr[l] = rhs[3 * b + l] - off[l];
-  const T *bi = dinv + 9 * b;
+  for (int l = 0; l < 3; ++l) lr[l] = bl[l] - off[l];
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     T nv = T(0);
     nv += bi[3 * l + 0] * lr[0];
     nv += bi[3 * l + 1] * lr[1];
     nv += bi[3 * l + 2] * lr[2];
-    T xi = x[3 * b + l];
+    T xi = xl[l];
     xi = (T(1) - omega) * xi + omega * nv;
     if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
     x[3 * b + l] = xi;
@@ -377,17 +474,20 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color_zero(const T *__restr
                                                                 int64_t end0, int64_t nb,
                                                                 const T *__restrict__ rhs, T *x,
                                                                 T omega, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t b = xcd_block() * kBlock + threadIdx.x;
   if (b >= nb) return;
   if (b >= end0) {
+    if (stop_word(stop)) return;
     x[3 * b] = T(0);
     x[3 * b + 1] = T(0);
     x[3 * b + 2] = T(0);
     return;
   }
-  const T *bi = dinv + 9 * b;
+  T bi[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) bi[e] = dinv[9 * b + e];
   const T r0 = rhs[3 * b], r1 = rhs[3 * b + 1], r2 = rhs[3 * b + 2];
+  if (stop_word(stop)) return;
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     T nv = T(0);
@@ -405,12 +505,14 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Sell<T> A, const T *__restric
                                                    const T *__restrict__ rhs,
                                                    const T *__restrict__ x, T *__restrict__ xo,
                                                    T omega, const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   if (row >= A.rows) return;
-  T c = rhs[row] - sell_row_dot(A, row, x);
-  c = c * dinv[row];
-  xo[row] = x[row] + omega * c;
+  const SliceHead h = sell_head(A, row);
+  const T bi = rhs[row], di = dinv[row], xi = x[row];
+  if (stop_word(stop)) return;
+  T c = bi - sell_dot_h(A, h, x);
+  c = c * di;
+  xo[row] = xi + omega * c;
 }
 
 // 3x3 block Jacobi sweep into a second buffer
@@ -444,15 +546,20 @@ __global__ __launch_bounds__(kBlock) void k_block_jacobi_b3(Bsr3<T> A, const T *
                                                             const T *__restrict__ x,
                                                             T *__restrict__ xo, T omega,
                                                             const int32_t *stop) {
-  if (stopped(stop)) return;
   const int64_t b = xcd_block() * kBlock + threadIdx.x;
   if (b >= A.nb) return;
+  const SliceHead h = bsr3_head(A, b);
+  T bl[3], bi[9];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) bl[l] = rhs[3 * b + l];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) bi[e] = dinv[9 * b + e];
+  if (stop_word(stop)) return;
   T v[3];
-  bsr3_row_dot(A, b, x, v, false);
+  bsr3_dot_h(A, h, b, x, v, false);
   T r[3];
 #pragma unroll
-  for (int l = 0; l < 3; ++l) r[l] = rhs[3 * b + l] - v[l];
-  const T *bi = dinv + 9 * b;
+  for (int l = 0; l < 3; ++l) r[l] = bl[l] - v[l];
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     T s = T(0);

@@ -45,83 +45,136 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Lane 0 of a G-lane group receives the sum over the entries of `row` in
-// CSR order of val * x[col], leaving out entries whose column equals
+// Entries per lane and round for scalar rows: a round covers G * G / 16
+// entries (16, 64, 256), so a long row needs few dependent load rounds.
+template <int G>
+constexpr int grp_unroll() { return G / 16; }
+
+// Adds staged products p[0..cnt) to s in order, leaving out those whose flag
+// is set; the last flagged value goes to diag. Branch-free and unrolled by
+// four, so the LDS reads of a batch are issued before its dependent adds.
+template <class T, int STRIDE>
+__device__ __forceinline__ T serial_add(const T *p, const uint8_t *f, int cnt, T s, T &diag) {
+  int j = 0;
+  for (; j + 4 <= cnt; j += 4) {
+    T q[4];
+    bool e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      q[u] = p[(j + u) * STRIDE];
+      e[u] = f[j + u] != 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T t = s + q[u];
+      s = e[u] ? s : t;
+      diag = e[u] ? q[u] : diag;
+    }
+  }
+  for (; j < cnt; ++j) {
+    const T q = p[j * STRIDE];
+    const bool e = f[j] != 0;
+    const T t = s + q;
+    s = e ? s : t;
+    diag = e ? q : diag;
+  }
+  return s;
+}
+
+// Lane 0 of a G-lane group receives the sum over the entries [rb, re) of a
+// row in CSR order of val * x[col], leaving out entries whose column equals
 // `excl` (-1: none); `diag` receives the value of the last such entry.
-// lp / lf: the group's G product / flag slots in LDS.
+// lp / lf: the group's G * grp_unroll<G>() product / flag slots in LDS.
 template <class T, int G>
-__device__ __forceinline__ T grp_row_dot(const Rows<T> &A, int64_t row, bool active, const T *x,
+__device__ __forceinline__ T grp_row_dot(const Rows<T> &A, int64_t rb, int64_t re, const T *x,
                                          int64_t excl, T &diag, T *lp, uint8_t *lf, int lane) {
+  constexpr int U = grp_unroll<G>();
+  constexpr int C = G * U;
   T s = T(0);
   diag = T(0);
-  int64_t rb = 0, re = 0;
-  if (active) {
-    rb = A.ptr[row];
-    re = A.ptr[row + 1];
-  }
-  for (int64_t base = rb; base < re; base += G) {
-    const int64_t k = base + lane;
-    if (k < re) {
-      const int32_t c = A.col[k];
-      const T v = A.val[k];
-      const bool d = c == excl;
-      lp[lane] = d ? v : v * x[c];
-      lf[lane] = d;
+  for (int64_t base = rb; base < re; base += C) {
+    int32_t c[U];
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = base + lane + u * G;
+      c[u] = 0;
+      v[u] = T(0);
+      if (k < re) {
+        c[u] = A.col[k];
+        v[u] = A.val[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t k = base + lane + u * G;
+      if (k < re) {
+        const bool d = c[u] == excl;
+        const T xv = x[c[u]];
+        lp[lane + u * G] = d ? v[u] : v[u] * xv;
+        lf[lane + u * G] = d;
+      }
     }
     wave_sync();
     if (lane == 0) {
-      const int cnt = static_cast<int>(re - base < G ? re - base : G);
-      for (int j = 0; j < cnt; ++j) {
-        const T p = lp[j];
-        if (lf[j]) diag = p;
-        else s += p;
-      }
+      const int cnt = static_cast<int>(re - base < C ? re - base : C);
+      s = serial_add<T, 1>(lp, lf, cnt, s, diag);
     }
     wave_sync();
   }
   return s;
 }
 
-// Lanes 0, 1, 2 of a G-lane group receive the three line sums of block row
-// b over its blocks in stored order (columns 0, 1, 2 within a block),
-// leaving out the block in block column `excl` (-1: none). lp: 9 G values,
-// lf: G flags.
+// Lanes 0, 1, 2 of a G-lane group receive the three line sums of a block
+// row (blocks [rb, re)) in stored order (columns 0, 1, 2 within a block),
+// leaving out the block in block column `excl` (-1: none). Products are
+// staged line-major (line l of block slot j at lp[3 G l + 3 j + c]) so
+// each summing lane reads a contiguous run. lp: 9 G values, lf: G flags.
 template <class T, int G>
-__device__ __forceinline__ T grp_brow_dot(const BRows<T> &A, int64_t b, bool active, const T *x,
+__device__ __forceinline__ T grp_brow_dot(const BRows<T> &A, int64_t rb, int64_t re, const T *x,
                                           int64_t excl, T *lp, uint8_t *lf, int lane) {
   T s = T(0);
-  int64_t rb = 0, re = 0;
-  if (active) {
-    rb = A.ptr[b];
-    re = A.ptr[b + 1];
-  }
   for (int64_t base = rb; base < re; base += G) {
     const int64_t k = base + lane;
     if (k < re) {
       const int32_t bc = A.bcol[k];
-      const bool d = bc == excl;
-      lf[lane] = d;
-      if (!d) {
-        const T *v = A.val + 9 * k;
-        const T x0 = x[3 * static_cast<int64_t>(bc)];
-        const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
-        const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
+      const T *v = A.val + 9 * k;
+      const T x0 = x[3 * static_cast<int64_t>(bc)];
+      const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
+      const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
 #pragma unroll
-        for (int l = 0; l < 3; ++l) {
-          lp[9 * lane + 3 * l + 0] = v[3 * l + 0] * x0;
-          lp[9 * lane + 3 * l + 1] = v[3 * l + 1] * x1;
-          lp[9 * lane + 3 * l + 2] = v[3 * l + 2] * x2;
-        }
+      for (int l = 0; l < 3; ++l) {
+        lp[3 * G * l + 3 * lane + 0] = v[3 * l + 0] * x0;
+        lp[3 * G * l + 3 * lane + 1] = v[3 * l + 1] * x1;
+        lp[3 * G * l + 3 * lane + 2] = v[3 * l + 2] * x2;
       }
+      lf[lane] = bc == excl;
     }
     wave_sync();
     if (lane < 3) {
       const int cnt = static_cast<int>(re - base < G ? re - base : G);
-      for (int j = 0; j < cnt; ++j) {
-        if (lf[j]) continue;
-        s += lp[9 * j + 3 * lane + 0];
-        s += lp[9 * j + 3 * lane + 1];
-        s += lp[9 * j + 3 * lane + 2];
+      const T *p = lp + 3 * G * lane;
+      int j = 0;
+      for (; j + 2 <= cnt; j += 2) {
+        T q[6];
+        bool e[2];
+#pragma unroll
+        for (int u = 0; u < 6; ++u) q[u] = p[3 * j + u];
+        e[0] = lf[j] != 0;
+        e[1] = lf[j + 1] != 0;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          T t = s + q[3 * u];
+          t = t + q[3 * u + 1];
+          t = t + q[3 * u + 2];
+          s = e[u] ? s : t;
+        }
+      }
+      for (; j < cnt; ++j) {
+        T t = s + p[3 * j];
+        t = t + p[3 * j + 1];
+        t = t + p[3 * j + 2];
+        s = lf[j] ? s : t;
       }
     }
     wave_sync();
@@ -129,7 +182,7 @@ __device__ __forceinline__ T grp_brow_dot(const BRows<T> &A, int64_t b, bool act
   return s;
 }
 
-// Row index handled by this thread's group, and the group's lane.
+// Row index handled by this thread's group.
 template <int G>
 __device__ __forceinline__ int64_t grp_row(int64_t begin) {
   return begin + xcd_block() * (kBlock / G) + threadIdx.x / G;
@@ -141,6 +194,24 @@ inline uint32_t grid_rows(int64_t rows) {
   return static_cast<uint32_t>((rows + R - 1) / R);
 }
 
+// Each kernel below first issues every load that does not depend on the
+// row's entries (row bounds, right-hand side, diagonal data, the stop
+// word), so they share one memory round trip, and only then tests the stop
+// word.
+#define GSK_GROUP_SCALAR_LDS(G)                           \
+  __shared__ T lp[kBlock * grp_unroll<G>()];              \
+  __shared__ uint8_t lf[kBlock * grp_unroll<G>()];        \
+  const int lane = threadIdx.x % G;                       \
+  const int g0 = (threadIdx.x - lane) * grp_unroll<G>()
+
+#define GSK_GROUP_BLOCK_LDS(G)            \
+  __shared__ T lp[9 * kBlock];            \
+  __shared__ uint8_t lf[kBlock];          \
+  __shared__ T lr[3 * (kBlock / G)];      \
+  const int lane = threadIdx.x % G;       \
+  const int g0 = threadIdx.x - lane;      \
+  T *const glr = lr + 3 * (threadIdx.x / G)
+
 // ---------------------------------------------------------------------------
 // Scalar rows.
 // ---------------------------------------------------------------------------
@@ -148,15 +219,20 @@ template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_residual_w(Rows<T> A, const T *__restrict__ rhs,
                                                        const T *__restrict__ x,
                                                        T *__restrict__ res, const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[kBlock];
-  __shared__ uint8_t lf[kBlock];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(0);
   const bool active = row < A.rows;
+  int64_t rb = 0, re = 0;
+  T bi = T(0);
+  if (active) {
+    rb = A.ptr[row];
+    re = A.ptr[row + 1];
+    bi = rhs[row];
+  }
+  if (stop_word(stop)) return;
   T diag;
-  const T s = grp_row_dot<T, G>(A, row, active, x, -1, diag, lp + g0, lf + g0, lane);
-  if (active && lane == 0) res[row] = rhs[row] - s;
+  const T s = grp_row_dot<T, G>(A, rb, re, x, -1, diag, lp + g0, lf + g0, lane);
+  if (active && lane == 0) res[row] = bi - s;
 }
 
 // rhs_c = R res, corr_c = 0 (as k_restrict).
@@ -164,14 +240,17 @@ template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_restrict_w(Rows<T> R, const T *__restrict__ res,
                                                        T *__restrict__ rhs_c, T *__restrict__ corr_c,
                                                        const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[kBlock];
-  __shared__ uint8_t lf[kBlock];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(0);
   const bool active = row < R.rows;
+  int64_t rb = 0, re = 0;
+  if (active) {
+    rb = R.ptr[row];
+    re = R.ptr[row + 1];
+  }
+  if (stop_word(stop)) return;
   T diag;
-  const T s = grp_row_dot<T, G>(R, row, active, res, -1, diag, lp + g0, lf + g0, lane);
+  const T s = grp_row_dot<T, G>(R, rb, re, res, -1, diag, lp + g0, lf + g0, lane);
   if (active && lane == 0) {
     rhs_c[row] = s;
     if (corr_c) corr_c[row] = T(0);
@@ -184,33 +263,45 @@ template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_prolong_w(Rows<T> P, const T *__restrict__ e,
                                                       T *__restrict__ x, int alias,
                                                       const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[kBlock];
-  __shared__ uint8_t lf[kBlock];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(0);
   const bool active = row < P.rows;
+  int64_t rb = 0, re = 0;
+  T xi = T(0);
+  if (active) {
+    rb = P.ptr[row];
+    re = P.ptr[row + 1];
+    if (!alias) xi = x[row];
+  }
+  if (stop_word(stop)) return;
   T diag;
-  const T c = grp_row_dot<T, G>(P, row, active, e, -1, diag, lp + g0, lf + g0, lane);
-  if (active && lane == 0) x[row] = alias ? c + T(1) * c : x[row] + T(1) * c;
+  const T c = grp_row_dot<T, G>(P, rb, re, e, -1, diag, lp + g0, lf + g0, lane);
+  if (active && lane == 0) x[row] = alias ? c + T(1) * c : xi + T(1) * c;
 }
 
-// One colour of multicolour Gauss-Seidel (as k_gs_color).
+// One colour of multicolour Gauss-Seidel (as k_gs_color). x[row] itself is
+// not read by the row sum (the diagonal entry is left out), so it can be
+// loaded up front.
 template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_gs_color_w(Rows<T> A, int64_t begin, int64_t end,
                                                        const T *__restrict__ rhs, T *x, T omega,
                                                        int reps, const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[kBlock];
-  __shared__ uint8_t lf[kBlock];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(begin);
   const bool active = row < end;
+  int64_t rb = 0, re = 0;
+  T bi = T(0), xi = T(0);
+  if (active) {
+    rb = A.ptr[row];
+    re = A.ptr[row + 1];
+    bi = rhs[row];
+    xi = x[row];
+  }
+  if (stop_word(stop)) return;
   T diag;
-  const T off = grp_row_dot<T, G>(A, row, active, x, row, diag, lp + g0, lf + g0, lane);
+  const T off = grp_row_dot<T, G>(A, rb, re, x, row, diag, lp + g0, lf + g0, lane);
   if (active && lane == 0) {
-    const T nv = (rhs[row] - off) / diag;
-    T xi = x[row];
+    const T nv = (bi - off) / diag;
     xi = (T(1) - omega) * xi + omega * nv;
     if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
     x[row] = xi;
@@ -223,18 +314,25 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_w(Rows<T> A, const T *__restr
                                                      const T *__restrict__ rhs,
                                                      const T *__restrict__ x, T *__restrict__ xo,
                                                      T omega, const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[kBlock];
-  __shared__ uint8_t lf[kBlock];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(0);
   const bool active = row < A.rows;
+  int64_t rb = 0, re = 0;
+  T bi = T(0), di = T(0), xi = T(0);
+  if (active) {
+    rb = A.ptr[row];
+    re = A.ptr[row + 1];
+    bi = rhs[row];
+    di = dinv[row];
+    xi = x[row];
+  }
+  if (stop_word(stop)) return;
   T diag;
-  const T s = grp_row_dot<T, G>(A, row, active, x, -1, diag, lp + g0, lf + g0, lane);
+  const T s = grp_row_dot<T, G>(A, rb, re, x, -1, diag, lp + g0, lf + g0, lane);
   if (active && lane == 0) {
-    T c = rhs[row] - s;
-    c = c * dinv[row];
-    xo[row] = x[row] + omega * c;
+    T c = bi - s;
+    c = c * di;
+    xo[row] = xi + omega * c;
   }
 }
 
@@ -245,30 +343,45 @@ template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_residual_bw(BRows<T> A, const T *__restrict__ rhs,
                                                         const T *__restrict__ x,
                                                         T *__restrict__ res, const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[9 * kBlock];
-  __shared__ uint8_t lf[kBlock];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_BLOCK_LDS(G);
+  (void)glr;
   const int64_t b = grp_row<G>(0);
   const bool active = b < A.nb;
-  const T s = grp_brow_dot<T, G>(A, b, active, x, -1, lp + 9 * g0, lf + g0, lane);
-  if (active && lane < 3) res[3 * b + lane] = rhs[3 * b + lane] - s;
+  int64_t rb = 0, re = 0;
+  T bl = T(0);
+  if (active) {
+    rb = A.ptr[b];
+    re = A.ptr[b + 1];
+    if (lane < 3) bl = rhs[3 * b + lane];
+  }
+  if (stop_word(stop)) return;
+  const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lf + g0, lane);
+  if (active && lane < 3) res[3 * b + lane] = bl - s;
 }
 
-// Applies x_b update of block GS / block Jacobi from the three line
-// residuals held by lanes 0..2 of the group: nv_l = sum_c Dinv_b[l][c] r_c.
+// nv_l = sum_c Dinv_b[l][c] r_c for lane l < 3 of the group, from the three
+// line residuals held by lanes 0..2 (d: this lane's row of Dinv_b).
 template <class T>
-__device__ __forceinline__ T block_apply(const T *__restrict__ bi, bool active, T rl, T *lr,
-                                         int lane) {
+__device__ __forceinline__ T block_apply(const T d[3], bool active, T rl, T *lr, int lane) {
   if (lane < 3) lr[lane] = rl;
   wave_sync();
   T nv = T(0);
   if (active && lane < 3) {
-    nv += bi[3 * lane + 0] * lr[0];
-    nv += bi[3 * lane + 1] * lr[1];
-    nv += bi[3 * lane + 2] * lr[2];
+    nv += d[0] * lr[0];
+    nv += d[1] * lr[1];
+    nv += d[2] * lr[2];
   }
   return nv;
+}
+
+// Loads row `lane` (< 3) of block b's inverse diagonal block.
+template <class T>
+__device__ __forceinline__ void load_dinv_row(const T *__restrict__ dinv, int64_t b, int lane,
+                                              T d[3]) {
+  const T *bi = dinv + 9 * b + 3 * lane;
+  d[0] = bi[0];
+  d[1] = bi[1];
+  d[2] = bi[2];
 }
 
 // One colour of 3x3 block Gauss-Seidel over block rows [begin, end)
@@ -279,21 +392,27 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color_bw(BRows<T> A, const 
                                                               const T *__restrict__ rhs, T *x,
                                                               T omega, int reps,
                                                               const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[9 * kBlock];
-  __shared__ uint8_t lf[kBlock];
-  __shared__ T lr[3 * (kBlock / G)];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_BLOCK_LDS(G);
   const int64_t b = grp_row<G>(begin);
   const bool active = b < end;
-  const T off = grp_brow_dot<T, G>(A, b, active, x, b, lp + 9 * g0, lf + g0, lane);
-  const T rl = (active && lane < 3) ? rhs[3 * b + lane] - off : T(0);
-  const T nv = block_apply(dinv + 9 * b, active, rl, lr + 3 * (threadIdx.x / G), lane);
+  int64_t rb = 0, re = 0;
+  T bl = T(0), xl = T(0), d[3] = {T(0), T(0), T(0)};
+  if (active) {
+    rb = A.ptr[b];
+    re = A.ptr[b + 1];
+    if (lane < 3) {
+      bl = rhs[3 * b + lane];
+      xl = x[3 * b + lane];
+      load_dinv_row(dinv, b, lane, d);
+    }
+  }
+  if (stop_word(stop)) return;
+  const T off = grp_brow_dot<T, G>(A, rb, re, x, b, lp + 9 * g0, lf + g0, lane);
+  const T nv = block_apply(d, active, active && lane < 3 ? bl - off : T(0), glr, lane);
   if (active && lane < 3) {
-    T xi = x[3 * b + lane];
-    xi = (T(1) - omega) * xi + omega * nv;
-    if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
-    x[3 * b + lane] = xi;
+    xl = (T(1) - omega) * xl + omega * nv;
+    if (reps > 1) xl = (T(1) - omega) * xl + omega * nv;
+    x[3 * b + lane] = xl;
   }
 }
 
@@ -304,17 +423,27 @@ __global__ __launch_bounds__(kBlock) void k_block_jacobi_bw(BRows<T> A, const T 
                                                             const T *__restrict__ x,
                                                             T *__restrict__ xo, T omega,
                                                             const int32_t *stop) {
-  if (stopped(stop)) return;
-  __shared__ T lp[9 * kBlock];
-  __shared__ uint8_t lf[kBlock];
-  __shared__ T lr[3 * (kBlock / G)];
-  const int lane = threadIdx.x % G, g0 = threadIdx.x - lane;
+  GSK_GROUP_BLOCK_LDS(G);
   const int64_t b = grp_row<G>(0);
   const bool active = b < A.nb;
-  const T s = grp_brow_dot<T, G>(A, b, active, x, -1, lp + 9 * g0, lf + g0, lane);
-  const T rl = (active && lane < 3) ? rhs[3 * b + lane] - s : T(0);
-  const T v = block_apply(dinv + 9 * b, active, rl, lr + 3 * (threadIdx.x / G), lane);
-  if (active && lane < 3) xo[3 * b + lane] = x[3 * b + lane] + omega * v;
+  int64_t rb = 0, re = 0;
+  T bl = T(0), xl = T(0), d[3] = {T(0), T(0), T(0)};
+  if (active) {
+    rb = A.ptr[b];
+    re = A.ptr[b + 1];
+    if (lane < 3) {
+      bl = rhs[3 * b + lane];
+      xl = x[3 * b + lane];
+      load_dinv_row(dinv, b, lane, d);
+    }
+  }
+  if (stop_word(stop)) return;
+  const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lf + g0, lane);
+  const T v = block_apply(d, active, active && lane < 3 ? bl - s : T(0), glr, lane);
+  if (active && lane < 3) xo[3 * b + lane] = xl + omega * v;
 }
+
+#undef GSK_GROUP_SCALAR_LDS
+#undef GSK_GROUP_BLOCK_LDS
 
 }  // namespace gsk
