@@ -242,6 +242,16 @@ int gs_inverse_diag_blocks3(gs_csr_h A, double *inv) {
   });
 }
 
+int gs_dense_spd_inverse(int64_t n, double *a) {
+  return guard([&] {
+    if (n < 0) throw std::runtime_error("dense SPD inverse: negative size.");
+    if (n > 0) need(a, "matrix");
+    std::vector<double> m(a, a + n * n);
+    gsx::spd_inverse(n, m);
+    std::memcpy(a, m.data(), m.size() * sizeof(double));
+  });
+}
+
 int gs_problem_poisson2d(int32_t nx, int32_t ny, gs_csr_h *out) {
   return guard([&] {
     auto h = std::make_unique<gs_csr_s>();

@@ -59,6 +59,7 @@ _SIGS = {
     "gs_block_graph": (C.c_int, [_vp, _i32, _vp, _vp, C.POINTER(_i64)]),
     "gs_expand_block_permutation": (C.c_int, [_i64, _i32, _i32p, _i32p, _i32p]),
     "gs_inverse_diag_blocks3": (C.c_int, [_vp, _f64p]),
+    "gs_dense_spd_inverse": (C.c_int, [_i64, _f64p]),
     "gs_problem_poisson2d": (C.c_int, [_i32, _i32, C.POINTER(_vp)]),
     "gs_problem_poisson3d": (C.c_int, [_i32, _i32, _i32, C.POINTER(_vp)]),
     "gs_problem_varcoef3d": (C.c_int, [_i32, C.c_uint64, C.c_double, C.c_double, C.POINTER(_vp)]),
@@ -359,6 +360,17 @@ def inverse_diag_blocks3(A: Csr) -> np.ndarray:
     out = np.empty(9 * (A.rows // 3), np.float64)
     check(lib.gs_inverse_diag_blocks3(h.h, out))
     return out
+
+
+def dense_spd_inverse(a: np.ndarray) -> np.ndarray:
+    """FP64 inverse of a dense SPD matrix through its Cholesky factor (the
+    mixed mode's coarsest-level direct solve)."""
+    m = np.ascontiguousarray(a, np.float64).copy()
+    n = m.shape[0]
+    if m.shape != (n, n):
+        raise ValueError("square matrix expected")
+    check(lib.gs_dense_spd_inverse(n, m.reshape(-1)))
+    return m
 
 
 # ---------------------------------------------------------------------------

@@ -95,6 +95,12 @@ int gs_expand_block_permutation(int64_t nodes, int32_t block_size, const int32_t
                                 int32_t *old_to_new, int32_t *new_to_old);
 /* extractInverseDiagonalBlocks3x3 (src/BlockUtils.cc:218-277), 9 per block. */
 int gs_inverse_diag_blocks3(gs_csr_h A, double *inverse_blocks);
+/* In-place FP64 inverse of a dense SPD matrix (row-major n x n) through its
+ * Cholesky factor: the coarsest-level direct solve of the mixed mode
+ * (GS_PREC_MIXED, "coarse_direct"), which replaces the reference's coarsest
+ * smoother sweeps (src/Smoothers/MultigridSmoother.cc:348-354). Fails when
+ * the matrix is not SPD. */
+int gs_dense_spd_inverse(int64_t n, double *a);
 
 /* ------------------------------------------------------------------ */
 /* Problem generators (replace the reference's mesh inputs).           */

@@ -467,6 +467,12 @@ static CsrT<T> make_csr(int64_t rows, int64_t cols, const int32_t *ptr,
   return A;
 }
 
+extern "C" int orc_dense_spd_inverse(int64_t n, double *a) {
+  ORC_TRY(std::vector<double> m(a, a + n * n);
+          if (!dense_spd_inverse(n, m)) throw std::runtime_error("matrix is not SPD");
+          std::copy(m.begin(), m.end(), a));
+}
+
 extern "C" int orc_spmv_d(int64_t rows, int64_t cols, const int32_t *ptr, const int32_t *col,
                const double *val, const double *x, double *y) {
   ORC_TRY(spmv(make_csr(rows, cols, ptr, col, val), x, y));

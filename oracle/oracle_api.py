@@ -40,6 +40,7 @@ def _lib() -> C.CDLL:
         "orc_set_parallel_dot": (None, [C.c_int]),
         "orc_num_threads": (C.c_int, []),
         "orc_spmv_d": (C.c_int, [_i64, _i64, _i32p, _i32p, _f64p, _f64p, _f64p]),
+        "orc_dense_spd_inverse": (C.c_int, [_i64, _f64p]),
         "orc_spmv_f": (C.c_int, [_i64, _i64, _i32p, _i32p, _f32p, _f32p, _f32p]),
         "orc_gs_color_d": (C.c_int, [_i64, _i32p, _i32p, _f64p, _i32p, _i64, _i64, _f64p, _f64p,
                                      C.c_double]),
@@ -83,6 +84,13 @@ lib = _lib()
 def _check(rc: int) -> None:
     if rc != 0:
         raise RuntimeError(lib.orc_last_error().decode())
+
+
+def dense_spd_inverse(a):
+    """FP64 inverse of a dense SPD matrix through its Cholesky factor."""
+    m = np.ascontiguousarray(a, np.float64).copy()
+    _check(lib.orc_dense_spd_inverse(m.shape[0], m.reshape(-1)))
+    return m
 
 
 def spmv(ptr, col, val, x, ncols=None):

@@ -69,3 +69,20 @@ def test_unknown_option_rejected():
         s.set("nonsense", 1)
     with pytest.raises(gs.GpuSolverError, match="unsupported smoother type"):
         s.set("smoother", 9)
+
+
+def test_dense_spd_inverse_matches_numpy_and_oracle():
+    """The product's coarsest-level inverse (host setup) against numpy and
+    the oracle's independent restatement."""
+    import oracle_api as orc
+    rng = np.random.default_rng(3)
+    for n in (1, 3, 17, 64):
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        a = (q * rng.uniform(0.5, 20.0, n)) @ q.T
+        inv = gs.dense_spd_inverse(a)
+        ref = np.linalg.inv(a)
+        assert np.max(np.abs(inv - ref)) <= 1e-12 * np.max(np.abs(ref))
+        assert np.max(np.abs(inv - orc.dense_spd_inverse(a))) <= 1e-12 * np.max(np.abs(ref))
+        assert np.array_equal(inv, inv.T)
+    with pytest.raises(gs.GpuSolverError):
+        gs.dense_spd_inverse(np.array([[0.0, 1.0], [1.0, 0.0]]))

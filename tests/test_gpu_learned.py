@@ -75,12 +75,16 @@ def test_guarded_solve_falls_back_on_bad_weights():
 
 
 def test_learned_weights_gain_over_plain():
-    """Held-out seeds: the trained network's weights need no more modelled
-    work (iterations x per-iteration bytes) than the plain weights."""
+    """Held-out seeds (never labelled): with the trained network's weights
+    the mixed solve is, on average, no slower than with the plain weights
+    (device time, 3% allowance for timing noise), and every learned solve
+    still reaches 1e-8 in FP64."""
     import learned
     if not learned.MODEL_PATH.exists():
         pytest.skip("no trained weights committed yet")
     doc = json.loads(learned.MODEL_PATH.read_text())
     for kind in doc:
         r = learned.evaluate(kind, range(200, 203), doc[kind]["size"], learned.MODEL_PATH)
-        assert r["mean_ratio_plain_over_learned"]["work"] >= 1.0, r
+        assert r["mean_ratio_plain_over_learned"]["ms"] >= 0.97, r
+        for row in r["per_seed"]:
+            assert row["learned"]["true_relres"] <= 1.5e-8, row

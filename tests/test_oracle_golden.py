@@ -285,3 +285,33 @@ def test_reference_vcycle_aliasing_only_below_the_finest_level():
     Mu = orc.vcycle(hv, u, reference_vcycle=True)
     Mv = orc.vcycle(hv, v, reference_vcycle=True)
     assert abs(u @ Mv - v @ Mu) <= 1e-11 * abs(u @ Mv)
+
+
+# ---------------------------------------------------------------------------
+# Coarsest-level direct solve: dense SPD inverse through Cholesky.
+# ---------------------------------------------------------------------------
+def _spd(n, seed, lo=1.0, hi=10.0):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    return (q * rng.uniform(lo, hi, n)) @ q.T
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 40])
+def test_oracle_dense_spd_inverse_matches_numpy(n):
+    a = _spd(n, n)
+    inv = orc.dense_spd_inverse(a)
+    ref = np.linalg.inv(a)
+    assert np.max(np.abs(inv - ref)) <= 1e-12 * np.max(np.abs(ref))
+    np.testing.assert_allclose(a @ inv, np.eye(n), atol=1e-12)
+
+
+def test_oracle_dense_spd_inverse_closed_forms():
+    assert orc.dense_spd_inverse(np.array([[4.0]]))[0, 0] == 0.25
+    a = np.array([[2.0, 1.0], [1.0, 3.0]])  # inverse = [[3, -1], [-1, 2]] / 5
+    np.testing.assert_allclose(orc.dense_spd_inverse(a), np.array([[3, -1], [-1, 2]]) / 5.0,
+                               rtol=1e-15, atol=1e-16)
+
+
+def test_oracle_dense_spd_inverse_rejects_indefinite():
+    with pytest.raises(RuntimeError):
+        orc.dense_spd_inverse(np.array([[1.0, 2.0], [2.0, 1.0]]))
