@@ -466,14 +466,15 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
                 stop);
           else
             gsk::k_gs_color_w<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
-                lv.A.w.view(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+                lv.A.w.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
+                stop);
         });
       } else if (block) {
         gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
             lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
       } else {
         gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-            lv.A.s.view(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+            lv.A.s.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
       }
     }
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {

@@ -143,29 +143,6 @@ __device__ __forceinline__ T sell_row_dot(const Sell<T> &A, int64_t row, const T
   return sum;
 }
 
-// Off-diagonal sum and diagonal of one row (rowDotExcluding with
-// exclusion [row, row + 1), src/CPU/CPUBackend.cc:62-86).
-template <class T>
-__device__ __forceinline__ T sell_row_offdiag(const Sell<T> &A, int64_t row, const T *x, T &diag) {
-  const int64_t s = row >> 6;
-  const int64_t base = A.slice_ptr[s] + (row & 63);
-  const int w = A.slice_w[s];
-  T sum = T(0);
-  diag = T(0);
-#pragma unroll 4
-  for (int j = 0; j < w; ++j) {
-    const int64_t e = base + static_cast<int64_t>(j) * kSlice;
-    const int32_t c = A.col[e];
-    const T v = A.val[e];
-    if (c == row) {
-      diag = v;
-    } else {
-      sum += v * x[c];
-    }
-  }
-  return sum;
-}
-
 // y(3) = A(3 rows of block row b) x, scalar-row order kept: blocks in slot
 // order, within a block columns 0, 1, 2.
 template <class T>
@@ -229,21 +206,17 @@ __device__ __forceinline__ T sell_dot_h(const Sell<T> &A, SliceHead h, const T *
   }
   return sum;
 }
+// Off-diagonal row sum (rowDotExcluding with exclusion [row, row + 1),
+// src/CPU/CPUBackend.cc:62-86).
 template <class T>
 __device__ __forceinline__ T sell_offdiag_h(const Sell<T> &A, SliceHead h, int64_t row,
-                                            const T *x, T &diag) {
+                                            const T *x) {
   T sum = T(0);
-  diag = T(0);
 #pragma unroll 4
   for (int j = 0; j < h.w; ++j) {
     const int64_t e = h.base + static_cast<int64_t>(j) * kSlice;
     const int32_t c = A.col[e];
-    const T v = A.val[e];
-    if (c == row) {
-      diag = v;
-    } else {
-      sum += v * x[c];
-    }
+    if (c != row) sum += A.val[e] * x[c];
   }
   return sum;
 }
@@ -385,19 +358,21 @@ __global__ __launch_bounds__(kBlock) void k_prolong_alias(Sell<T> P, const T *__
 // backward pass) with no other colour in between, so the off-diagonal sum
 // and b_i / a_ii are the same both times and one read of the row serves
 // both updates, with the reference's operations in the reference's order.
+// The diagonal comes from `diag` (the last entry of each row with column ==
+// row, the value the reference's row scan finds).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, int64_t begin, int64_t end,
+__global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, const T *__restrict__ diag,
+                                                     int64_t begin, int64_t end,
                                                      const T *__restrict__ rhs, T *x, T omega,
                                                      int reps, const int32_t *stop) {
   const int64_t row = begin + xcd_block() * kBlock + threadIdx.x;
   if (row >= end) return;
   const SliceHead h = sell_head(A, row);
-  const T bi = rhs[row];
+  const T bi = rhs[row], di = diag[row];
   T xi = x[row];  // not read by the off-diagonal sum
   if (stop_word(stop)) return;
-  T diag;
-  const T off = sell_offdiag_h(A, h, row, x, diag);
-  const T nv = (bi - off) / diag;
+  const T off = sell_offdiag_h(A, h, row, x);
+  const T nv = (bi - off) / di;
   xi = (T(1) - omega) * xi + omega * nv;
   if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
   x[row] = xi;

@@ -49,49 +49,42 @@ __device__ __forceinline__ void wave_sync() {
 // entries (16, 64, 256), so a long row needs few dependent load rounds.
 template <int G>
 constexpr int grp_unroll() { return G / 16; }
+// Blocks per lane and round for block rows: 128 blocks per round at G = 64.
+template <int G>
+constexpr int grp_bunroll() { return G == 64 ? 2 : 1; }
 
-// Adds staged products p[0..cnt) to s in order, leaving out those whose flag
-// is set; the last flagged value goes to diag. Branch-free and unrolled by
-// four, so the LDS reads of a batch are issued before its dependent adds.
-template <class T, int STRIDE>
-__device__ __forceinline__ T serial_add(const T *p, const uint8_t *f, int cnt, T s, T &diag) {
+// s + p[0] + p[1] + ... in this order: a pure chain of adds, unrolled by
+// eight so the LDS reads of a batch issue before its adds.
+template <class T>
+__device__ __forceinline__ T chain_add(const T *p, int cnt, T s) {
   int j = 0;
-  for (; j + 4 <= cnt; j += 4) {
-    T q[4];
-    bool e[4];
+  for (; j + 8 <= cnt; j += 8) {
+    T q[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      q[u] = p[(j + u) * STRIDE];
-      e[u] = f[j + u] != 0;
-    }
+    for (int u = 0; u < 8; ++u) q[u] = p[j + u];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const T t = s + q[u];
-      s = e[u] ? s : t;
-      diag = e[u] ? q[u] : diag;
-    }
+    for (int u = 0; u < 8; ++u) s += q[u];
   }
-  for (; j < cnt; ++j) {
-    const T q = p[j * STRIDE];
-    const bool e = f[j] != 0;
-    const T t = s + q;
-    s = e ? s : t;
-    diag = e ? q : diag;
-  }
+  for (; j < cnt; ++j) s += p[j];
   return s;
 }
 
+// An entry left out of a row sum (the diagonal of a Gauss-Seidel row, the
+// diagonal block of a block row) is staged as -0.0: s + (-0.0) == s for
+// every s, signed zeros included, so the sum is bit-identical to one that
+// skips the entry, without a data-dependent select in the add chain.
+template <class T>
+__device__ __forceinline__ T neg_zero() { return -T(0); }
+
 // Lane 0 of a G-lane group receives the sum over the entries [rb, re) of a
 // row in CSR order of val * x[col], leaving out entries whose column equals
-// `excl` (-1: none); `diag` receives the value of the last such entry.
-// lp / lf: the group's G * grp_unroll<G>() product / flag slots in LDS.
+// `excl` (-1: none). lp: the group's G * grp_unroll<G>() LDS slots.
 template <class T, int G>
 __device__ __forceinline__ T grp_row_dot(const Rows<T> &A, int64_t rb, int64_t re, const T *x,
-                                         int64_t excl, T &diag, T *lp, uint8_t *lf, int lane) {
+                                         int64_t excl, T *lp, int lane) {
   constexpr int U = grp_unroll<G>();
   constexpr int C = G * U;
   T s = T(0);
-  diag = T(0);
   for (int64_t base = rb; base < re; base += C) {
     int32_t c[U];
     T v[U];
@@ -109,17 +102,12 @@ __device__ __forceinline__ T grp_row_dot(const Rows<T> &A, int64_t rb, int64_t r
     for (int u = 0; u < U; ++u) {
       const int64_t k = base + lane + u * G;
       if (k < re) {
-        const bool d = c[u] == excl;
         const T xv = x[c[u]];
-        lp[lane + u * G] = d ? v[u] : v[u] * xv;
-        lf[lane + u * G] = d;
+        lp[lane + u * G] = c[u] == excl ? neg_zero<T>() : v[u] * xv;
       }
     }
     wave_sync();
-    if (lane == 0) {
-      const int cnt = static_cast<int>(re - base < C ? re - base : C);
-      s = serial_add<T, 1>(lp, lf, cnt, s, diag);
-    }
+    if (lane == 0) s = chain_add(lp, static_cast<int>(re - base < C ? re - base : C), s);
     wave_sync();
   }
   return s;
@@ -127,55 +115,40 @@ __device__ __forceinline__ T grp_row_dot(const Rows<T> &A, int64_t rb, int64_t r
 
 // Lanes 0, 1, 2 of a G-lane group receive the three line sums of a block
 // row (blocks [rb, re)) in stored order (columns 0, 1, 2 within a block),
-// leaving out the block in block column `excl` (-1: none). Products are
-// staged line-major (line l of block slot j at lp[3 G l + 3 j + c]) so
-// each summing lane reads a contiguous run. lp: 9 G values, lf: G flags.
+// leaving out the block in block column `excl` (-1: none). A round covers
+// C = G * grp_bunroll<G>() blocks; products are staged line-major (line l
+// of round slot j at lp[3 C l + 3 j + c]) so each summing lane adds one
+// contiguous run. lp: 9 C values.
 template <class T, int G>
 __device__ __forceinline__ T grp_brow_dot(const BRows<T> &A, int64_t rb, int64_t re, const T *x,
-                                          int64_t excl, T *lp, uint8_t *lf, int lane) {
+                                          int64_t excl, T *lp, int lane) {
+  constexpr int U = grp_bunroll<G>();
+  constexpr int C = G * U;
   T s = T(0);
-  for (int64_t base = rb; base < re; base += G) {
-    const int64_t k = base + lane;
-    if (k < re) {
-      const int32_t bc = A.bcol[k];
-      const T *v = A.val + 9 * k;
-      const T x0 = x[3 * static_cast<int64_t>(bc)];
-      const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
-      const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
+  for (int64_t base = rb; base < re; base += C) {
 #pragma unroll
-      for (int l = 0; l < 3; ++l) {
-        lp[3 * G * l + 3 * lane + 0] = v[3 * l + 0] * x0;
-        lp[3 * G * l + 3 * lane + 1] = v[3 * l + 1] * x1;
-        lp[3 * G * l + 3 * lane + 2] = v[3 * l + 2] * x2;
+    for (int u = 0; u < U; ++u) {
+      const int j = lane + u * G;
+      const int64_t k = base + j;
+      if (k < re) {
+        const int32_t bc = A.bcol[k];
+        const T *v = A.val + 9 * k;
+        const T x0 = x[3 * static_cast<int64_t>(bc)];
+        const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
+        const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
+        const bool d = bc == excl;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          lp[3 * C * l + 3 * j + 0] = d ? neg_zero<T>() : v[3 * l + 0] * x0;
+          lp[3 * C * l + 3 * j + 1] = d ? neg_zero<T>() : v[3 * l + 1] * x1;
+          lp[3 * C * l + 3 * j + 2] = d ? neg_zero<T>() : v[3 * l + 2] * x2;
+        }
       }
-      lf[lane] = bc == excl;
     }
     wave_sync();
     if (lane < 3) {
-      const int cnt = static_cast<int>(re - base < G ? re - base : G);
-      const T *p = lp + 3 * G * lane;
-      int j = 0;
-      for (; j + 2 <= cnt; j += 2) {
-        T q[6];
-        bool e[2];
-#pragma unroll
-        for (int u = 0; u < 6; ++u) q[u] = p[3 * j + u];
-        e[0] = lf[j] != 0;
-        e[1] = lf[j + 1] != 0;
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          T t = s + q[3 * u];
-          t = t + q[3 * u + 1];
-          t = t + q[3 * u + 2];
-          s = e[u] ? s : t;
-        }
-      }
-      for (; j < cnt; ++j) {
-        T t = s + p[3 * j];
-        t = t + p[3 * j + 1];
-        t = t + p[3 * j + 2];
-        s = lf[j] ? s : t;
-      }
+      const int cnt = static_cast<int>(re - base < C ? re - base : C);
+      s = chain_add(lp + 3 * C * lane, 3 * cnt, s);
     }
     wave_sync();
   }
@@ -198,18 +171,16 @@ inline uint32_t grid_rows(int64_t rows) {
 // row's entries (row bounds, right-hand side, diagonal data, the stop
 // word), so they share one memory round trip, and only then tests the stop
 // word.
-#define GSK_GROUP_SCALAR_LDS(G)                           \
-  __shared__ T lp[kBlock * grp_unroll<G>()];              \
-  __shared__ uint8_t lf[kBlock * grp_unroll<G>()];        \
-  const int lane = threadIdx.x % G;                       \
+#define GSK_GROUP_SCALAR_LDS(G)              \
+  __shared__ T lp[kBlock * grp_unroll<G>()]; \
+  const int lane = threadIdx.x % G;          \
   const int g0 = (threadIdx.x - lane) * grp_unroll<G>()
 
-#define GSK_GROUP_BLOCK_LDS(G)            \
-  __shared__ T lp[9 * kBlock];            \
-  __shared__ uint8_t lf[kBlock];          \
-  __shared__ T lr[3 * (kBlock / G)];      \
-  const int lane = threadIdx.x % G;       \
-  const int g0 = threadIdx.x - lane;      \
+#define GSK_GROUP_BLOCK_LDS(G)                       \
+  __shared__ T lp[9 * kBlock * grp_bunroll<G>()];    \
+  __shared__ T lr[3 * (kBlock / G)];                 \
+  const int lane = threadIdx.x % G;                  \
+  const int g0 = (threadIdx.x - lane) * grp_bunroll<G>(); \
   T *const glr = lr + 3 * (threadIdx.x / G)
 
 // ---------------------------------------------------------------------------
@@ -230,8 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_w(Rows<T> A, const T *__res
     bi = rhs[row];
   }
   if (stop_word(stop)) return;
-  T diag;
-  const T s = grp_row_dot<T, G>(A, rb, re, x, -1, diag, lp + g0, lf + g0, lane);
+  const T s = grp_row_dot<T, G>(A, rb, re, x, -1, lp + g0, lane);
   if (active && lane == 0) res[row] = bi - s;
 }
 
@@ -249,8 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict_w(Rows<T> R, const T *__res
     re = R.ptr[row + 1];
   }
   if (stop_word(stop)) return;
-  T diag;
-  const T s = grp_row_dot<T, G>(R, rb, re, res, -1, diag, lp + g0, lf + g0, lane);
+  const T s = grp_row_dot<T, G>(R, rb, re, res, -1, lp + g0, lane);
   if (active && lane == 0) {
     rhs_c[row] = s;
     if (corr_c) corr_c[row] = T(0);
@@ -274,34 +243,35 @@ __global__ __launch_bounds__(kBlock) void k_prolong_w(Rows<T> P, const T *__rest
     if (!alias) xi = x[row];
   }
   if (stop_word(stop)) return;
-  T diag;
-  const T c = grp_row_dot<T, G>(P, rb, re, e, -1, diag, lp + g0, lf + g0, lane);
+  const T c = grp_row_dot<T, G>(P, rb, re, e, -1, lp + g0, lane);
   if (active && lane == 0) x[row] = alias ? c + T(1) * c : xi + T(1) * c;
 }
 
-// One colour of multicolour Gauss-Seidel (as k_gs_color). x[row] itself is
-// not read by the row sum (the diagonal entry is left out), so it can be
-// loaded up front.
+// One colour of multicolour Gauss-Seidel (as k_gs_color). The diagonal
+// comes from `diag` (the level's diagonal, the last entry of each row with
+// column == row, as the reference's row scan finds it), and x[row] is not
+// read by the off-diagonal sum, so both load up front.
 template <class T, int G>
-__global__ __launch_bounds__(kBlock) void k_gs_color_w(Rows<T> A, int64_t begin, int64_t end,
+__global__ __launch_bounds__(kBlock) void k_gs_color_w(Rows<T> A, const T *__restrict__ diag,
+                                                       int64_t begin, int64_t end,
                                                        const T *__restrict__ rhs, T *x, T omega,
                                                        int reps, const int32_t *stop) {
   GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(begin);
   const bool active = row < end;
   int64_t rb = 0, re = 0;
-  T bi = T(0), xi = T(0);
+  T bi = T(0), xi = T(0), di = T(1);
   if (active) {
     rb = A.ptr[row];
     re = A.ptr[row + 1];
     bi = rhs[row];
     xi = x[row];
+    di = diag[row];
   }
   if (stop_word(stop)) return;
-  T diag;
-  const T off = grp_row_dot<T, G>(A, rb, re, x, row, diag, lp + g0, lf + g0, lane);
+  const T off = grp_row_dot<T, G>(A, rb, re, x, row, lp + g0, lane);
   if (active && lane == 0) {
-    const T nv = (bi - off) / diag;
+    const T nv = (bi - off) / di;
     xi = (T(1) - omega) * xi + omega * nv;
     if (reps > 1) xi = (T(1) - omega) * xi + omega * nv;
     x[row] = xi;
@@ -327,8 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_w(Rows<T> A, const T *__restr
     xi = x[row];
   }
   if (stop_word(stop)) return;
-  T diag;
-  const T s = grp_row_dot<T, G>(A, rb, re, x, -1, diag, lp + g0, lf + g0, lane);
+  const T s = grp_row_dot<T, G>(A, rb, re, x, -1, lp + g0, lane);
   if (active && lane == 0) {
     T c = bi - s;
     c = c * di;
@@ -355,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_bw(BRows<T> A, const T *__r
     if (lane < 3) bl = rhs[3 * b + lane];
   }
   if (stop_word(stop)) return;
-  const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lf + g0, lane);
+  const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lane);
   if (active && lane < 3) res[3 * b + lane] = bl - s;
 }
 
@@ -407,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color_bw(BRows<T> A, const 
     }
   }
   if (stop_word(stop)) return;
-  const T off = grp_brow_dot<T, G>(A, rb, re, x, b, lp + 9 * g0, lf + g0, lane);
+  const T off = grp_brow_dot<T, G>(A, rb, re, x, b, lp + 9 * g0, lane);
   const T nv = block_apply(d, active, active && lane < 3 ? bl - off : T(0), glr, lane);
   if (active && lane < 3) {
     xl = (T(1) - omega) * xl + omega * nv;
@@ -438,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_block_jacobi_bw(BRows<T> A, const T 
     }
   }
   if (stop_word(stop)) return;
-  const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lf + g0, lane);
+  const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lane);
   const T v = block_apply(d, active, active && lane < 3 ? bl - s : T(0), glr, lane);
   if (active && lane < 3) xo[3 * b + lane] = xl + omega * v;
 }

@@ -106,6 +106,37 @@ def test_vcycle_bitexact_block(elast, prec, kind, ref, grp):
     np.testing.assert_array_equal(z, zo)
 
 
+@pytest.mark.parametrize("grp", [-1.0, 0.0])
+def test_vcycle_signed_zeros_match_oracle(cfg1, elast, grp):
+    """Left-out diagonal entries are added as -0.0 inside the kernels; the
+    results must match the oracle (which skips them) bit for bit, signs of
+    zeros included: residuals with exact zeros, compared as raw bits."""
+    A, H = cfg1
+    r = gs.uniform_pm1(12, A.rows)
+    r[::3] = 0.0
+    r[1::7] = -0.0
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_steps=2, group_min_scalar=grp,
+                  group_min_block=grp)
+    s.setup(H)
+    hv, _ = hier_view(H)
+    z = s.vcycle(r)
+    zo = orc.vcycle(hv, r, kinds=gs.SGS, coarse_steps=2)
+    assert np.array_equal(z.view(np.uint64), zo.view(np.uint64))
+    np.testing.assert_array_equal(s.vcycle(np.zeros(A.rows)).view(np.uint64),
+                                  orc.vcycle(hv, np.zeros(A.rows), kinds=gs.SGS,
+                                             coarse_steps=2).view(np.uint64))
+    E, rhs, HE = elast
+    rb = gs.uniform_pm1(13, E.rows)
+    rb[::4] = 0.0
+    sb = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2,
+                   group_min_scalar=grp, group_min_block=grp)
+    sb.setup(HE)
+    hve, _ = hier_view(HE)
+    zb = sb.vcycle(rb)
+    zbo = orc.vcycle(hve, rb, kinds=gs.BLOCK_GS, omegas=0.7, coarse_steps=2)
+    assert np.array_equal(zb.view(np.uint64), zbo.view(np.uint64))
+
+
 def test_vcycle_bitexact_hybrid(elast):
     """Fine block GS, coarse block Jacobi (MultigridHybridBlockPreconditioner,
     src/Preconditioners/MultigridHybridBlockPreconditioner.cc:9-61)."""
