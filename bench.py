@@ -145,7 +145,10 @@ def main() -> None:
                                true_relres=ip[-1].true_relres,
                                device_ms=float(np.mean([i.solve_ms for i in ip])))
     mixed_kw = dict(sk)
+    # FP64 dense direct solve at the first level within max_direct unknowns
+    # (the inverse is built on the device); deeper levels are not used.
     mixed_kw["coarse_direct"] = 1
+    mixed_kw["direct_truncate"] = 1
     # FP64 throughout with the FP64 coarsest Cholesky: separates the gain of
     # the direct coarsest solve from that of the FP32 V-cycle.
     fd = gs.Solver(gs.PREC_DOUBLE, graph_iters=a.graph_iters, **mixed_kw)
@@ -155,7 +158,7 @@ def main() -> None:
                                 true_relres=idd[-1].true_relres,
                                 device_ms=float(np.mean([i.solve_ms for i in idd])))
     del fd
-    # Mixed: FP32 V-cycle in FP64 CG, FP64 coarsest Cholesky.
+    # Mixed: FP32 V-cycle in FP64 CG, FP64 dense direct coarse solve.
     mixed = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
     mixed.setup(H)
     xm, im, ms_mixed = time_solves(mixed, b, a.steps, a.warmup)

@@ -34,7 +34,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              "-Wno-unknown-pragmas", "-Wno-unused-result"]
 
 HOST_SOURCES = ["host/sparse.cc", "host/ordering.cc", "host/amg.cc", "host/problems.cc"]
-HIP_SOURCES = ["engine.hip", "backend.hip", "capi.cc", "mlp.hip"]
+HIP_SOURCES = ["engine.hip", "backend.hip", "capi.cc", "mlp.hip", "dense.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "dense.hpp"
 #include "engine.hpp"
 #include "host/setup.h"
 
@@ -252,6 +253,29 @@ int gs_dense_spd_inverse(int64_t n, double *a) {
   });
 }
 
+int gs_dense_spd_inverse_device(int64_t n, double *a) {
+  return guard([&] {
+    if (n < 0) throw std::runtime_error("dense SPD inverse: negative size.");
+    if (n == 0) return;
+    need(a, "matrix");
+    std::vector<double> m(a, a + n * n);
+    const int64_t N = gsdense::padded_size(n);
+    hipStream_t st = nullptr;
+    GS_HIP(hipStreamCreate(&st));
+    gsd::DevBuf inv(static_cast<size_t>(N) * N * sizeof(double));
+    try {
+      gsdense::spd_inverse_device(n, m, inv.as<double>(), st);
+      GS_HIP(hipMemcpy2DAsync(a, n * sizeof(double), inv.get(), N * sizeof(double),
+                              n * sizeof(double), n, hipMemcpyDeviceToHost, st));
+      GS_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+      (void)hipStreamDestroy(st);
+      throw;
+    }
+    GS_HIP(hipStreamDestroy(st));
+  });
+}
+
 int gs_problem_poisson2d(int32_t nx, int32_t ny, gs_csr_h *out) {
   return guard([&] {
     auto h = std::make_unique<gs_csr_s>();
@@ -416,6 +440,7 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "cg_block3") o.cg_block3 = v != 0;
     else if (k == "reference_vcycle") o.reference_vcycle = v != 0;
     else if (k == "max_direct") o.max_direct = v;
+    else if (k == "direct_truncate") o.direct_truncate = v != 0;
     else throw std::runtime_error("unknown integer option: " + k);
     if (o.pre < 0 || o.post < 0 || o.coarse_steps < 0 || o.vcycles < 0 || o.graph_iters < 0)
       throw std::runtime_error("option values must be non-negative.");

@@ -1,0 +1,20 @@
+// Device-built dense FP64 SPD inverse (coarse-level direct solve).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace gsdense {
+
+// Row stride of the device inverse: n rounded up to the 64-row pivot block.
+int64_t padded_size(int64_t n);
+
+// Writes A^-1 (row-major, row stride padded_size(n), padding rows/columns
+// identity) to d_inv, which must hold padded_size(n)^2 doubles. `a` is the
+// dense row-major n x n SPD matrix. Throws when a pivot is not positive.
+void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
+                        hipStream_t stream);
+
+}  // namespace gsdense

@@ -17,6 +17,7 @@
 // after convergence are no-ops. Precision is a template pair: CG type TC
 // and V-cycle type TM (double/double plain, double/float mixed,
 // float/float as-reference).
+#include "dense.hpp"
 #include "engine.hpp"
 
 #include <algorithm>
@@ -105,6 +106,7 @@ struct Level {
   std::vector<int64_t> coff;  // colour offsets (rows, or blocks for block GS)
   int ncolors = 0;
   DevBuf diag, dinv, inv;
+  int64_t inv_ld = 0;  // row stride of the dense inverse
   DevBuf rhs_buf, x_buf, res, tmp;
   TM *rhs = nullptr, *x = nullptr;  // bound to own buffers or the CG vectors
   DMat<TM> R, P;                     // transfer to level k + 1
@@ -239,8 +241,9 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     std::vector<double> dense(static_cast<size_t>(lv.n) * lv.n, 0.0);
     for (int64_t r = 0; r < lv.n; ++r)
       for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e) dense[r * lv.n + Ai.col[e]] += Ai.val[e];
-    gsx::spd_inverse(lv.n, dense);
-    lv.inv.upload(dense);
+    lv.inv_ld = gsdense::padded_size(lv.n);
+    lv.inv.alloc(static_cast<size_t>(lv.inv_ld) * lv.inv_ld * sizeof(double));
+    gsdense::spd_inverse_device(lv.n, dense, lv.inv.template as<double>(), stream_);
   }
   // Blocks summed in the level's own (pre-relabelling) column order, the
   // order of the CSR rows the reference walks.
@@ -297,7 +300,8 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
     throw std::runtime_error("MultigridSmoother: invalid number of permutations.");
   if (static_cast<int>(H.P.size()) + 1 != nlev_ || static_cast<int>(H.R.size()) + 1 != nlev_)
     throw std::runtime_error("MultigridSmoother: invalid number of transfers.");
-  for (int k = 0; k < nlev_; ++k) {
+  const int hier_levels = nlev_;
+  for (int k = 0; k < hier_levels; ++k) {
     H.A[k].check();
     if (H.A[k].rows != H.A[k].cols) throw std::runtime_error("CGSolver: matrix must be square.");
     if (static_cast<int64_t>(perms[k].old_to_new.size()) != H.A[k].rows)
@@ -306,6 +310,15 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   }
   if (opt_.precond == kPrecIdentity && !std::is_same<TC, TM>::value)
     throw std::runtime_error("identity preconditioner requires one precision for CG and V-cycle.");
+  // Direct solve at the first level small enough for the dense inverse; the
+  // levels below it are not used.
+  if (opt_.coarse_direct && opt_.direct_truncate) {
+    for (int k = 0; k < hier_levels; ++k)
+      if (H.A[k].rows <= opt_.max_direct) {
+        nlev_ = k + 1;
+        break;
+      }
+  }
   n_ = H.A[0].rows;
   L_.clear();
   L_.resize(nlev_);
@@ -518,7 +531,7 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
   if (k == nlev_ - 1) {
     if (lv.direct) {
       gsk::k_dense_solve<TM><<<static_cast<uint32_t>((lv.n + 3) / 4), kBlock, 0, stream_>>>(
-          lv.inv.template as<double>(), lv.n, lv.rhs, lv.x, stop);
+          lv.inv.template as<double>(), lv.n, lv.inv_ld, lv.rhs, lv.x, stop);
     } else {
       encode_smooth(lv, opt_.coarse_steps, zero_x, stop);
     }

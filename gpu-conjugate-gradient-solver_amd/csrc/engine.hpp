@@ -29,8 +29,11 @@ struct Options {
   // Reproduce the reference's coarse-level correction aliasing
   // (solution = 2 P e below the finest level); off: x += P e.
   bool reference_vcycle = false;
-  // Largest coarsest level accepted for the dense FP64 inverse.
-  int64_t max_direct = 8192;
+  // Largest level accepted for the dense FP64 inverse (built on the device).
+  int64_t max_direct = 16384;
+  // With coarse_direct: solve directly at the first level of at most
+  // max_direct unknowns and leave the levels below it unused.
+  bool direct_truncate = false;
   // Operators whose average row length reaches these (scalar entries per
   // row; 3x3 blocks per block row) run on lane-group kernels (same
   // arithmetic, same order); negative: never.

@@ -561,17 +561,17 @@ __global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ y, T v, int64_t
   if (i < n) y[i] = v;
 }
 
-// Coarsest level: x = A_c^-1 rhs with the FP64 dense inverse from the
-// Cholesky factorisation, one wavefront per row.
+// Coarsest level: x = A_c^-1 rhs with the FP64 dense inverse (row stride
+// ld), one wavefront per row.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dense_solve(const double *__restrict__ inv, int64_t n,
-                                                        const T *__restrict__ rhs,
+                                                        int64_t ld, const T *__restrict__ rhs,
                                                         T *__restrict__ x, const int32_t *stop) {
   if (stopped(stop)) return;
   const int64_t row = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const double *a = inv + row * n;
+  const double *a = inv + row * ld;
   double s = 0.0;
   for (int64_t j = lane; j < n; j += 64) s += a[j] * static_cast<double>(rhs[j]);
   s = wave_sum(s);

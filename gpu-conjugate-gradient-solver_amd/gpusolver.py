@@ -60,6 +60,7 @@ _SIGS = {
     "gs_expand_block_permutation": (C.c_int, [_i64, _i32, _i32p, _i32p, _i32p]),
     "gs_inverse_diag_blocks3": (C.c_int, [_vp, _f64p]),
     "gs_dense_spd_inverse": (C.c_int, [_i64, _f64p]),
+    "gs_dense_spd_inverse_device": (C.c_int, [_i64, _f64p]),
     "gs_problem_poisson2d": (C.c_int, [_i32, _i32, C.POINTER(_vp)]),
     "gs_problem_poisson3d": (C.c_int, [_i32, _i32, _i32, C.POINTER(_vp)]),
     "gs_problem_varcoef3d": (C.c_int, [_i32, C.c_uint64, C.c_double, C.c_double, C.POINTER(_vp)]),
@@ -370,6 +371,16 @@ def dense_spd_inverse(a: np.ndarray) -> np.ndarray:
     if m.shape != (n, n):
         raise ValueError("square matrix expected")
     check(lib.gs_dense_spd_inverse(n, m.reshape(-1)))
+    return m
+
+
+def dense_spd_inverse_device(a: np.ndarray) -> np.ndarray:
+    """The same inverse built on the GPU (blocked Gauss-Jordan, FP64)."""
+    m = np.ascontiguousarray(a, np.float64).copy()
+    n = m.shape[0]
+    if m.shape != (n, n):
+        raise ValueError("square matrix expected")
+    check(lib.gs_dense_spd_inverse_device(n, m.reshape(-1)))
     return m
 
 

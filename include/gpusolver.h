@@ -101,6 +101,10 @@ int gs_inverse_diag_blocks3(gs_csr_h A, double *inverse_blocks);
  * smoother sweeps (src/Smoothers/MultigridSmoother.cc:348-354). Fails when
  * the matrix is not SPD. */
 int gs_dense_spd_inverse(int64_t n, double *a);
+/* The same inverse built on the GPU (blocked Gauss-Jordan elimination in
+ * FP64, the engine's coarse-level setup); a is read and overwritten on the
+ * host. */
+int gs_dense_spd_inverse_device(int64_t n, double *a);
 
 /* ------------------------------------------------------------------ */
 /* Problem generators (replace the reference's mesh inputs).           */
@@ -158,7 +162,10 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     below the finest level the prolongated correction
  *                     overwrites the level's solution and is then added to
  *                     itself); 0 (default): x += P e
- *   "max_direct"      largest coarsest level for "coarse_direct" (8192)
+ *   "max_direct"      largest level solved by the dense FP64 inverse (16384)
+ *   "direct_truncate" with "coarse_direct": solve directly at the first
+ *                     level of at most "max_direct" unknowns, leaving the
+ *                     levels below unused (0)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (12): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed

@@ -297,3 +297,38 @@ def test_zero_rhs_and_nan():
     b[3] = np.nan
     with pytest.raises(gs.GpuSolverError, match="NaN"):
         s.solve(b)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 130, 700])
+def test_device_spd_inverse_matches_numpy(n):
+    """The coarse-level inverse built on the GPU (blocked Gauss-Jordan,
+    64 x 64 pivot blocks) against numpy, across block-size boundaries."""
+    rng = np.random.default_rng(n)
+    q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    a = (q * rng.uniform(1.0, 20.0, n)) @ q.T
+    inv = gs.dense_spd_inverse_device(a)
+    ref = np.linalg.inv(a)
+    assert np.max(np.abs(inv - ref)) <= 1e-12 * np.max(np.abs(ref))
+    with pytest.raises(gs.GpuSolverError):
+        gs.dense_spd_inverse_device(np.array([[1.0, 2.0], [2.0, 1.0]]))
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+def test_pcg_direct_truncate(elast, prec):
+    """coarse_direct + direct_truncate: the direct solve moves up to the first
+    level within max_direct; the oracle runs the same truncated hierarchy."""
+    E, rhs, H = elast
+    nl1 = H.get("A", 1).rows
+    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=0, coarse_direct=1,
+                  direct_truncate=1, max_direct=nl1, cg_block3=1)
+    s.setup(H)
+    d = s.describe()
+    assert len(d["levels"]) == 2 and d["levels"][1]["direct"]
+    A = [H.get("A", k) for k in range(2)]
+    hv = orc.HierView([(a.ptr, a.col, a.val) for a in A],
+                      [(p.ptr, p.col, p.val) for p in [H.get("P", 0)]],
+                      [(r.ptr, r.col, r.val) for r in [H.get("R", 0)]])
+    x, xo, info, true = _pcg_parity(s, hv, E, rhs, prec, kinds=gs.BLOCK_GS, omegas=0.7,
+                                    coarse_steps=0, direct_coarse=True)
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    assert true <= 1.5e-8
