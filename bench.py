@@ -64,6 +64,23 @@ def build_workload(name: str):
     raise SystemExit(f"unknown config {name}")
 
 
+def measured_traffic(config: str, kernel_id: str):
+    """HBM bytes per launch of kernel `kernel_id` (cg_spmv, l0_sweep,
+    l1_sweep) from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor the same
+    counter reads for a copy of known size), or None when none was taken
+    for this workload and kernel."""
+    path = REPO / "profiles" / f"pmc_{config}.md"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic|"):
+            _, name, val = line.split("|")
+            if name.strip() == kernel_id:
+                return float(val)
+    return None
+
+
 def learned_weights(config: str, H):
     """Per-level (omega, sweeps) from the trained smoother model (GPU
     inference), or None when no model has been trained for this smoother."""
@@ -166,6 +183,7 @@ def main() -> None:
                           true_relres=im[-1].true_relres,
                           device_ms=float(np.mean([i.solve_ms for i in im])))
     head = "mixed"
+    ml = None
     lw = learned_weights(a.config, H)
     if lw is not None:
         ml = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
@@ -183,15 +201,34 @@ def main() -> None:
     hm = modes[head]
     sols = {"plain_fp64": xp, "fp64_direct": xd, "mixed": xm}
     if head == "mixed_learned":
-        sols["mixed_learned"] = xl  # noqa: F821
+        sols["mixed_learned"] = xl
     # Error of the headline solution against the plain FP64 GPU solve (both
     # verified by their FP64 true residual).
     err_vs_plain = float(np.linalg.norm(sols[head] - xp) / np.linalg.norm(xp))
 
-    # Dominant kernel: the CG operator SpMV (FP64), HIP events on the
-    # solver's stream.
+    # Hot kernels, HIP events on each solver's own stream: the plain mode's
+    # CG SpMV (FP64), and the headline solver's level-0 and level-1
+    # symmetric Gauss-Seidel sweeps (one colour launch per colour and
+    # direction). The roofline entry is the headline kernel with the largest
+    # time per PCG iteration (sweep time x sweeps per V-cycle).
+    hs = ml if head == "mixed_learned" else mixed
+    hdesc = hs.describe()
     spmv_ms, spmv_bytes = plain.time_kernel(0, 50)
-    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    cands = [{"id": "cg_spmv", "kernel": "CG operator SpMV (FP64, plain mode)", "ms": spmv_ms,
+              "bytes": spmv_bytes, "launches": 1, "per_iteration_ms": spmv_ms}]
+    for lvl in (0, 1):
+        if lvl >= len(hdesc["levels"]) or hdesc["levels"][lvl]["direct"]:
+            continue
+        lv = hdesc["levels"][lvl]
+        sw_ms, sw_bytes = hs.time_kernel(3 + lvl, 20)
+        cands.append({"id": f"l{lvl}_sweep",
+                      "kernel": f"level-{lvl} Gauss-Seidel colour launches ("
+                                f"{'lane groups of %d' % lv['group'] if lv['group'] else 'thread per row'}"
+                                f", FP32, headline mode)",
+                      "ms": sw_ms, "bytes": sw_bytes, "launches": 2 * lv["colors"] - 1,
+                      "per_iteration_ms": sw_ms * (lv["pre"] + lv["post"])})
+    dom = max(cands[1:] or cands, key=lambda c: c["per_iteration_ms"])
+    achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     desc_plain = plain.describe()
 
     cpu = None
@@ -226,9 +263,17 @@ def main() -> None:
         "speedup_vs_plain": round(modes["plain_fp64"]["device_ms"] / value, 3),
         "rel_err_vs_plain_fp64": err_vs_plain,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "CG SpMV (FP64)", "kernel_ms": round(spmv_ms, 5),
-                     "algorithmic_bytes": spmv_bytes},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(a.config, dom["id"]),
+                     "kernel": dom["kernel"], "launches": dom["launches"],
+                     "kernel_ms": round(dom["ms"] / dom["launches"], 5),
+                     "algorithmic_bytes": dom["bytes"] / dom["launches"]},
+        "kernels": [{"kernel": c["kernel"], "achieved_GBs": round(c["bytes"] / c["ms"] / 1e6, 1),
+                     "frac": round(c["bytes"] / c["ms"] / 1e6 / HBM_PEAK_GBS, 4),
+                     "avg_launch_ms": round(c["ms"] / c["launches"], 5),
+                     "launches": c["launches"],
+                     "per_iteration_ms": round(c["per_iteration_ms"], 4),
+                     "traffic": measured_traffic(a.config, c["id"])} for c in cands],
         "cpu_baseline": cpu,
         "setup_s": {"hierarchy": round(t_hier, 2), "plain_upload": round(t_setup_plain, 2)},
         "levels": desc_plain["levels"],

@@ -194,6 +194,15 @@ inline int pick_group(double avg, double min_avg) {
   if (avg <= 32.0) return 32;
   return 64;
 }
+// Block rows: two blocks per lane and round, so a round of G lanes covers
+// 2 G blocks.
+inline int pick_block_group(double avg, double min_avg) {
+  if (min_avg < 0.0 || avg < min_avg) return 0;
+  if (avg <= 16.0) return 8;
+  if (avg <= 32.0) return 16;
+  if (avg <= 64.0) return 32;
+  return 64;
+}
 
 // Scalar operator (transfers): SELL-64 or row-contiguous CSR.
 template <class T>
@@ -241,7 +250,7 @@ struct DOp {
         b3 = true;
         nblocks = B.nblocks;
         nnz = 9 * B.nblocks;
-        G = pick_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min_block);
+        G = pick_block_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min_block);
         if (G) bw.build(B);
         else b.build(B);
         return;

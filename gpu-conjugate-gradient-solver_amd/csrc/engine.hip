@@ -83,10 +83,11 @@ void launch_fill(T *p, T v, int64_t n, hipStream_t s, const int32_t *stop) {
 template <class F>
 void with_group(int G, F &&f) {
   switch (G) {
+    case 8: f(std::integral_constant<int, 8>{}); break;
     case 16: f(std::integral_constant<int, 16>{}); break;
     case 32: f(std::integral_constant<int, 32>{}); break;
     case 64: f(std::integral_constant<int, 64>{}); break;
-    default: throw std::runtime_error("lane-group size must be 16, 32 or 64.");
+    default: throw std::runtime_error("lane-group size must be 8, 16, 32 or 64.");
   }
 }
 
@@ -104,6 +105,7 @@ struct Level {
   int pre = 1, post = 1;
   DOp<TM> A;
   std::vector<int64_t> coff;  // colour offsets (rows, or blocks for block GS)
+  std::vector<int64_t> cunits;  // stored entries (blocks for block GS) per colour
   int ncolors = 0;
   DevBuf diag, dinv, inv;
   int64_t inv_ld = 0;  // row stride of the dense inverse
@@ -254,6 +256,24 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   }
   lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
              opt_.group_min_block);
+  // Stored entries (3x3 blocks for block GS) per colour, for the bytes of a
+  // colour launch.
+  lv.cunits.assign(lv.ncolors, 0);
+  for (int c = 0; c < lv.ncolors; ++c) {
+    if (lv.kind == kBlockGS) {
+      std::vector<int32_t> seen;
+      for (int64_t bi = lv.coff[c]; bi < lv.coff[c + 1]; ++bi) {
+        seen.clear();
+        for (int l = 0; l < 3; ++l)
+          for (int32_t e = Ai.ptr[3 * bi + l]; e < Ai.ptr[3 * bi + l + 1]; ++e)
+            seen.push_back(Ai.col[e] / 3);
+        std::sort(seen.begin(), seen.end());
+        lv.cunits[c] += std::unique(seen.begin(), seen.end()) - seen.begin();
+      }
+    } else {
+      lv.cunits[c] = Ai.ptr[lv.coff[c + 1]] - Ai.ptr[lv.coff[c]];
+    }
+  }
   if (!lv.direct) {
     if (lv.kind == kSGS || lv.kind == kDampedJacobi) {
       std::vector<TM> d(lv.n);
@@ -759,9 +779,12 @@ void Engine<TC, TM>::spmv_level_order(const double *x, double *y) {
 
 // Average duration (ms) of one launch of a hot kernel, timed with HIP events
 // on the solver's own stream over `reps` back-to-back launches. which: 0 =
-// CG SpMV (y = A p), 1 = one V-cycle, 2 = one full PCG iteration, 3 = level-0
-// forward+backward smoothing sweep. *bytes receives the algorithmic bytes
-// of one launch (0 when not defined).
+// CG SpMV (y = A p), 1 = one V-cycle, 2 = one full PCG iteration, 3 / 4 =
+// one symmetric (forward + backward) Gauss-Seidel sweep of level 0 / 1.
+// *bytes receives the algorithmic bytes of one launch (0 when not defined);
+// for a sweep, the sum over its colour launches of the colour's matrix
+// entries and its rows' vector traffic (rhs, x read and write, diagonal or
+// inverse block).
 template <class TC, class TM>
 double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
   if (!setup_done_) throw std::runtime_error("solver is not set up.");
@@ -779,15 +802,26 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
     } else if (which == 2) {
       encode_iteration(nullptr);
     } else {
-      encode_smooth(L_[0], 1, false, nullptr);
+      encode_smooth(L_[which - 3], 1, false, nullptr);
     }
   };
+  if ((which == 3 || which == 4) &&
+      (which - 3 >= nlev_ || L_[which - 3].direct ||
+       (L_[which - 3].kind != kSGS && L_[which - 3].kind != kBlockGS)))
+    throw std::runtime_error("time_kernel: that level has no Gauss-Seidel sweep.");
   if (which == 0) *bytes = Acg_.spmv_bytes();
-  if (which == 3 && !L_.empty()) {
+  if (which == 3 || which == 4) {
     const double s = sizeof(TM);
-    const auto &lv = L_[0];
-    *bytes = lv.A.b3 ? 2 * (lv.A.nblocks * (9 * s + 4) + 3.0 * lv.n * s + 9.0 * (lv.n / 3) * s)
-                     : 2 * (lv.A.nnz * (s + 4) + 3.0 * lv.n * s);
+    const auto &lv = L_[which - 3];
+    const bool blk = lv.kind == kBlockGS;
+    double by = 0.0;
+    for (const auto &step : sweep_sequence(lv.ncolors, 1)) {
+      const int c = step.first;
+      const double rows = static_cast<double>(lv.coff[c + 1] - lv.coff[c]);
+      by += blk ? lv.cunits[c] * (9 * s + 4) + rows * 18 * s  // rhs 3, x 6, D^-1 9
+                : lv.cunits[c] * (s + 4) + rows * 4 * s;      // rhs, x 2, diagonal
+    }
+    *bytes = by;
   }
   // Keep the iteration kernels live: a zeroed control block with stop = 0.
   if (which == 2) {

@@ -38,7 +38,7 @@ struct Options {
   // row; 3x3 blocks per block row) run on lane-group kernels (same
   // arithmetic, same order); negative: never.
   double group_min_scalar = 24.0;
-  double group_min_block = 12.0;
+  double group_min_block = 20.0;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (> 0 applies)
 };

@@ -48,10 +48,11 @@ __device__ __forceinline__ void wave_sync() {
 // Entries per lane and round for scalar rows: a round covers G * G / 16
 // entries (16, 64, 256), so a long row needs few dependent load rounds.
 template <int G>
-constexpr int grp_unroll() { return G / 16; }
-// Blocks per lane and round for block rows: 128 blocks per round at G = 64.
+constexpr int grp_unroll() { return G >= 16 ? G / 16 : 1; }
+// Blocks per lane and round for block rows: two, so each lane keeps two
+// independent block loads in flight (a round covers 2 G blocks).
 template <int G>
-constexpr int grp_bunroll() { return G == 64 ? 2 : 1; }
+constexpr int grp_bunroll() { return 2; }
 
 // s + p[0] + p[1] + ... in this order: a pure chain of adds, unrolled by
 // eight so the LDS reads of a batch issue before its adds.

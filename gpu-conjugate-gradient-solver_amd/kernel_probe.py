@@ -1,8 +1,10 @@
-"""Runs the benchmark workload's dominant kernel in isolation, for counter
-collection: `reps` launches of the CG-operator SpMV of the selected
-workload, then `reps` launches of a plain device copy of known size (the
-calibration for FETCH_SIZE / WRITE_SIZE). Prints the algorithmic bytes of
-both so a profiler summary can be scaled.
+"""Runs the benchmark workload's hot kernels in isolation, for counter
+collection: `reps` launches of the CG-operator SpMV (FP64), `reps`
+level-0 and `reps` level-1 symmetric smoothing sweeps in the V-cycle
+precision of the headline (mixed) mode, then `reps` launches of a plain
+device copy of known
+size (the calibration for FETCH_SIZE / WRITE_SIZE). Prints the algorithmic
+bytes of each so a profiler summary can be scaled.
 
 Usage (under rocprofv3):
   rocprofv3 --pmc FETCH_SIZE -d OUT -- python gpu-conjugate-gradient-solver_amd/kernel_probe.py
@@ -37,6 +39,14 @@ def main() -> None:
     s.setup(H)
     s.solve(b, tol=1e-8, maxit=3)  # leaves valid vectors in p
     ms, by = s.time_kernel(0, a.reps)
+    # Level-0 sweep of the mixed-precision solver (FP32 V-cycle).
+    mk = dict(sk)
+    mk.update(coarse_direct=1, direct_truncate=1)
+    m = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mk)
+    m.setup(H)
+    m.solve(b, tol=1e-8, maxit=3)
+    sw_ms, sw_by = m.time_kernel(3, a.reps)
+    sw1_ms, sw1_by = m.time_kernel(4, a.reps)
     # Calibration copy: n doubles read and written once.
     n = a.copy_mib * (1 << 20) // 8
     be = gs.Backend(np.float64)
@@ -47,6 +57,8 @@ def main() -> None:
         be.encode_copy(e, x, y)
         be.submit_and_wait(e)
     print(json.dumps({"config": a.config, "spmv_ms": ms, "spmv_algorithmic_bytes": by,
+                      "sweep_ms": sw_ms, "sweep_algorithmic_bytes": sw_by,
+                      "sweep1_ms": sw1_ms, "sweep1_algorithmic_bytes": sw1_by,
                       "copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
                       "reps": a.reps}))
 

@@ -167,7 +167,7 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     level of at most "max_direct" unknowns, leaving the
  *                     levels below unused (0)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
- * "group_min_block" (12): operators whose average row length reaches the
+ * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed
  * by lane groups, one row per group of 16/32/64 lanes, with the same
  * arithmetic in the same order; negative values keep one thread per row.
