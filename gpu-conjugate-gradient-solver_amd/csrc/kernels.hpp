@@ -220,29 +220,72 @@ __device__ __forceinline__ T sell_offdiag_h(const Sell<T> &A, SliceHead h, int64
   }
   return sum;
 }
+// Row sums of block row b, slots in order. The slots are processed in
+// batches (8 in FP32, 4 in FP64): the block indices of a batch are loaded
+// together, then its x gathers and block values, then the products are
+// added in slot order --
+// two dependent memory round trips per batch instead of one per slot, with
+// the arithmetic (order included) of a plain slot loop. A skipped slot
+// (past the row's slice width, or the diagonal block when skip_diag) adds
+// nothing.
+// Keeps a loaded value live at this point: an empty asm statement that
+// takes it as a register input, so the compiler cannot sink the load into a
+// later conditional block (the load itself is ordinary compiled code).
+template <class T>
+__device__ __forceinline__ void pin(const T &v) {
+  asm volatile("" ::"v"(v));
+}
+
 template <class T>
 __device__ __forceinline__ void bsr3_dot_h(const Bsr3<T> &A, SliceHead h, int64_t b, const T *x,
                                            T y[3], bool skip_diag) {
+  constexpr int CH = sizeof(T) == 4 ? 8 : 4;
   const int64_t lane = b & 63;
   T y0 = T(0), y1 = T(0), y2 = T(0);
-#pragma unroll 2
-  for (int j = 0; j < h.w; ++j) {
-    const int64_t slot = h.base + static_cast<int64_t>(j) * kSlice;
-    const int32_t bc = A.bcol[slot + lane];
-    if (skip_diag && bc == b) continue;
-    const T *v = A.val + slot * 9 + lane;
-    const T x0 = x[3 * static_cast<int64_t>(bc)];
-    const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
-    const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
-    y0 += v[0 * kSlice] * x0;
-    y0 += v[1 * kSlice] * x1;
-    y0 += v[2 * kSlice] * x2;
-    y1 += v[3 * kSlice] * x0;
-    y1 += v[4 * kSlice] * x1;
-    y1 += v[5 * kSlice] * x2;
-    y2 += v[6 * kSlice] * x0;
-    y2 += v[7 * kSlice] * x1;
-    y2 += v[8 * kSlice] * x2;
+  for (int j0 = 0; j0 < h.w; j0 += CH) {
+    int32_t bc[CH];
+    bool use[CH];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int j = j0 + u < h.w ? j0 + u : h.w - 1;  // in-bounds address
+      bc[u] = A.bcol[h.base + static_cast<int64_t>(j) * kSlice + lane];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u) use[u] = j0 + u < h.w && !(skip_diag && bc[u] == b);
+    T xv[CH][3];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int64_t c = 3 * static_cast<int64_t>(bc[u]);
+      xv[u][0] = x[c];
+      xv[u][1] = x[c + 1];
+      xv[u][2] = x[c + 2];
+    }
+    T vv[CH][9];
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      const int j = j0 + u < h.w ? j0 + u : h.w - 1;
+      const T *v = A.val + (h.base + static_cast<int64_t>(j) * kSlice) * 9 + lane;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) vv[u][e] = v[e * kSlice];
+    }
+#pragma unroll
+    for (int u = 0; u < CH; ++u)
+#pragma unroll
+      for (int e = 0; e < 9; ++e) pin(vv[u][e]);
+#pragma unroll
+    for (int u = 0; u < CH; ++u) {
+      if (use[u]) {
+        y0 += vv[u][0] * xv[u][0];
+        y0 += vv[u][1] * xv[u][1];
+        y0 += vv[u][2] * xv[u][2];
+        y1 += vv[u][3] * xv[u][0];
+        y1 += vv[u][4] * xv[u][1];
+        y1 += vv[u][5] * xv[u][2];
+        y2 += vv[u][6] * xv[u][0];
+        y2 += vv[u][7] * xv[u][1];
+        y2 += vv[u][8] * xv[u][2];
+      }
+    }
   }
   y[0] = y0;
   y[1] = y1;
