@@ -466,7 +466,7 @@ int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega
   return guard([&] {
     need(s, "solver");
     s->opt.level_omega.assign(nlev, 0.0);
-    s->opt.level_sweeps.assign(nlev, 0);
+    s->opt.level_sweeps.assign(nlev, -1);
     for (int32_t k = 0; k < nlev; ++k) {
       if (omega) s->opt.level_omega[k] = omega[k];
       if (sweeps) s->opt.level_sweeps[k] = sweeps[k];

@@ -148,7 +148,11 @@ class Engine final : public EngineBase {
           throw std::runtime_error("Gauss-Seidel smoother: omega must be in (0, 2).");
         lv.omega = static_cast<TM>(omega[k]);
       }
-      if (k < static_cast<int>(sweeps.size()) && sweeps[k] > 0) lv.pre = lv.post = sweeps[k];
+      if (k < static_cast<int>(sweeps.size()) && sweeps[k] >= 0) {
+        if (sweeps[k] == 0 && lv.kind == kSGS && !lv.direct && k + 1 < nlev_)
+          throw std::runtime_error("Preconditioner requires at least one smoothing step.");
+        lv.pre = lv.post = sweeps[k];
+      }
     }
     if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
@@ -195,7 +199,7 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   int pre = opt_.pre, post = opt_.post;
   if (k < static_cast<int>(opt_.level_omega.size()) && opt_.level_omega[k] > 0.0)
     om = opt_.level_omega[k];
-  if (k < static_cast<int>(opt_.level_sweeps.size()) && opt_.level_sweeps[k] > 0)
+  if (k < static_cast<int>(opt_.level_sweeps.size()) && opt_.level_sweeps[k] >= 0)
     pre = post = opt_.level_sweeps[k];
   lv.omega = static_cast<TM>(om);
   lv.pre = pre;

@@ -40,7 +40,7 @@ struct Options {
   double group_min_scalar = 24.0;
   double group_min_block = 20.0;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
-  std::vector<int> level_sweeps;    // per-level pre = post sweeps (> 0 applies)
+  std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };
 
 struct SolveResult {

@@ -37,18 +37,21 @@ import gpusolver as gs  # noqa: E402
 MODEL_PATH = HERE / "learned_weights.json"
 # Training / evaluation problem sizes: varcoef3d grid edge, elasticity
 # vertices per edge.
-DEFAULT_SIZE = {"sgs": 64, "bgs": 32}
+DEFAULT_SIZE = {"sgs": 64, "bgs": 48}
 HIDDEN = 64
 
 # Search grids per smoother kind; the plain weight is the reference's
 # (MG-GS w = 1, examples/multigridPoisson/mainAMGtest.cc:697-698; block GS
 # w = 0.7, examples/elasticity/cantileverBeam.cc:572-573).
+# coarse_sweeps: the sweep counts tried on levels below the finest (0 turns a
+# level's smoothing off: a no-op block smoother, as the reference's
+# 0-sweep block GS; symmetric GS requires a sweep).
 KINDS = {
     "sgs": dict(smoother=gs.SGS, plain_omega=1.0, omegas=[0.7, 0.85, 1.0, 1.15, 1.3, 1.45, 1.6],
-                sweeps=[1, 2], block=1, coarse_steps=2),
+                sweeps=[1, 2], coarse_sweeps=[1, 2], block=1, coarse_steps=2),
     "bgs": dict(smoother=gs.BLOCK_GS, plain_omega=0.7,
-                omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2], block=3,
-                coarse_steps=0),
+                omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2],
+                coarse_sweeps=[0, 1, 2], block=3, coarse_steps=0),
 }
 
 
@@ -82,9 +85,12 @@ def encode_targets(omega, sweeps) -> np.ndarray:
     return np.stack([(np.asarray(omega) - 1.0) / 0.5, (np.asarray(sweeps) - 1.5) / 0.5], axis=1)
 
 
-def decode(Y: np.ndarray):
+def decode(Y: np.ndarray, min_sweeps: int = 1):
+    """(w, sweeps) per level; the finest level keeps at least one sweep."""
     omega = np.clip(1.0 + 0.5 * Y[:, 0], 0.2, 1.9)
-    sweeps = np.clip(np.rint(1.5 + 0.5 * Y[:, 1]), 1, 3).astype(np.int32)
+    sweeps = np.clip(np.rint(1.5 + 0.5 * Y[:, 1]), min_sweeps, 3).astype(np.int32)
+    if sweeps.size:
+        sweeps[0] = max(int(sweeps[0]), 1)
     return omega, sweeps
 
 
@@ -103,7 +109,10 @@ def work_per_iteration(nnz: np.ndarray, sweeps) -> float:
 
 
 def make_solver(H, spec, omega, sweeps):
-    s = gs.Solver(gs.PREC_MIXED, smoother=spec["smoother"], coarse_direct=1,
+    """The mixed-precision solver of the headline configuration: FP32
+    V-cycle, FP64 CG, FP64 direct solve at the first level within
+    max_direct unknowns."""
+    s = gs.Solver(gs.PREC_MIXED, smoother=spec["smoother"], coarse_direct=1, direct_truncate=1,
                   coarse_steps=spec["coarse_steps"], omega=spec["plain_omega"], graph_iters=8,
                   cg_block3=1 if spec["block"] == 3 else 0)
     s.setup(H)
@@ -122,6 +131,7 @@ def sweep_labels(H, b, spec, maxit=400):
     omega = [spec["plain_omega"]] * L
     sweeps = [1] * L
     s = make_solver(H, spec, omega, sweeps)
+    used = len(s.describe()["levels"])  # levels in use; the last is solved directly
 
     def cost(om, sw):
         s.set_level_weights(omega=om, sweeps=sw)
@@ -135,15 +145,15 @@ def sweep_labels(H, b, spec, maxit=400):
 
     best, it0 = cost(omega, sweeps)
     plain_cost = best
-    for k in range(L - 1):  # the coarsest level is solved directly
+    for k in range(used - 1):  # the last level in use is solved directly
         for om in spec["omegas"]:
-            for sw in spec["sweeps"]:
+            for sw in (spec["sweeps"] if k == 0 else spec["coarse_sweeps"]):
                 o2, s2 = list(omega), list(sweeps)
                 o2[k], s2[k] = om, sw
                 c, _ = cost(o2, s2)
                 if c < ACCEPT * best:
                     best, omega, sweeps = c, o2, s2
-    return np.array(omega[:L - 1]), np.array(sweeps[:L - 1]), plain_cost, best
+    return np.array(omega[:used - 1]), np.array(sweeps[:used - 1]), plain_cost, best
 
 
 def load_model(kind: str, path: Path = MODEL_PATH):
@@ -160,7 +170,7 @@ def predict(H, kind: str, path: Path = MODEL_PATH):
     spec = KINDS[kind]
     net = load_model(kind, path)
     F = H.features(spec["block"])
-    omega, sweeps = decode(net.forward(normalize(F)))
+    omega, sweeps = decode(net.forward(normalize(F)), min(spec["coarse_sweeps"]))
     return omega, sweeps
 
 
@@ -182,8 +192,8 @@ def label(kind: str, seeds, size: int, out: Path) -> None:
     with open(out, "a") as f:
         for seed in seeds:
             A, b, H = training_problem(kind, seed, size)
-            F = H.features(spec["block"])[:H.levels - 1]
             om, sw, pc, bc = sweep_labels(H, b, spec)
+            F = H.features(spec["block"])[:len(om)]
             f.write(json.dumps({"kind": kind, "seed": int(seed), "size": size,
                                 "levels": H.levels, "features": F.tolist(),
                                 "omega": om.tolist(), "sweeps": sw.tolist(),

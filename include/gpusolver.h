@@ -174,8 +174,11 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  * Takes effect at the next gs_solver_setup. */
 int gs_solver_set_int(gs_solver_h s, const char *key, int64_t value);
 int gs_solver_set_real(gs_solver_h s, const char *key, double value);
-/* Per-level damping and pre = post sweep counts (learned weights); entries
- * <= 0 keep the global setting. */
+/* Per-level damping and pre = post sweep counts (learned weights). An omega
+ * entry <= 0 keeps the global setting; a sweep entry < 0 keeps it, 0 turns
+ * that level's smoothing off (a no-op for block Gauss-Seidel and Jacobi
+ * levels, as the reference's 0-sweep smoothers; rejected for symmetric
+ * Gauss-Seidel, which requires a sweep). */
 int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega,
                                 const int32_t *sweeps);
 /* Upload the hierarchy. level_old_to_new[k] permutes level k
