@@ -562,7 +562,11 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
     return;
   }
   encode_smooth(lv, lv.pre, zero_x, stop);
-  encode_residual(lv, stop);
+  // With no pre-smoothing from x = 0 the residual is rhs bit for bit (the
+  // row sums of A 0 are +0, and rhs - (+0) == rhs), so restrict rhs itself.
+  const bool res_is_rhs = zero_x && lv.pre == 0;
+  if (!res_is_rhs) encode_residual(lv, stop);
+  const TM *res = res_is_rhs ? lv.rhs : lv.res.template as<TM>();
   Level<TM> &cl = L_[k + 1];
   // The coarse correction starts at zero; the coarse level's first kernel
   // zeroes it itself when it is a zero-start Gauss-Seidel colour.
@@ -574,11 +578,11 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
     with_group(lv.R.G, [&](auto g) {
       constexpr int G = decltype(g)::value;
       gsk::k_restrict_w<TM, G><<<gsk::grid_rows<G>(cl.n), kBlock, 0, stream_>>>(
-          lv.R.w.view(), lv.res.template as<TM>(), cl.rhs, corr_c, stop);
+          lv.R.w.view(), res, cl.rhs, corr_c, stop);
     });
   else
-    gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(
-        lv.R.s.view(), lv.res.template as<TM>(), cl.rhs, corr_c, stop);
+    gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(lv.R.s.view(), res, cl.rhs,
+                                                                corr_c, stop);
   encode_vcycle(k + 1, stop, true);
   const bool alias = opt_.reference_vcycle && k > 0;
   if (lv.P.G)

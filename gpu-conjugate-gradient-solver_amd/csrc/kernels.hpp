@@ -605,7 +605,8 @@ __global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ y, T v, int64_t
 }
 
 // Coarsest level: x = A_c^-1 rhs with the FP64 dense inverse (row stride
-// ld), one wavefront per row.
+// ld, a multiple of 64; entries past column n of a row are zero), one
+// wavefront per row.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_dense_solve(const double *__restrict__ inv, int64_t n,
                                                         int64_t ld, const T *__restrict__ rhs,
@@ -614,10 +615,19 @@ __global__ __launch_bounds__(kBlock) void k_dense_solve(const double *__restrict
   const int64_t row = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const double *a = inv + row * ld;
-  double s = 0.0;
-  for (int64_t j = lane; j < n; j += 64) s += a[j] * static_cast<double>(rhs[j]);
-  s = wave_sum(s);
+  // Two inverse entries per lane and step (16-byte loads; ld is even and
+  // the buffer 256-byte aligned, so a + j is 16-byte aligned for even j).
+  const double2 *a = reinterpret_cast<const double2 *>(inv + row * ld);
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
+  for (int64_t j = 2 * lane; j < n; j += 128) {
+    const double2 av = a[j >> 1];
+    const double r0 = static_cast<double>(rhs[j]);
+    const double r1 = j + 1 < n ? static_cast<double>(rhs[j + 1]) : 0.0;
+    s0 += av.x * r0;
+    s1 += av.y * r1;
+  }
+  const double s = wave_sum(s0 + s1);
   if (lane == 0) x[row] = static_cast<T>(s);
 }
 

@@ -224,8 +224,12 @@ def fit(kind: str, label_paths, epochs: int, out: Path) -> dict:
     rows = read_labels(kind, label_paths)
     if not rows:
         raise SystemExit(f"no {kind} labels in {label_paths}")
+    spec = KINDS[kind]
     X = np.concatenate([normalize(np.array(r["features"])) for r in rows])
-    T = np.concatenate([encode_targets(r["omega"], r["sweeps"]) for r in rows])
+    # A level with 0 sweeps has no damping to learn: its target is the plain w.
+    T = np.concatenate([encode_targets([o if sw > 0 else spec["plain_omega"]
+                                        for o, sw in zip(r["omega"], r["sweeps"])], r["sweeps"])
+                        for r in rows])
     net = gs.MLP(5, HIDDEN, 2, seed=1234)
     l0, _ = net.loss_grad(X, T)
     loss = net.train(X, T, epochs=epochs, lr=3e-3)

@@ -332,3 +332,22 @@ def test_pcg_direct_truncate(elast, prec):
                                     coarse_steps=0, direct_coarse=True)
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
     assert true <= 1.5e-8
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
+def test_vcycle_bitexact_zero_sweep_level(elast, prec):
+    """Per-level weights with level 1 unsmoothed (0 sweeps, a no-op block
+    smoother): the level's residual is its right-hand side and is restricted
+    directly; bit-exact against the oracle's full V-cycle."""
+    E, rhs, H = elast
+    L = H.levels
+    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2)
+    s.setup(H)
+    om = [0.9, 1.1] + [0.7] * (L - 2)
+    sw = [1, 0] + [1] * (L - 2)
+    s.set_level_weights(omega=om, sweeps=sw)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(9, E.rows)
+    zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=gs.BLOCK_GS, omegas=om,
+                    pres=sw, posts=sw, coarse_steps=2)
+    np.testing.assert_array_equal(s.vcycle(r), zo)
