@@ -403,16 +403,18 @@ bool dense_spd_inverse(int64_t n, std::vector<double> &a) {
     for (int64_t k = 0; k < j; ++k) d -= L[j * n + k] * L[j * n + k];
     if (!(d > 0)) return false;
     L[j * n + j] = std::sqrt(d);
+#pragma omp parallel for schedule(static) if (n - j > 256)
     for (int64_t i = j + 1; i < n; ++i) {
       double s = a[i * n + j];
       for (int64_t k = 0; k < j; ++k) s -= L[i * n + k] * L[j * n + k];
       L[i * n + j] = s / L[j * n + j];
     }
   }
-  // Solve L L^T X = I column by column.
+  // Solve L L^T X = I column by column (columns are independent).
   std::vector<double> inv(n * n);
-  std::vector<double> y(n);
+#pragma omp parallel for schedule(dynamic, 4)
   for (int64_t c = 0; c < n; ++c) {
+    std::vector<double> y(n);
     for (int64_t i = 0; i < n; ++i) {
       double s = (i == c) ? 1.0 : 0.0;
       for (int64_t k = 0; k < i; ++k) s -= L[i * n + k] * y[k];
