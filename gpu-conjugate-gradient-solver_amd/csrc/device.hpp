@@ -188,11 +188,12 @@ struct DBRows {
 // Lanes per row for the group-per-row kernels given the average row length
 // (entries, or blocks for block rows); 0 keeps the thread-per-row SELL
 // layout. min_avg < 0 never groups.
-inline int pick_group(double avg, double min_avg) {
+inline int pick_group(double avg, double min_avg, int max_g = 64) {
   if (min_avg < 0.0 || avg < min_avg) return 0;
-  if (avg <= 16.0) return 16;
-  if (avg <= 32.0) return 32;
-  return 64;
+  int g = 64;
+  if (avg <= 16.0) g = 16;
+  else if (avg <= 32.0) g = 32;
+  return g < max_g ? g : max_g;
 }
 // Block rows: two blocks per lane and round, so a round of G lanes covers
 // 2 G blocks.
@@ -211,11 +212,11 @@ struct DMat {
   int64_t rows = 0, cols = 0, nnz = 0;
   DSell<T> s;
   DRows<T> w;
-  void build(const gsx::Csr &A, double group_min) {
+  void build(const gsx::Csr &A, double group_min, int group_max = 64) {
     rows = A.rows;
     cols = A.cols;
     nnz = A.nnz();
-    G = pick_group(rows ? static_cast<double>(nnz) / rows : 0.0, group_min);
+    G = pick_group(rows ? static_cast<double>(nnz) / rows : 0.0, group_min, group_max);
     if (G) w.build(A);
     else s.build(A);
   }

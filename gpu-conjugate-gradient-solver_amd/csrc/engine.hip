@@ -367,8 +367,10 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
         throw std::runtime_error("MultigridSmoother: transfer operator shape mismatch.");
       const std::vector<int32_t> qf_inv = inverse_of(L_[k].q);
       const std::vector<int32_t> qc_inv = inverse_of(L_[k + 1].q);
-      L_[k].P.build(relabel(Pk, L_[k].q, qc_inv), opt_.group_min_scalar);
-      L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv), opt_.group_min_scalar);
+      L_[k].P.build(relabel(Pk, L_[k].q, qc_inv), opt_.group_min_scalar,
+                    opt_.group_max_transfer);
+      L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv), opt_.group_min_scalar,
+                    opt_.group_max_transfer);
     }
   }
   // CG operator: level-0 input with the order-keeping permutation, then the

@@ -39,6 +39,10 @@ struct Options {
   // arithmetic, same order); negative: never.
   double group_min_scalar = 24.0;
   double group_min_block = 20.0;
+  // Largest lane group for the transfer operators' rows (16, 32 or 64);
+  // 32 measured best for the ~190-entry rows of R = P^T (two rows per
+  // wavefront, two load rounds per row).
+  int group_max_transfer = 32;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };

@@ -166,6 +166,8 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *   "direct_truncate" with "coarse_direct": solve directly at the first
  *                     level of at most "max_direct" unknowns, leaving the
  *                     levels below unused (0)
+ *   "group_max_transfer" largest lane group (16, 32, 64) for the rows of the
+ *                     restriction / prolongation operators (32)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed
