@@ -116,8 +116,18 @@ __device__ __forceinline__ void block_partial(T v, T *partials) {
 // threads; result valid in thread 0.
 template <class T>
 __device__ __forceinline__ T fixed_sum(const T *partials, uint32_t nparts) {
+  // Thread t adds partials t, t + kBlock, ... in increasing order; the loads
+  // of eight such terms are issued before their adds.
   T s = T(0);
-  for (uint32_t i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
+  uint32_t i = threadIdx.x;
+  for (; i + 7 * kBlock < nparts; i += 8 * kBlock) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = partials[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s += v[u];
+  }
+  for (; i < nparts; i += kBlock) s += partials[i];
   return block_sum(s);
 }
 
