@@ -21,6 +21,7 @@
 // traffic per pivot block.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -173,22 +174,31 @@ __global__ __launch_bounds__(kThreads) void k_rank_update(double *__restrict__ A
     }
 }
 
-}  // namespace
+// A = 0 with ones on the padding diagonal (rows and columns n..N-1).
+__global__ __launch_bounds__(kThreads) void k_fill_padded(double *__restrict__ A, int64_t N,
+                                                          int64_t n) {
+  const int64_t total = N * N;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x; e < total;
+       e += static_cast<int64_t>(gridDim.x) * kThreads) {
+    const int64_t i = e / N, j = e % N;
+    A[e] = (i == j && i >= n) ? 1.0 : 0.0;
+  }
+}
 
-int64_t padded_size(int64_t n) { return (n + NB - 1) / NB * NB; }
+// A[r][col[k]] += val[k] over row r's entries in CSR order (one thread per
+// row, so duplicates are summed in stored order).
+__global__ __launch_bounds__(kThreads) void k_scatter_rows(const int32_t *__restrict__ ptr,
+                                                           const int32_t *__restrict__ col,
+                                                           const double *__restrict__ val,
+                                                           int64_t n, double *__restrict__ A,
+                                                           int64_t N) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (r >= n) return;
+  for (int32_t k = ptr[r]; k < ptr[r + 1]; ++k) A[r * N + col[k]] += val[k];
+}
 
-void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
-                        hipStream_t stream) {
-  if (static_cast<int64_t>(a.size()) != n * n)
-    throw std::runtime_error("device SPD inverse: size mismatch.");
-  const int64_t N = padded_size(n);
-  // Padded host copy: identity on the padding.
-  std::vector<double> h(static_cast<size_t>(N) * N, 0.0);
-  for (int64_t i = 0; i < n; ++i)
-    for (int64_t j = 0; j < n; ++j) h[i * N + j] = a[i * n + j];
-  for (int64_t i = n; i < N; ++i) h[i * N + i] = 1.0;
-  GS_DHIP(hipMemcpyAsync(d_inv, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                         stream));
+// Blocked Gauss-Jordan on the padded N x N matrix in d_inv (see above).
+void eliminate(int64_t N, double *d_inv, hipStream_t stream) {
   double *dinv = nullptr, *C = nullptr, *R = nullptr;
   int *flag = nullptr;
   GS_DHIP(hipMalloc(&dinv, NB * NB * sizeof(double)));
@@ -215,6 +225,62 @@ void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
   GS_DHIP(hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, stream));
   GS_DHIP(hipStreamSynchronize(stream));
   if (hflag) throw std::runtime_error("coarse level matrix is not SPD.");
+}
+
+}  // namespace
+
+int64_t padded_size(int64_t n) { return (n + NB - 1) / NB * NB; }
+
+void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
+                        hipStream_t stream) {
+  if (static_cast<int64_t>(a.size()) != n * n)
+    throw std::runtime_error("device SPD inverse: size mismatch.");
+  const int64_t N = padded_size(n);
+  // Padded host copy: identity on the padding.
+  std::vector<double> h(static_cast<size_t>(N) * N, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t j = 0; j < n; ++j) h[i * N + j] = a[i * n + j];
+  for (int64_t i = n; i < N; ++i) h[i * N + i] = 1.0;
+  GS_DHIP(hipMemcpyAsync(d_inv, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                         stream));
+  eliminate(N, d_inv, stream);
+}
+
+void spd_inverse_device_csr(int64_t n, const int32_t *ptr, const int32_t *col,
+                            const double *val, double *d_inv, hipStream_t stream) {
+  if (n <= 0) throw std::runtime_error("device SPD inverse: empty matrix.");
+  const int64_t nnz = ptr[n];
+  for (int64_t r = 0; r < n; ++r)
+    if (ptr[r + 1] < ptr[r]) throw std::runtime_error("device SPD inverse: invalid row pointer.");
+  for (int64_t k = 0; k < nnz; ++k)
+    if (col[k] < 0 || col[k] >= n)
+      throw std::runtime_error("device SPD inverse: column index out of range.");
+  const int64_t N = padded_size(n);
+  int32_t *dptr = nullptr, *dcol = nullptr;
+  double *dval = nullptr;
+  struct Free {
+    void *p[3] = {nullptr, nullptr, nullptr};
+    ~Free() {
+      for (void *q : p)
+        if (q) (void)hipFree(q);
+    }
+  } guard;
+  GS_DHIP(hipMalloc(&dptr, (n + 1) * sizeof(int32_t)));
+  guard.p[0] = dptr;
+  GS_DHIP(hipMalloc(&dcol, (nnz > 0 ? nnz : 1) * sizeof(int32_t)));
+  guard.p[1] = dcol;
+  GS_DHIP(hipMalloc(&dval, (nnz > 0 ? nnz : 1) * sizeof(double)));
+  guard.p[2] = dval;
+  GS_DHIP(hipMemcpyAsync(dptr, ptr, (n + 1) * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  if (nnz > 0) {
+    GS_DHIP(hipMemcpyAsync(dcol, col, nnz * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    GS_DHIP(hipMemcpyAsync(dval, val, nnz * sizeof(double), hipMemcpyHostToDevice, stream));
+  }
+  const int64_t fill_blocks = std::min<int64_t>((N * N + kThreads - 1) / kThreads, 16384);
+  k_fill_padded<<<static_cast<uint32_t>(fill_blocks), kThreads, 0, stream>>>(d_inv, N, n);
+  k_scatter_rows<<<static_cast<uint32_t>((n + kThreads - 1) / kThreads), kThreads, 0, stream>>>(
+      dptr, dcol, dval, n, d_inv, N);
+  eliminate(N, d_inv, stream);
 }
 
 }  // namespace gsdense

@@ -16,5 +16,9 @@ int64_t padded_size(int64_t n);
 // dense row-major n x n SPD matrix. Throws when a pivot is not positive.
 void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
                         hipStream_t stream);
+// The same from the n x n matrix in CSR form (host arrays; duplicates are
+// summed in stored order): the padded matrix is assembled on the device.
+void spd_inverse_device_csr(int64_t n, const int32_t *ptr, const int32_t *col,
+                            const double *val, double *d_inv, hipStream_t stream);
 
 }  // namespace gsdense

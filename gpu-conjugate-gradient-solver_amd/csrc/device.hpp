@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "host/setup.h"
@@ -58,8 +59,8 @@ class DevBuf {
   T *as() const { return static_cast<T *>(p_); }
   void *get() const { return p_; }
   size_t bytes() const { return n_; }
-  template <class T>
-  void upload(const std::vector<T> &v) {
+  template <class T, class A>
+  void upload(const std::vector<T, A> &v) {
     alloc(v.size() * sizeof(T));
     if (!v.empty()) GS_HIP(hipMemcpy(p_, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
   }
@@ -81,6 +82,18 @@ std::vector<T> cast_values(const std::vector<double> &v) {
   return out;
 }
 
+// Uploads n FP64 values as T (the FP32 cast runs in parallel on the host).
+template <class T>
+void upload_values(DevBuf &buf, const double *v, size_t n) {
+  if constexpr (std::is_same<T, double>::value) {
+    buf.upload(v, n);
+  } else {
+    gsx::RawVec<float> f(n);
+    gsx::to_float(v, n, f.data());
+    buf.upload(f.data(), n);
+  }
+}
+
 // SELL-64 device matrix (layout and validation in gsx::sell64_pack).
 template <class T>
 struct DSell {
@@ -98,7 +111,7 @@ struct DSell {
     slice_ptr.upload(P.slice_ptr);
     slice_w.upload(P.slice_w);
     col.upload(P.col);
-    val.upload(cast_values<T>(P.val));
+    upload_values<T>(val, P.val.data(), P.val.size());
   }
   gsk::Sell<T> view() const {
     return gsk::Sell<T>{rows, cols, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(),
@@ -117,7 +130,7 @@ struct DBsr3 {
   int64_t slots = 0;
   DevBuf slice_ptr, slice_w, bcol, val;
 
-  void build(const gsx::Bsr3Ell &B) {
+  void build(const gsx::Bsr3Csr &B) {
     const gsx::Bsr3Sell64 P = gsx::bsr3_sell64_pack(B);
     nb = P.nb;
     nblocks = P.nblocks;
@@ -125,7 +138,7 @@ struct DBsr3 {
     slice_ptr.upload(P.slice_ptr);
     slice_w.upload(P.slice_w);
     bcol.upload(P.bcol);
-    val.upload(cast_values<T>(P.val));
+    upload_values<T>(val, P.val.data(), P.val.size());
   }
   gsk::Bsr3<T> view() const {
     return gsk::Bsr3<T>{nb, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(), bcol.as<int32_t>(),
@@ -146,11 +159,10 @@ struct DRows {
     cols = A.cols;
     nnz = A.nnz();
     std::vector<int64_t> p(A.ptr.begin(), A.ptr.end());
-    for (int32_t c : A.col)
-      if (c < 0 || c >= A.cols) throw std::runtime_error("CSR upload: column index out of range.");
+    gsx::check_columns(A, "CSR upload: column index out of range.");
     ptr.upload(p);
     col.upload(A.col);
-    val.upload(cast_values<T>(A.val));
+    upload_values<T>(val, A.val.data(), A.val.size());
   }
   gsk::Rows<T> view() const {
     return gsk::Rows<T>{rows, cols, ptr.as<int64_t>(), col.as<int32_t>(), val.as<T>()};
@@ -162,23 +174,12 @@ template <class T>
 struct DBRows {
   int64_t nb = 0, nblocks = 0;
   DevBuf ptr, bcol, val;
-  void build(const gsx::Bsr3Ell &B) {
+  void build(const gsx::Bsr3Csr &B) {
     nb = B.nb;
     nblocks = B.nblocks;
-    std::vector<int64_t> p(nb + 1, 0);
-    for (int64_t b = 0; b < nb; ++b) p[b + 1] = p[b] + B.nblk[b];
-    std::vector<int32_t> bc(nblocks);
-    std::vector<double> v(9 * nblocks);
-    for (int64_t b = 0; b < nb; ++b)
-      for (int32_t s = 0; s < B.nblk[b]; ++s) {
-        const int64_t k = p[b] + s;
-        bc[k] = B.bcol[static_cast<int64_t>(s) * nb + b];
-        if (bc[k] < 0 || bc[k] >= nb) throw std::runtime_error("BSR upload: block column out of range.");
-        for (int e = 0; e < 9; ++e) v[9 * k + e] = B.val[(static_cast<int64_t>(s) * 9 + e) * nb + b];
-      }
-    ptr.upload(p);
-    bcol.upload(bc);
-    val.upload(cast_values<T>(v));
+    ptr.upload(B.ptr);
+    bcol.upload(B.bcol);
+    upload_values<T>(val, B.val.data(), B.val.size());
   }
   gsk::BRows<T> view() const {
     return gsk::BRows<T>{nb, ptr.as<int64_t>(), bcol.as<int32_t>(), val.as<T>()};
@@ -235,6 +236,7 @@ struct DOp {
   DBsr3<T> b;
   DRows<T> w;
   DBRows<T> bw;
+  std::vector<int64_t> bptr;  // block rows: host copy of the block row pointer
   // block_key: optional rank of each block column, the order in which a
   // block row's blocks are stored and summed. group_min: average row length
   // (entries; blocks for block rows) from which rows are processed by lane
@@ -245,11 +247,13 @@ struct DOp {
     b3 = false;
     G = 0;
     nblocks = 0;
+    bptr.clear();
     if (prefer_b3) {
-      gsx::Bsr3Ell B;
+      gsx::Bsr3Csr B;
       if (gsx::bsr3_pack(A, B, block_key)) {
         b3 = true;
         nblocks = B.nblocks;
+        bptr = B.ptr;
         nnz = 9 * B.nblocks;
         G = pick_block_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min_block);
         if (G) bw.build(B);

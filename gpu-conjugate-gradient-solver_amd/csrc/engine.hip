@@ -39,26 +39,10 @@ using gsk::kBlock;
 
 namespace {
 
-// Keep-order relabelling: row i of the result is row rq[i] of A, column c
-// becomes cinv[c]. Entry order inside each row is unchanged.
+// Keep-order relabelling (row i <- row rq[i], column c -> cinv[c]).
 gsx::Csr relabel(const gsx::Csr &A, const std::vector<int32_t> &rq,
                  const std::vector<int32_t> &cinv) {
-  gsx::Csr B;
-  B.rows = A.rows;
-  B.cols = A.cols;
-  B.ptr.assign(A.rows + 1, 0);
-  for (int64_t i = 0; i < A.rows; ++i) B.ptr[i + 1] = B.ptr[i] + (A.ptr[rq[i] + 1] - A.ptr[rq[i]]);
-  B.col.resize(A.nnz());
-  B.val.resize(A.nnz());
-#pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < A.rows; ++i) {
-    int64_t d = B.ptr[i];
-    for (int32_t k = A.ptr[rq[i]]; k < A.ptr[rq[i] + 1]; ++k, ++d) {
-      B.col[d] = cinv[A.col[k]];
-      B.val[d] = A.val[k];
-    }
-  }
-  return B;
+  return gsx::relabel(A, rq.data(), cinv.data());
 }
 
 std::vector<int32_t> inverse_of(const std::vector<int32_t> &q) {
@@ -187,6 +171,14 @@ class Engine final : public EngineBase {
   int64_t hist_cap_ = 0;
   std::vector<double> hist_host_;
   bool setup_done_ = false;
+  // Host wall time of the setup phases (ms), in order, for describe().
+  std::vector<std::pair<std::string, double>> setup_ms_;
+  std::chrono::steady_clock::time_point tmark_;
+  void mark(const std::string &phase) {
+    const auto t = std::chrono::steady_clock::now();
+    setup_ms_.emplace_back(phase, std::chrono::duration<double, std::milli>(t - tmark_).count());
+    tmark_ = t;
+  }
 };
 
 template <class TC, class TM>
@@ -218,6 +210,7 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     if (blocky && lv.n % 3 != 0)
       throw std::runtime_error("BlockGaussSeidelSmoother: number of DOFs must be divisible by 3.");
   }
+  mark("level " + std::to_string(k) + " checks");
   // Internal ordering.
   lv.coff.clear();
   if (!lv.direct && lv.kind == kSGS) {
@@ -237,20 +230,21 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     lv.q = iota_vec(Ak.rows);
     lv.ncolors = 0;
   }
+  mark("level " + std::to_string(k) + " colouring");
   const std::vector<int32_t> qinv = inverse_of(lv.q);
   gsx::Csr Ai = relabel(Ak, lv.q, qinv);
+  mark("level " + std::to_string(k) + " relabel");
   if (lv.direct) {
     if (lv.n > opt_.max_direct)
       throw std::runtime_error("coarsest level has " + std::to_string(lv.n) +
                                " unknowns, above the dense direct-solve limit of " +
                                std::to_string(opt_.max_direct) + "; coarsen further.");
-    std::vector<double> dense(static_cast<size_t>(lv.n) * lv.n, 0.0);
-    for (int64_t r = 0; r < lv.n; ++r)
-      for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e) dense[r * lv.n + Ai.col[e]] += Ai.val[e];
     lv.inv_ld = gsdense::padded_size(lv.n);
     lv.inv.alloc(static_cast<size_t>(lv.inv_ld) * lv.inv_ld * sizeof(double));
-    gsdense::spd_inverse_device(lv.n, dense, lv.inv.template as<double>(), stream_);
+    gsdense::spd_inverse_device_csr(lv.n, Ai.ptr.data(), Ai.col.data(), Ai.val.data(),
+                                    lv.inv.template as<double>(), stream_);
   }
+  if (lv.direct) mark("level " + std::to_string(k) + " dense inverse");
   // Blocks summed in the level's own (pre-relabelling) column order, the
   // order of the CSR rows the reference walks.
   std::vector<int32_t> key;
@@ -260,20 +254,14 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   }
   lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
              opt_.group_min_block);
+  mark("level " + std::to_string(k) + " pack + upload");
   // Stored entries (3x3 blocks for block GS) per colour, for the bytes of a
   // colour launch.
   lv.cunits.assign(lv.ncolors, 0);
   for (int c = 0; c < lv.ncolors; ++c) {
     if (lv.kind == kBlockGS) {
-      std::vector<int32_t> seen;
-      for (int64_t bi = lv.coff[c]; bi < lv.coff[c + 1]; ++bi) {
-        seen.clear();
-        for (int l = 0; l < 3; ++l)
-          for (int32_t e = Ai.ptr[3 * bi + l]; e < Ai.ptr[3 * bi + l + 1]; ++e)
-            seen.push_back(Ai.col[e] / 3);
-        std::sort(seen.begin(), seen.end());
-        lv.cunits[c] += std::unique(seen.begin(), seen.end()) - seen.begin();
-      }
+      if (!lv.A.b3) throw std::runtime_error("block Gauss-Seidel level without block storage.");
+      lv.cunits[c] = lv.A.bptr[lv.coff[c + 1]] - lv.A.bptr[lv.coff[c]];
     } else {
       lv.cunits[c] = Ai.ptr[lv.coff[c + 1]] - Ai.ptr[lv.coff[c]];
     }
@@ -303,7 +291,8 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
       if (lv.kind == kSGS) lv.diag.upload(d);
       else lv.dinv.upload(di);
     } else {
-      lv.dinv.upload(gsd::cast_values<TM>(gsx::inverse_diag_blocks3(Ai)));
+      const std::vector<double> dinv = gsx::inverse_diag_blocks3(Ai);
+      gsd::upload_values<TM>(lv.dinv, dinv.data(), dinv.size());
     }
   }
   if (k > 0) {
@@ -314,10 +303,13 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   }
   lv.res.alloc(lv.n * sizeof(TM));
   lv.tmp.alloc(lv.n * sizeof(TM));
+  mark("level " + std::to_string(k) + " diagonal + vectors");
 }
 
 template <class TC, class TM>
 void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm> &perms) {
+  setup_ms_.clear();
+  tmark_ = std::chrono::steady_clock::now();
   nlev_ = static_cast<int>(H.A.size());
   if (nlev_ == 0) throw std::runtime_error("MultigridSmoother: empty hierarchy.");
   if (static_cast<int>(perms.size()) != nlev_)
@@ -347,9 +339,11 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   L_.clear();
   L_.resize(nlev_);
   const int used_levels = opt_.precond == kPrecMultigrid ? nlev_ : 1;
+  mark("validate");
   for (int k = 0; k < used_levels; ++k) {
     const gsx::Csr Ak = gsx::permute_sorted(H.A[k], perms[k].old_to_new.data(),
                                             perms[k].old_to_new.data());
+    mark("level " + std::to_string(k) + " permute");
     if (opt_.precond == kPrecMultigrid) build_level(k, Ak, k == nlev_ - 1);
     else {
       L_[0].n = Ak.rows;
@@ -371,14 +365,21 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
                     opt_.group_max_transfer);
       L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv), opt_.group_min_scalar,
                     opt_.group_max_transfer);
+      mark("transfers " + std::to_string(k));
     }
   }
   // CG operator: level-0 input with the order-keeping permutation, then the
   // internal relabelling of level 0.
   {
-    gsx::Csr Acg = gsx::permute_keep_order(H.A[0], perms[0]);
+    // One keep-order relabelling of the input: internal row i is input row
+    // new_to_old[q[i]], input column c becomes q^-1[old_to_new[c]].
     const std::vector<int32_t> q0inv = inverse_of(L_[0].q);
-    gsx::Csr Ai = relabel(Acg, L_[0].q, q0inv);
+    std::vector<int32_t> rq(n_), cmap(n_);
+    for (int64_t i = 0; i < n_; ++i) {
+      rq[i] = perms[0].new_to_old[L_[0].q[i]];
+      cmap[i] = q0inv[perms[0].old_to_new[i]];
+    }
+    gsx::Csr Ai = relabel(H.A[0], rq, cmap);
     const bool blocky = opt_.fine_kind == kBlockGS || opt_.fine_kind == kBlockJacobi ||
                         opt_.cg_block3;
     int2lvl0_ = L_[0].q;
@@ -392,6 +393,7 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       for (int64_t j = 0; j < n_ / 3; ++j) key[j] = int2orig_[3 * j] / 3;
     }
     Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data());
+    mark("CG operator");
   }
   // CG vectors and control block.
   b_.alloc(n_ * sizeof(TC));
@@ -424,6 +426,7 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   if (!ev1_) GS_HIP(hipEventCreate(&ev1_));
   if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
   if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+  mark("vectors");
   setup_done_ = true;
 }
 
@@ -891,7 +894,10 @@ std::string Engine<TC, TM>::describe() const {
        << ", \"group\": " << li[k].group << ", \"r_group\": " << li[k].r_group
        << ", \"p_group\": " << li[k].p_group << "}";
   }
-  os << "], \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
+  os << "], \"setup_ms\": {";
+  for (size_t i = 0; i < setup_ms_.size(); ++i)
+    os << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
+  os << "}, \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
      << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters
      << "}";
   return os.str();

@@ -3,6 +3,7 @@
 #include "setup.h"
 
 #include <algorithm>
+#include <climits>
 #include <limits>
 #include <queue>
 
@@ -99,26 +100,77 @@ Graph block_graph(const Csr &A, int32_t bs) {
   if (A.rows % bs != 0)
     throw std::runtime_error("buildBlockGraph: matrix size is not divisible by block size.");
   const int64_t nb = A.rows / bs;
-  std::vector<std::vector<int32_t>> adj(nb);
-  for (int64_t r = 0; r < A.rows; ++r) {
-    const int64_t br = r / bs;
-    for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
-      const int64_t bc = A.col[k] / bs;
-      if (bc == br) continue;
-      adj[br].push_back(static_cast<int32_t>(bc));
-      adj[bc].push_back(static_cast<int32_t>(br));
+  // The symmetrised block adjacency without self loops, each row sorted and
+  // unique: out-neighbours per block row (R), their transpose (T), and the
+  // merge of both.
+  auto gather = [&](int64_t br, std::vector<int32_t> &tmp) {
+    tmp.clear();
+    for (int64_t r = br * bs; r < (br + 1) * bs; ++r)
+      for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+        const int32_t bc = A.col[k] / bs;
+        if (bc != br) tmp.push_back(bc);
+      }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+  };
+  std::vector<int64_t> rp(nb + 1, 0);
+#pragma omp parallel
+  {
+    std::vector<int32_t> tmp;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t br = 0; br < nb; ++br) {
+      gather(br, tmp);
+      rp[br + 1] = static_cast<int64_t>(tmp.size());
     }
   }
-  Graph g;
-  g.ptr.assign(nb + 1, 0);
-  for (int64_t b = 0; b < nb; ++b) {
-    auto &v = adj[b];
-    std::sort(v.begin(), v.end());
-    v.erase(std::unique(v.begin(), v.end()), v.end());
-    g.ptr[b + 1] = g.ptr[b] + static_cast<int32_t>(v.size());
+  for (int64_t b = 0; b < nb; ++b) rp[b + 1] += rp[b];
+  std::vector<int32_t> rc(rp[nb]);
+#pragma omp parallel
+  {
+    std::vector<int32_t> tmp;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t br = 0; br < nb; ++br) {
+      gather(br, tmp);
+      std::copy(tmp.begin(), tmp.end(), rc.begin() + rp[br]);
+    }
   }
-  g.col.reserve(g.ptr[nb]);
-  for (auto &v : adj) g.col.insert(g.col.end(), v.begin(), v.end());
+  // Transpose (rows come out sorted: sources visited in increasing order).
+  std::vector<int64_t> tp(nb + 1, 0);
+  for (int32_t c : rc) tp[c + 1]++;
+  for (int64_t b = 0; b < nb; ++b) tp[b + 1] += tp[b];
+  std::vector<int32_t> tc(rp[nb]);
+  {
+    std::vector<int64_t> next(tp.begin(), tp.end() - 1);
+    for (int64_t br = 0; br < nb; ++br)
+      for (int64_t k = rp[br]; k < rp[br + 1]; ++k) tc[next[rc[k]]++] = static_cast<int32_t>(br);
+  }
+  // Merge R and T row by row.
+  auto merged = [&](int64_t b, int32_t *out) {
+    int64_t i = rp[b], j = tp[b], n = 0;
+    int32_t last = -1;
+    while (i < rp[b + 1] || j < tp[b + 1]) {
+      int32_t v;
+      if (j >= tp[b + 1] || (i < rp[b + 1] && rc[i] <= tc[j])) v = rc[i++];
+      else v = tc[j++];
+      if (v != last) {
+        if (out) out[n] = v;
+        ++n;
+        last = v;
+      }
+    }
+    return n;
+  };
+  Graph g;
+  std::vector<int64_t> gp(nb + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t b = 0; b < nb; ++b) gp[b + 1] = merged(b, nullptr);
+  for (int64_t b = 0; b < nb; ++b) gp[b + 1] += gp[b];
+  if (gp[nb] > INT32_MAX) throw std::runtime_error("buildBlockGraph: too many edges.");
+  g.ptr.resize(nb + 1);
+  for (int64_t b = 0; b <= nb; ++b) g.ptr[b] = static_cast<int32_t>(gp[b]);
+  g.col.resize(gp[nb]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t b = 0; b < nb; ++b) merged(b, g.col.data() + gp[b]);
   return g;
 }
 

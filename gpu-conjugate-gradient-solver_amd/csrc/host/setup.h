@@ -11,9 +11,36 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <memory>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace gsx {
+
+// Vector whose elements are default-initialised (left uninitialised for
+// arithmetic types): large packing buffers are written in parallel by their
+// packers, so a serial zero fill beforehand would only cost time.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  template <class U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <class U>
+  NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+  template <class U>
+  void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value) {
+    ::new (static_cast<void *>(p)) U;
+  }
+  template <class U, class... Args>
+  void construct(U *p, Args &&...args) {
+    ::new (static_cast<void *>(p)) U(std::forward<Args>(args)...);
+  }
+};
+template <class T>
+using RawVec = std::vector<T, NoInitAlloc<T>>;
 
 // Compressed sparse row matrix with int32 indices (the reference's index
 // type, include/GPUSolver/HostSparseMatrix.h:285-291) and FP64 values.
@@ -23,8 +50,8 @@ struct Csr {
   int64_t rows = 0;
   int64_t cols = 0;
   std::vector<int32_t> ptr;  // rows + 1
-  std::vector<int32_t> col;  // nnz
-  std::vector<double> val;   // nnz
+  RawVec<int32_t> col;      // nnz (default-initialised)
+  RawVec<double> val;       // nnz
   int64_t nnz() const { return static_cast<int64_t>(col.size()); }
   void check() const;        // structural validation, throws on error
 };
@@ -50,6 +77,12 @@ Csr permute_keep_order(const Csr &A, const Perm &p);
 Csr permute_sorted(const Csr &A, const int32_t *row_old_to_new,
                    const int32_t *col_old_to_new);
 
+// Keep-order relabelling: row i of the result is row rq[i] of A, column c
+// becomes cmap[c]; entry order inside each row is unchanged.
+Csr relabel(const Csr &A, const int32_t *rq, const int32_t *cmap);
+// Throws std::runtime_error(what) when a column index lies outside
+// [0, A.cols).
+void check_columns(const Csr &A, const char *what);
 Csr transpose(const Csr &A);                // sorted output
 Csr spgemm(const Csr &A, const Csr &B);     // sorted output, OpenMP
 void sort_rows(Csr &A);                     // sort + sum duplicates
@@ -72,25 +105,25 @@ struct Ell {
 Ell ell_pack(const Csr &A, int32_t width);
 int32_t max_row_length(const Csr &A);
 
-// 3x3-block ELL: block row b holds up to `width` blocks stored block-slot
-// major; the 9 values of block slot s of block row b sit at
-// val[(s * 9 + e) * nb + b] for e = 3 * r + c, so a wavefront reading 64
-// consecutive block rows issues 64-wide unit-stride loads per element.
-// A block row stores the union of the block columns of its three scalar
-// rows; blocks missing a scalar entry are padded with exact zeros.
-struct Bsr3Ell {
+// Row-contiguous 3x3 block CSR: block k of block row b
+// (ptr[b] <= k < ptr[b + 1]) has block column bcol[k] and values
+// val[9 k + 3 r + c]. A block row stores the union of the block columns of
+// its three scalar rows; blocks missing a scalar entry hold exact zeros.
+struct Bsr3Csr {
   int64_t nb = 0;            // block rows
+  int64_t nblocks = 0;       // stored blocks
   int32_t width = 0;         // max blocks per block row
-  std::vector<int32_t> bcol; // width * nb, block column index (padding: own row)
-  std::vector<double> val;   // 9 * width * nb
-  std::vector<int32_t> nblk; // blocks per block row
-  int64_t nblocks = 0;       // stored (non-padding) blocks
+  std::vector<int64_t> ptr;  // nb + 1
+  std::vector<int32_t> bcol;
+  RawVec<double> val;        // 9 * nblocks
 };
 // Returns false (and leaves out untouched) when A is not square with a
 // multiple of 3 rows. Blocks of a block row are stored in increasing
 // key[block column] (block column itself when key is null); duplicate
-// entries are summed.
-bool bsr3_pack(const Csr &A, Bsr3Ell &out, const int32_t *key = nullptr);
+// entries are summed in their CSR order.
+bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key = nullptr);
+// dst[i] = (float)src[i], in parallel.
+void to_float(const double *src, size_t n, float *dst);
 
 // SELL-64: the device layout of every scalar operator. Rows are cut into
 // slices of 64; slice s stores its rows slot-major with width slice_w[s]
@@ -118,9 +151,9 @@ struct Bsr3Sell64 {
   std::vector<int64_t> slice_ptr;
   std::vector<int32_t> slice_w;
   std::vector<int32_t> bcol;
-  std::vector<double> val;
+  RawVec<double> val;
 };
-Bsr3Sell64 bsr3_sell64_pack(const Bsr3Ell &B);
+Bsr3Sell64 bsr3_sell64_pack(const Bsr3Csr &B);
 
 // ---------------------------------------------------------------------------
 // Colouring and orderings.

@@ -101,41 +101,45 @@ void sort_rows(Csr &A) {
   std::vector<int32_t> counts(A.rows, 0);
 #pragma omp parallel
   {
-    std::vector<std::pair<int32_t, int64_t>> tmp;
+    // Sort key: column (sign bit flipped, so int32 order is unsigned order)
+    // above the entry's position in the row; keys are unique, so the order
+    // is that of a stable sort by column.
+    std::vector<uint64_t> key;
+    std::vector<double> v;
 #pragma omp for schedule(dynamic, 256)
     for (int64_t r = 0; r < A.rows; ++r) {
       const int64_t b = A.ptr[r], e = A.ptr[r + 1];
-      tmp.clear();
-      for (int64_t k = b; k < e; ++k) tmp.emplace_back(A.col[k], k);
-      std::stable_sort(tmp.begin(), tmp.end(),
-                       [](const std::pair<int32_t, int64_t> &x,
-                          const std::pair<int32_t, int64_t> &y) {
-                         return x.first < y.first;
-                       });
-      // Write the collapsed row in place after buffering values.
-      std::vector<double> v;
-      std::vector<int32_t> c;
-      v.reserve(tmp.size());
-      c.reserve(tmp.size());
-      for (size_t i = 0; i < tmp.size(); ++i) {
-        if (!c.empty() && c.back() == tmp[i].first)
-          v.back() += A.val[tmp[i].second];
-        else {
-          c.push_back(tmp[i].first);
-          v.push_back(A.val[tmp[i].second]);
+      key.resize(e - b);
+      for (int64_t k = b; k < e; ++k)
+        key[k - b] = (static_cast<uint64_t>(static_cast<uint32_t>(A.col[k]) ^ 0x80000000u) << 32) |
+                     static_cast<uint64_t>(k - b);
+      std::sort(key.begin(), key.end());
+      // Collapse duplicates (summed in stored order), buffering values.
+      v.clear();
+      int64_t n = 0;
+      int32_t last = 0;
+      for (size_t i = 0; i < key.size(); ++i) {
+        const int32_t c = static_cast<int32_t>(static_cast<uint32_t>(key[i] >> 32) ^ 0x80000000u);
+        const double x = A.val[b + static_cast<int64_t>(key[i] & 0xffffffffu)];
+        if (n > 0 && last == c) {
+          v.back() += x;
+        } else {
+          key[n++] = static_cast<uint64_t>(static_cast<uint32_t>(c));
+          v.push_back(x);
+          last = c;
         }
       }
-      for (size_t i = 0; i < c.size(); ++i) {
-        A.col[b + i] = c[i];
+      for (int64_t i = 0; i < n; ++i) {
+        A.col[b + i] = static_cast<int32_t>(static_cast<uint32_t>(key[i]));
         A.val[b + i] = v[i];
       }
-      counts[r] = static_cast<int32_t>(c.size());
+      counts[r] = static_cast<int32_t>(n);
     }
   }
   for (int64_t r = 0; r < A.rows; ++r) newptr[r + 1] = newptr[r] + counts[r];
   if (newptr[A.rows] != A.ptr[A.rows]) {
-    std::vector<int32_t> col(newptr[A.rows]);
-    std::vector<double> val(newptr[A.rows]);
+    RawVec<int32_t> col(newptr[A.rows]);
+    RawVec<double> val(newptr[A.rows]);
     for (int64_t r = 0; r < A.rows; ++r) {
       std::memcpy(&col[newptr[r]], &A.col[A.ptr[r]], counts[r] * sizeof(int32_t));
       std::memcpy(&val[newptr[r]], &A.val[A.ptr[r]], counts[r] * sizeof(double));
@@ -171,6 +175,33 @@ Csr permute_sorted(const Csr &A, const int32_t *row_o2n,
   }
   sort_rows(B);
   return B;
+}
+
+Csr relabel(const Csr &A, const int32_t *rq, const int32_t *cmap) {
+  Csr B;
+  B.rows = A.rows;
+  B.cols = A.cols;
+  B.ptr.assign(A.rows + 1, 0);
+  for (int64_t i = 0; i < A.rows; ++i) B.ptr[i + 1] = B.ptr[i] + (A.ptr[rq[i] + 1] - A.ptr[rq[i]]);
+  B.col.resize(A.nnz());
+  B.val.resize(A.nnz());
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < A.rows; ++i) {
+    int64_t d = B.ptr[i];
+    for (int32_t k = A.ptr[rq[i]]; k < A.ptr[rq[i] + 1]; ++k, ++d) {
+      B.col[d] = cmap[A.col[k]];
+      B.val[d] = A.val[k];
+    }
+  }
+  return B;
+}
+
+void check_columns(const Csr &A, const char *what) {
+  bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+  for (int64_t k = 0; k < static_cast<int64_t>(A.col.size()); ++k)
+    bad = bad || A.col[k] < 0 || A.col[k] >= A.cols;
+  if (bad) throw std::runtime_error(what);
 }
 
 Csr transpose(const Csr &A) {
@@ -309,60 +340,74 @@ Ell ell_pack(const Csr &A, int32_t width) {
   return E;
 }
 
-bool bsr3_pack(const Csr &A, Bsr3Ell &out, const int32_t *key) {
+bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key) {
   if (A.rows != A.cols || A.rows % 3 != 0) return false;
   const int64_t nb = A.rows / 3;
   auto rank = [key](int32_t cb) { return key ? key[cb] : cb; };
-  // Block columns of each block row: union over its three scalar rows,
-  // ordered by rank, so that a row whose entries come in rank order is
-  // summed in that same order by the block kernels.
-  std::vector<std::vector<int32_t>> bcols(nb);
-  int32_t width = 0;
-  int64_t nblocks = 0;
-#pragma omp parallel for schedule(static) reduction(max : width) reduction(+ : nblocks)
-  for (int64_t b = 0; b < nb; ++b) {
-    std::vector<int32_t> &bc = bcols[b];
-    for (int l = 0; l < 3; ++l) {
-      const int64_t r = 3 * b + l;
+  // Block columns of block row b: union over its three scalar rows, ordered
+  // by rank, so that a row whose entries come in rank order is summed in
+  // that same order by the block kernels.
+  auto gather = [&](int64_t b, std::vector<int32_t> &bc) {
+    bc.clear();
+    for (int64_t r = 3 * b; r < 3 * b + 3; ++r)
       for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) bc.push_back(A.col[k] / 3);
-    }
     std::sort(bc.begin(), bc.end(), [&](int32_t u, int32_t v) {
       return rank(u) != rank(v) ? rank(u) < rank(v) : u < v;
     });
     bc.erase(std::unique(bc.begin(), bc.end()), bc.end());
-    width = std::max(width, static_cast<int32_t>(bc.size()));
-    nblocks += static_cast<int64_t>(bc.size());
+  };
+  check_columns(A, "BSR upload: block column out of range.");
+  std::vector<int64_t> ptr(nb + 1, 0);
+  int32_t width = 0;
+#pragma omp parallel reduction(max : width)
+  {
+    std::vector<int32_t> bc;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t b = 0; b < nb; ++b) {
+      gather(b, bc);
+      ptr[b + 1] = static_cast<int64_t>(bc.size());
+      width = std::max(width, static_cast<int32_t>(bc.size()));
+    }
   }
-  out.nb = nb;
-  out.width = width;
-  out.nblocks = nblocks;
-  out.bcol.assign(static_cast<int64_t>(width) * nb, 0);
-  out.val.assign(static_cast<int64_t>(9) * width * nb, 0.0);
-  out.nblk.assign(nb, 0);
-#pragma omp parallel for schedule(static)
-  for (int64_t b = 0; b < nb; ++b) {
-    const std::vector<int32_t> &bc = bcols[b];
-    out.nblk[b] = static_cast<int32_t>(bc.size());
-    for (int32_t s = 0; s < width; ++s)
-      out.bcol[static_cast<int64_t>(s) * nb + b] =
-          s < static_cast<int32_t>(bc.size()) ? bc[s] : static_cast<int32_t>(b);
-    // Slot of each block column (bc holds distinct columns).
-    std::vector<std::pair<int32_t, int32_t>> slot_of(bc.size());
-    for (size_t s = 0; s < bc.size(); ++s) slot_of[s] = {bc[s], static_cast<int32_t>(s)};
-    std::sort(slot_of.begin(), slot_of.end());
-    for (int l = 0; l < 3; ++l) {
-      const int64_t r = 3 * b + l;
-      for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
-        const int32_t cb = A.col[k] / 3;
-        const auto it = std::lower_bound(slot_of.begin(), slot_of.end(),
-                                         std::make_pair(cb, static_cast<int32_t>(-1)));
-        const int64_t s = it->second;
-        const int e = 3 * l + (A.col[k] % 3);
-        out.val[(s * 9 + e) * nb + b] += A.val[k];
+  for (int64_t b = 0; b < nb; ++b) ptr[b + 1] += ptr[b];
+  const int64_t nblocks = ptr[nb];
+  std::vector<int32_t> bcol(nblocks);
+  RawVec<double> val(9 * nblocks);
+#pragma omp parallel
+  {
+    std::vector<int32_t> bc;
+    // slot[c]: slot of block column c in the current block row (only the
+    // row's own columns are written before they are read).
+    std::vector<int32_t> slot(nb);
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t b = 0; b < nb; ++b) {
+      gather(b, bc);
+      const int64_t k0 = ptr[b];
+      for (size_t s = 0; s < bc.size(); ++s) {
+        bcol[k0 + s] = bc[s];
+        slot[bc[s]] = static_cast<int32_t>(s);
+      }
+      double *v = val.data() + 9 * k0;
+      std::fill(v, v + 9 * bc.size(), 0.0);
+      for (int l = 0; l < 3; ++l) {
+        const int64_t r = 3 * b + l;
+        for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k)
+          v[9 * slot[A.col[k] / 3] + 3 * l + A.col[k] % 3] += A.val[k];
       }
     }
   }
+  out.nb = nb;
+  out.nblocks = nblocks;
+  out.width = width;
+  out.ptr = std::move(ptr);
+  out.bcol = std::move(bcol);
+  out.val = std::move(val);
   return true;
+}
+
+void to_float(const double *src, size_t n, float *dst) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) dst[i] = static_cast<float>(src[i]);
 }
 
 Sell64 sell64_pack(const Csr &A) {
@@ -429,7 +474,7 @@ Sell64 sell64_pack(const Csr &A) {
   return out;
 }
 
-Bsr3Sell64 bsr3_sell64_pack(const Bsr3Ell &B) {
+Bsr3Sell64 bsr3_sell64_pack(const Bsr3Csr &B) {
   constexpr int64_t S = 64;
   Bsr3Sell64 out;
   out.nb = B.nb;
@@ -438,31 +483,32 @@ Bsr3Sell64 bsr3_sell64_pack(const Bsr3Ell &B) {
   out.slice_ptr.assign(ns + 1, 0);
   out.slice_w.assign(ns, 0);
   for (int64_t s = 0; s < ns; ++s) {
-    int32_t w = 0;
-    for (int64_t b = s * S; b < std::min(B.nb, (s + 1) * S); ++b) w = std::max(w, B.nblk[b]);
-    out.slice_w[s] = w;
-    out.slice_ptr[s + 1] = out.slice_ptr[s] + static_cast<int64_t>(w) * S;
+    int64_t w = 0;
+    for (int64_t b = s * S; b < std::min(B.nb, (s + 1) * S); ++b)
+      w = std::max(w, B.ptr[b + 1] - B.ptr[b]);
+    out.slice_w[s] = static_cast<int32_t>(w);
+    out.slice_ptr[s + 1] = out.slice_ptr[s] + w * S;
   }
   out.slots = out.slice_ptr[ns];
-  out.bcol.assign(out.slots, 0);
-  out.val.assign(out.slots * 9, 0.0);
-#pragma omp parallel for schedule(static)
+  out.bcol.resize(out.slots);
+  out.val.resize(out.slots * 9);
+#pragma omp parallel for schedule(dynamic, 16)
   for (int64_t s = 0; s < ns; ++s)
-    for (int64_t l = 0; l < S; ++l) {
-      const int64_t b = s * S + l;
-      for (int32_t j = 0; j < out.slice_w[s]; ++j) {
-        const int64_t slot = out.slice_ptr[s] + static_cast<int64_t>(j) * S;
-        if (b < B.nb && j < B.nblk[b]) {
-          out.bcol[slot + l] = B.bcol[static_cast<int64_t>(j) * B.nb + b];
-          for (int e = 0; e < 9; ++e)
-            out.val[slot * 9 + e * S + l] = B.val[(static_cast<int64_t>(j) * 9 + e) * B.nb + b];
+    for (int32_t j = 0; j < out.slice_w[s]; ++j) {
+      const int64_t slot = out.slice_ptr[s] + static_cast<int64_t>(j) * S;
+      for (int64_t l = 0; l < S; ++l) {
+        const int64_t b = s * S + l;
+        double *v = out.val.data() + slot * 9 + l;
+        if (b < B.nb && j < B.ptr[b + 1] - B.ptr[b]) {
+          const int64_t k = B.ptr[b] + j;
+          out.bcol[slot + l] = B.bcol[k];
+          for (int e = 0; e < 9; ++e) v[e * S] = B.val[9 * k + e];
         } else {
           out.bcol[slot + l] = static_cast<int32_t>(b < B.nb ? b : 0);
+          for (int e = 0; e < 9; ++e) v[e * S] = 0.0;
         }
       }
     }
-  for (int32_t c : out.bcol)
-    if (c < 0 || c >= B.nb) throw std::runtime_error("BSR3-SELL-64 packing: block column out of range.");
   return out;
 }
 

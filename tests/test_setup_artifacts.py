@@ -209,6 +209,31 @@ def test_block_graph_matches_oracle():
     assert all(i != j for i, j in edges)
 
 
+@pytest.mark.parametrize("seed", [0, 1])
+def test_block_graph_nonsymmetric_pattern(seed):
+    """A one-sided coupling still yields the edge in both directions
+    (src/BlockUtils.cc:126-216); checked against the oracle and against a
+    set-based construction."""
+    rng = np.random.default_rng(seed)
+    n = 3 * 97
+    rows = [sorted(set(rng.integers(0, n, rng.integers(0, 9)).tolist()) | {i}) for i in range(n)]
+    ptr = np.zeros(n + 1, np.int32)
+    ptr[1:] = np.cumsum([len(r) for r in rows])
+    col = np.array([c for r in rows for c in r], np.int32)
+    gp, gc = gs.block_graph(as_csr(ptr, col), 3)
+    op, oc = orc.block_graph(ptr, col, 3)
+    np.testing.assert_array_equal(gp, op)
+    np.testing.assert_array_equal(gc, oc)
+    adj = [set() for _ in range(n // 3)]
+    for i, r in enumerate(rows):
+        for c in r:
+            if i // 3 != c // 3:
+                adj[i // 3].add(c // 3)
+                adj[c // 3].add(i // 3)
+    assert gp.tolist() == np.cumsum([0] + [len(a) for a in adj]).tolist()
+    assert gc.tolist() == [c for a in adj for c in sorted(a)]
+
+
 def test_expand_block_permutation():
     node = np.array([2, 0, 1], np.int32)
     o2n, n2o = gs.expand_block_permutation(node, 3)
