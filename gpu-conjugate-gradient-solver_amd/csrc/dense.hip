@@ -1,8 +1,9 @@
 // Dense FP64 SPD inverse on the device: the coarse-level direct solve.
 //
 // The coarse level is solved as x = A_c^-1 rhs with an explicit FP64
-// inverse (one bandwidth-bound row-per-wavefront product per V-cycle, see
-// k_dense_solve). Building that inverse on the host costs O(n^3) serial
+// inverse (one bandwidth-bound product per V-cycle over the inverse's lower
+// 64 x 64 tiles, see k_dense_sym_strip). Building that inverse on the host
+// costs O(n^3) serial
 // flops, minutes at n ~ 10^4; here it is blocked Gauss-Jordan elimination in
 // place on the device, 64 x 64 pivot blocks:
 //
@@ -227,9 +228,39 @@ void eliminate(int64_t N, double *d_inv, hipStream_t stream) {
   if (hflag) throw std::runtime_error("coarse level matrix is not SPD.");
 }
 
+// tiles[(I (I + 1) / 2 + J) 4096 + 64 r + c] = A[64 I + r][64 J + c], I >= J;
+// one workgroup per tile.
+__global__ __launch_bounds__(kThreads) void k_pack_lower_tiles(const double *__restrict__ A,
+                                                               int64_t N,
+                                                               double *__restrict__ tiles) {
+  const int64_t t = blockIdx.x;
+  // Tile (I, J) of linear index t: the largest I with I (I + 1) / 2 <= t.
+  int64_t I = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(t) + 1.0) - 1.0) / 2.0);
+  while (I * (I + 1) / 2 > t) --I;
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  const int64_t J = t - I * (I + 1) / 2;
+  for (int e = threadIdx.x; e < NB * NB; e += kThreads) {
+    const int r = e / NB, c = e % NB;
+    tiles[t * NB * NB + e] = A[(I * NB + r) * N + J * NB + c];
+  }
+}
+
 }  // namespace
 
 int64_t padded_size(int64_t n) { return (n + NB - 1) / NB * NB; }
+
+int64_t lower_tiles_size(int64_t n) {
+  const int64_t nt = padded_size(n) / NB;
+  return nt * (nt + 1) / 2 * NB * NB;
+}
+
+void pack_lower_tiles(int64_t n, const double *d_full, double *d_tiles, hipStream_t stream) {
+  const int64_t N = padded_size(n), nt = N / NB;
+  const int64_t ntiles = nt * (nt + 1) / 2;
+  if (ntiles > 0)
+    k_pack_lower_tiles<<<static_cast<uint32_t>(ntiles), kThreads, 0, stream>>>(d_full, N, d_tiles);
+  GS_DHIP(hipGetLastError());
+}
 
 void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
                         hipStream_t stream) {

@@ -21,4 +21,11 @@ void spd_inverse_device(int64_t n, const std::vector<double> &a, double *d_inv,
 void spd_inverse_device_csr(int64_t n, const int32_t *ptr, const int32_t *col,
                             const double *val, double *d_inv, hipStream_t stream);
 
+// Doubles held by the lower 64 x 64 tiles of the padded inverse.
+int64_t lower_tiles_size(int64_t n);
+// Copies the lower tiles (I >= J) of the padded row-major inverse d_full
+// (row stride padded_size(n)) to d_tiles: tile (I, J) at
+// (I (I + 1) / 2 + J) * 4096, row-major.
+void pack_lower_tiles(int64_t n, const double *d_full, double *d_tiles, hipStream_t stream);
+
 }  // namespace gsdense

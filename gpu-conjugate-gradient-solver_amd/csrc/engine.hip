@@ -91,8 +91,12 @@ struct Level {
   std::vector<int64_t> coff;  // colour offsets (rows, or blocks for block GS)
   std::vector<int64_t> cunits;  // stored entries (blocks for block GS) per colour
   int ncolors = 0;
-  DevBuf diag, dinv, inv;
-  int64_t inv_ld = 0;  // row stride of the dense inverse
+  DevBuf diag, dinv;
+  // Direct level: lower 64 x 64 tiles of the symmetric FP64 inverse, the
+  // strip table (tile column, first and last tile row) and the partial sums
+  // of the tiled product.
+  DevBuf inv, sym_strips, sym_prow, sym_pcol;
+  int64_t sym_nt = 0, sym_kmax = 0, sym_nstrips = 0;
   DevBuf rhs_buf, x_buf, res, tmp;
   TM *rhs = nullptr, *x = nullptr;  // bound to own buffers or the CG vectors
   DMat<TM> R, P;                     // transfer to level k + 1
@@ -239,10 +243,29 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
       throw std::runtime_error("coarsest level has " + std::to_string(lv.n) +
                                " unknowns, above the dense direct-solve limit of " +
                                std::to_string(opt_.max_direct) + "; coarsen further.");
-    lv.inv_ld = gsdense::padded_size(lv.n);
-    lv.inv.alloc(static_cast<size_t>(lv.inv_ld) * lv.inv_ld * sizeof(double));
-    gsdense::spd_inverse_device_csr(lv.n, Ai.ptr.data(), Ai.col.data(), Ai.val.data(),
-                                    lv.inv.template as<double>(), stream_);
+    const int64_t N = gsdense::padded_size(lv.n);
+    {
+      DevBuf full(static_cast<size_t>(N) * N * sizeof(double));
+      gsdense::spd_inverse_device_csr(lv.n, Ai.ptr.data(), Ai.col.data(), Ai.val.data(),
+                                      full.as<double>(), stream_);
+      lv.inv.alloc(static_cast<size_t>(gsdense::lower_tiles_size(lv.n)) * sizeof(double));
+      gsdense::pack_lower_tiles(lv.n, full.as<double>(), lv.inv.template as<double>(), stream_);
+      GS_HIP(hipStreamSynchronize(stream_));
+    }
+    const int64_t nt = N / 64, S = gsk::kSymStrip;
+    std::vector<int32_t> strips;
+    for (int64_t J = 0; J < nt; ++J)
+      for (int64_t I0 = J; I0 < nt; I0 += S) {
+        strips.push_back(static_cast<int32_t>(J));
+        strips.push_back(static_cast<int32_t>(I0));
+        strips.push_back(static_cast<int32_t>(std::min(I0 + S - 1, nt - 1)));
+      }
+    lv.sym_nt = nt;
+    lv.sym_kmax = (nt + S - 1) / S;
+    lv.sym_nstrips = static_cast<int64_t>(strips.size() / 3);
+    lv.sym_strips.upload(strips);
+    lv.sym_prow.alloc(static_cast<size_t>(nt * (nt + 1) / 2) * 64 * sizeof(double));
+    lv.sym_pcol.alloc(static_cast<size_t>(nt * lv.sym_kmax) * 64 * sizeof(double));
   }
   if (lv.direct) mark("level " + std::to_string(k) + " dense inverse");
   // Blocks summed in the level's own (pre-relabelling) column order, the
@@ -559,8 +582,12 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
   Level<TM> &lv = L_[k];
   if (k == nlev_ - 1) {
     if (lv.direct) {
-      gsk::k_dense_solve<TM><<<static_cast<uint32_t>((lv.n + 3) / 4), kBlock, 0, stream_>>>(
-          lv.inv.template as<double>(), lv.n, lv.inv_ld, lv.rhs, lv.x, stop);
+      double *prow = lv.sym_prow.template as<double>(), *pcol = lv.sym_pcol.template as<double>();
+      gsk::k_dense_sym_strip<TM><<<static_cast<uint32_t>(lv.sym_nstrips), kBlock, 0, stream_>>>(
+          lv.inv.template as<double>(), lv.sym_strips.template as<int32_t>(), lv.n, lv.sym_nt,
+          lv.sym_kmax, lv.rhs, prow, pcol, stop);
+      gsk::k_dense_sym_reduce<TM><<<static_cast<uint32_t>(lv.sym_nt), 1024, 0, stream_>>>(
+          prow, pcol, lv.n, lv.sym_nt, lv.sym_kmax, lv.x, stop);
     } else {
       encode_smooth(lv, opt_.coarse_steps, zero_x, stop);
     }
@@ -655,8 +682,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   SolveResult res;
   // src/CGSolver.cc:113-118: a right-hand side below 1e-12 in norm returns
   // x = 0 without iterating.
-  double nb = 0.0;
-  for (int64_t i = 0; i < n_; ++i) nb += b[i] * b[i];
+  const double nb = gsx::sum_squares(b, n_);
   if (std::sqrt(nb) < 1e-12) {
     for (int64_t i = 0; i < n_; ++i) x[i] = 0.0;
     res.iterations = 0;
@@ -665,10 +691,8 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   }
   // Gather into pinned staging (internal order), one DMA each way.
   TC *bi = stage_, *xi = stage_ + n_;
-  for (int64_t i = 0; i < n_; ++i) {
-    bi[i] = static_cast<TC>(b[int2orig_[i]]);
-    xi[i] = static_cast<TC>(x[int2orig_[i]]);
-  }
+  gsx::gather_cast(b, int2orig_.data(), n_, bi);
+  gsx::gather_cast(x, int2orig_.data(), n_, xi);
   GS_HIP(hipMemcpyAsync(b_.get(), bi, n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
   GS_HIP(hipMemcpyAsync(x_.get(), xi, n_ * sizeof(TC), hipMemcpyHostToDevice, stream_));
   if (hist_cap_ < maxit + 1) {
@@ -740,7 +764,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   // Solution back to the caller's ordering.
   GS_HIP(hipMemcpyAsync(xi, x_.get(), n_ * sizeof(TC), hipMemcpyDeviceToHost, stream_));
   GS_HIP(hipStreamSynchronize(stream_));
-  for (int64_t i = 0; i < n_; ++i) x[int2orig_[i]] = static_cast<double>(xi[i]);
+  gsx::scatter_cast(xi, int2orig_.data(), n_, x);
   // True residual of the CG operator in FP64 arithmetic.
   const uint32_t gt = grid_for(Acg_.b3 ? n_ / 3 : n_);
   if (Acg_.b3)

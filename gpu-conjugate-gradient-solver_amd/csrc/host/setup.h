@@ -124,6 +124,15 @@ struct Bsr3Csr {
 bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key = nullptr);
 // dst[i] = (float)src[i], in parallel.
 void to_float(const double *src, size_t n, float *dst);
+// Parallel permuted copies with a cast: gather dst[i] = src[idx[i]],
+// scatter dst[idx[i]] = src[i], for i < n.
+void gather_cast(const double *src, const int32_t *idx, int64_t n, double *dst);
+void gather_cast(const double *src, const int32_t *idx, int64_t n, float *dst);
+void scatter_cast(const double *src, const int32_t *idx, int64_t n, double *dst);
+void scatter_cast(const float *src, const int32_t *idx, int64_t n, double *dst);
+// sum of v[i]^2 (parallel; the order of the partial sums is fixed for a
+// given thread count).
+double sum_squares(const double *v, int64_t n);
 
 // SELL-64: the device layout of every scalar operator. Rows are cut into
 // slices of 64; slice s stores its rows slot-major with width slice_w[s]

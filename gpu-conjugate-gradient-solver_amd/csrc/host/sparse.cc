@@ -410,6 +410,36 @@ void to_float(const double *src, size_t n, float *dst) {
   for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) dst[i] = static_cast<float>(src[i]);
 }
 
+template <class D>
+static void gather_cast_t(const double *src, const int32_t *idx, int64_t n, D *dst) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) dst[i] = static_cast<D>(src[idx[i]]);
+}
+template <class S>
+static void scatter_cast_t(const S *src, const int32_t *idx, int64_t n, double *dst) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) dst[idx[i]] = static_cast<double>(src[i]);
+}
+void gather_cast(const double *src, const int32_t *idx, int64_t n, double *dst) {
+  gather_cast_t(src, idx, n, dst);
+}
+void gather_cast(const double *src, const int32_t *idx, int64_t n, float *dst) {
+  gather_cast_t(src, idx, n, dst);
+}
+void scatter_cast(const double *src, const int32_t *idx, int64_t n, double *dst) {
+  scatter_cast_t(src, idx, n, dst);
+}
+void scatter_cast(const float *src, const int32_t *idx, int64_t n, double *dst) {
+  scatter_cast_t(src, idx, n, dst);
+}
+
+double sum_squares(const double *v, int64_t n) {
+  double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (int64_t i = 0; i < n; ++i) s += v[i] * v[i];
+  return s;
+}
+
 Sell64 sell64_pack(const Csr &A) {
   constexpr int64_t S = 64;
   Sell64 out;

@@ -614,31 +614,126 @@ __global__ __launch_bounds__(kBlock) void k_fill(T *__restrict__ y, T v, int64_t
   if (i < n) y[i] = v;
 }
 
-// Coarsest level: x = A_c^-1 rhs with the FP64 dense inverse (row stride
-// ld, a multiple of 64; entries past column n of a row are zero), one
-// wavefront per row.
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_dense_solve(const double *__restrict__ inv, int64_t n,
-                                                        int64_t ld, const T *__restrict__ rhs,
-                                                        T *__restrict__ x, const int32_t *stop) {
-  if (stopped(stop)) return;
-  const int64_t row = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+// Coarsest level: x = A_c^-1 rhs with the symmetric FP64 dense inverse,
+// stored as its lower 64 x 64 tiles only: tile (I, J), I >= J,
+// at tiles + (I (I + 1) / 2 + J) * 4096, row-major. Each tile is read once
+// and applied twice: its row sums T x_J go to y_I and, off the diagonal, its
+// column sums T^T x_I to y_J. Workgroups take strips of kSymStrip tiles of
+// one tile column J (tile rows I0..I1); the partial sums land in
+//   prow[(I (I + 1) / 2 + J) 64 + r]   row sums of tile (I, J),
+//   pcol[(J kmax + s) 64 + c]          column sums of strip s of column J,
+// and k_dense_sym_reduce adds them per output block in a fixed order.
+constexpr int kSymStrip = 4;
+
+// Sums 16 row-vectors p[0..15] across the 64 lanes of a wavefront: each
+// halving step keeps half of the vectors in a lane and adds the partner
+// lane's copy of them. Lane l ends with the sum of vector l >> 2 when
+// (l & 3) == 0.
+__device__ __forceinline__ double wave_rowsums16(const double p[16]) {
   const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  // Two inverse entries per lane and step (16-byte loads; ld is even and
-  // the buffer 256-byte aligned, so a + j is 16-byte aligned for even j).
-  const double2 *a = reinterpret_cast<const double2 *>(inv + row * ld);
-  double s0 = 0.0, s1 = 0.0;
-#pragma unroll 4
-  for (int64_t j = 2 * lane; j < n; j += 128) {
-    const double2 av = a[j >> 1];
-    const double r0 = static_cast<double>(rhs[j]);
-    const double r1 = j + 1 < n ? static_cast<double>(rhs[j + 1]) : 0.0;
-    s0 += av.x * r0;
-    s1 += av.y * r1;
+  const bool u5 = lane & 32, u4 = lane & 16, u3 = lane & 8, u2 = lane & 4;
+  double q8[8], q4[4], q2[2];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const double send = u5 ? p[i] : p[8 + i];
+    const double keep = u5 ? p[8 + i] : p[i];
+    q8[i] = keep + __shfl_xor(send, 32, 64);
   }
-  const double s = wave_sum(s0 + s1);
-  if (lane == 0) x[row] = static_cast<T>(s);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double send = u4 ? q8[i] : q8[4 + i];
+    const double keep = u4 ? q8[4 + i] : q8[i];
+    q4[i] = keep + __shfl_xor(send, 16, 64);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double send = u3 ? q4[i] : q4[2 + i];
+    const double keep = u3 ? q4[2 + i] : q4[i];
+    q2[i] = keep + __shfl_xor(send, 8, 64);
+  }
+  const double send = u2 ? q2[0] : q2[1];
+  const double keep = u2 ? q2[1] : q2[0];
+  double r = keep + __shfl_xor(send, 4, 64);
+  r += __shfl_xor(r, 2, 64);
+  r += __shfl_xor(r, 1, 64);
+  return r;
+}
+
+// One workgroup per strip; strips[3 s .. 3 s + 2] = (J, I0, I1). Wave w
+// holds rows w, w + 4, ..., w + 60 of each tile, one column per lane, so
+// every load is one contiguous 512-byte tile row.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dense_sym_strip(const double *__restrict__ tiles,
+                                                            const int32_t *__restrict__ strips,
+                                                            int64_t n, int64_t nt, int64_t kmax,
+                                                            const T *__restrict__ rhs,
+                                                            double *__restrict__ prow,
+                                                            double *__restrict__ pcol,
+                                                            const int32_t *stop) {
+  if (stopped(stop)) return;
+  __shared__ double csum[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wu = __builtin_amdgcn_readfirstlane(w);
+  const int64_t J = strips[3 * blockIdx.x], I0 = strips[3 * blockIdx.x + 1],
+                I1 = strips[3 * blockIdx.x + 2];
+  // x entries past n are zero (clamped in-bounds load, then select).
+  const int64_t gj = 64 * J + lane;
+  const double xjv = static_cast<double>(rhs[gj < n ? gj : n - 1]);
+  const double xj = gj < n ? xjv : 0.0;
+  double v = 0.0;
+  for (int64_t I = I0; I <= I1; ++I) {
+    const double *t = tiles + (I * (I + 1) / 2 + J) * 4096;
+    double a[16], xi[16], p[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = t[(w + 4 * k) * 64 + lane];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const int64_t gi = 64 * I + wu + 4 * k;  // wave-uniform: scalar loads
+      const double xv = static_cast<double>(rhs[gi < n ? gi : n - 1]);
+      xi[k] = gi < n ? xv : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) p[k] = a[k] * xj;
+    if (I != J) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v += a[k] * xi[k];
+    }
+    const double rs = wave_rowsums16(p);
+    if ((lane & 3) == 0) prow[(I * (I + 1) / 2 + J) * 64 + w + 4 * (lane >> 2)] = rs;
+  }
+  csum[w][lane] = v;
+  __syncthreads();
+  if (w == 0)
+    pcol[(J * kmax + (I0 - J) / kSymStrip) * 64 + lane] =
+        (csum[0][lane] + csum[1][lane]) + (csum[2][lane] + csum[3][lane]);
+}
+
+// y[64 I + r] = sum over J <= I of prow[I][J][r] plus the column-sum
+// strips of tile column I, 16 waves per block I, combined in a fixed order.
+template <class T>
+__global__ __launch_bounds__(1024) void k_dense_sym_reduce(const double *__restrict__ prow,
+                                                           const double *__restrict__ pcol,
+                                                           int64_t n, int64_t nt, int64_t kmax,
+                                                           T *__restrict__ y,
+                                                           const int32_t *stop) {
+  if (stopped(stop)) return;
+  __shared__ double accw[16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t I = blockIdx.x;
+  const int64_t ncs = (nt - I + kSymStrip - 1) / kSymStrip;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int64_t J = w; J <= I; J += 16) acc += prow[(I * (I + 1) / 2 + J) * 64 + lane];
+  for (int64_t k = w; k < ncs; k += 16) acc += pcol[(I * kmax + k) * 64 + lane];
+  accw[w][lane] = acc;
+  __syncthreads();
+  if (w == 0) {
+    double t = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; ++u) t += accw[u][lane];
+    const int64_t row = I * 64 + lane;
+    if (row < n) y[row] = static_cast<T>(t);
+  }
 }
 
 // ---------------------------------------------------------------------------

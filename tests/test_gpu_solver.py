@@ -229,6 +229,32 @@ def test_pcg_coarse_direct(cfg1):
     assert true <= 1.5e-8
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 333])
+def test_direct_level_tiled_product(n):
+    """A one-level hierarchy with the direct coarse solve: the V-cycle is
+    x = A^-1 r through the lower-tile symmetric product, at sizes on both
+    sides of the 64-row tile; checked against numpy's dense solve."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((n, n))
+    dense = B @ B.T + n * np.eye(n)
+    dense[np.abs(dense) < 0.5] = 0.0  # sparsify off-diagonal entries
+    dense = 0.5 * (dense + dense.T) + n * np.eye(n)
+    A = gs.Csr.from_scipy(sp.csr_matrix(dense))
+    H = gs.Hierarchy.from_levels([A], [], [])
+    r = rng.standard_normal(n)
+    ref = np.linalg.solve(dense, r)
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_direct=1)
+    s.setup(H)
+    assert s.describe()["levels"][0]["direct"]
+    z = s.vcycle(r)
+    assert np.linalg.norm(z - ref) <= 1e-12 * np.linalg.norm(ref)
+    m = gs.Solver(gs.PREC_MIXED, smoother=gs.SGS, coarse_direct=1)
+    m.setup(H)
+    zm = m.vcycle(r)
+    assert np.linalg.norm(zm - ref) <= 1e-6 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
 def test_pcg_parity_elasticity_block_gs(elast, prec):
     E, rhs, H = elast
