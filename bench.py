@@ -66,7 +66,7 @@ def build_workload(name: str):
 
 def measured_traffic(config: str, kernel_id: str):
     """HBM bytes per launch of kernel `kernel_id` (cg_spmv, l0_sweep,
-    l1_sweep) from the committed rocprofv3 counter
+    l1_sweep, l0_restrict, l0_prolong) from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor the same
     counter reads for a copy of known size), or None when none was taken
     for this workload and kernel."""
@@ -227,6 +227,17 @@ def main() -> None:
                                 f", FP32, headline mode)",
                       "ms": sw_ms, "bytes": sw_bytes, "launches": 2 * lv["colors"] - 1,
                       "per_iteration_ms": sw_ms * (lv["pre"] + lv["post"])})
+    if len(hdesc["levels"]) > 1:
+        # Level-0 transfers of the headline solver, once each per V-cycle.
+        l0 = hdesc["levels"][0]
+        for kid, tid, what, grp in (("l0_restrict", 5, "restriction R0 (level 0 -> 1)", l0["r_group"]),
+                                    ("l0_prolong", 6, "prolongation P0 (level 1 -> 0)", l0["p_group"])):
+            t_ms, t_bytes = hs.time_kernel(tid, 20)
+            cands.append({"id": kid,
+                          "kernel": f"level-0 {what} ("
+                                    f"{'lane groups of %d' % grp if grp else 'thread per row, SELL-C-sigma'}"
+                                    f", FP32, headline mode)",
+                          "ms": t_ms, "bytes": t_bytes, "launches": 1, "per_iteration_ms": t_ms})
     dom = max(cands[1:] or cands, key=lambda c: c["per_iteration_ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     desc_plain = plain.describe()

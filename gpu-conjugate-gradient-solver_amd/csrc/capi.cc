@@ -441,6 +441,11 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "reference_vcycle") o.reference_vcycle = v != 0;
     else if (k == "max_direct") o.max_direct = v;
     else if (k == "direct_truncate") o.direct_truncate = v != 0;
+    else if (k == "transfer_level_order") o.transfer_level_order = v != 0;
+    else if (k == "transfer_sigma") {
+      if (v < 0) throw std::runtime_error("transfer_sigma must be non-negative.");
+      o.transfer_sigma = v;
+    }
     else if (k == "group_max_transfer") {
       if (v != 16 && v != 32 && v != 64)
         throw std::runtime_error("group_max_transfer must be 16, 32 or 64.");

@@ -100,10 +100,23 @@ struct DSell {
   int64_t rows = 0, cols = 0;
   int64_t nnz = 0;    // stored CSR entries
   int64_t slots = 0;  // including padding
-  DevBuf slice_ptr, slice_w, col, val;
+  DevBuf slice_ptr, slice_w, col, val, row_map;
+  bool mapped = false;  // rows sorted by length within windows (row_map)
 
-  void build(const gsx::Csr &A) {
-    const gsx::Sell64 P = gsx::sell64_pack(A);
+  // sigma > 0 (non-square operators only): SELL-C-sigma row order.
+  // order (non-square operators only): stored position p holds row
+  // order[p] before any sorting.
+  void build(const gsx::Csr &A, int64_t sigma = 0, const int32_t *order = nullptr) {
+    gsx::Sell64 P;
+    if (order) {
+      const gsx::Csr B = gsx::take_rows(A, order, A.rows);
+      P = sigma > 0 ? gsx::sell64_pack_sorted(B, sigma) : gsx::sell64_pack(B);
+      if (P.row_map.empty()) P.row_map.assign(order, order + A.rows);
+      else
+        for (auto &r : P.row_map) r = order[r];
+    } else {
+      P = sigma > 0 ? gsx::sell64_pack_sorted(A, sigma) : gsx::sell64_pack(A);
+    }
     rows = P.rows;
     cols = P.cols;
     nnz = P.nnz;
@@ -112,13 +125,17 @@ struct DSell {
     slice_w.upload(P.slice_w);
     col.upload(P.col);
     upload_values<T>(val, P.val.data(), P.val.size());
+    mapped = !P.row_map.empty();
+    if (mapped) row_map.upload(P.row_map);
+    else row_map.release();
   }
   gsk::Sell<T> view() const {
     return gsk::Sell<T>{rows, cols, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(),
-                        col.as<int32_t>(), val.as<T>()};
+                        col.as<int32_t>(), val.as<T>(),
+                        mapped ? row_map.as<int32_t>() : nullptr};
   }
   size_t device_bytes() const {
-    return slice_ptr.bytes() + slice_w.bytes() + col.bytes() + val.bytes();
+    return slice_ptr.bytes() + slice_w.bytes() + col.bytes() + val.bytes() + row_map.bytes();
   }
 };
 
@@ -153,8 +170,16 @@ struct DBsr3 {
 template <class T>
 struct DRows {
   int64_t rows = 0, cols = 0, nnz = 0;
-  DevBuf ptr, col, val;
-  void build(const gsx::Csr &A) {
+  DevBuf ptr, col, val, row_map;
+  bool mapped = false;
+  // order: stored position p holds row order[p] (null: natural order).
+  void build(const gsx::Csr &A, const int32_t *order = nullptr) {
+    if (order) {
+      build(gsx::take_rows(A, order, A.rows));
+      row_map.upload(order, static_cast<size_t>(A.rows));
+      mapped = true;
+      return;
+    }
     rows = A.rows;
     cols = A.cols;
     nnz = A.nnz();
@@ -163,9 +188,12 @@ struct DRows {
     ptr.upload(p);
     col.upload(A.col);
     upload_values<T>(val, A.val.data(), A.val.size());
+    mapped = false;
+    row_map.release();
   }
   gsk::Rows<T> view() const {
-    return gsk::Rows<T>{rows, cols, ptr.as<int64_t>(), col.as<int32_t>(), val.as<T>()};
+    return gsk::Rows<T>{rows, cols, ptr.as<int64_t>(), col.as<int32_t>(), val.as<T>(),
+                        mapped ? row_map.as<int32_t>() : nullptr};
   }
 };
 
@@ -210,16 +238,30 @@ inline int pick_block_group(double avg, double min_avg) {
 template <class T>
 struct DMat {
   int G = 0;  // lanes per row; 0: SELL-64, one thread per row
+  int K = 2;  // lane groups: entries per lane and load round (2, 4 or 8)
   int64_t rows = 0, cols = 0, nnz = 0;
   DSell<T> s;
   DRows<T> w;
-  void build(const gsx::Csr &A, double group_min, int group_max = 64) {
+  // sigma: SELL-C-sigma window for the thread-per-row layout (0: none).
+  // order (non-square operators only): rows are executed in this order,
+  // order[p] being the row at position p (null: natural order).
+  void build(const gsx::Csr &A, double group_min, int group_max = 64, int64_t sigma = 0,
+             const int32_t *order = nullptr) {
     rows = A.rows;
     cols = A.cols;
     nnz = A.nnz();
-    G = pick_group(rows ? static_cast<double>(nnz) / rows : 0.0, group_min, group_max);
-    if (G) w.build(A);
-    else s.build(A);
+    const bool square = A.rows == A.cols;
+    const double mean = rows ? static_cast<double>(nnz) / rows : 0.0;
+    G = pick_group(mean, group_min, group_max);
+    // One load round for rows up to 1.25x the mean length where possible.
+    K = 8;
+    for (int k : {2, 4})
+      if (G && G * k >= 1.25 * mean) {
+        K = k;
+        break;
+      }
+    if (G) w.build(A, square ? nullptr : order);
+    else s.build(A, square ? 0 : sigma, square ? nullptr : order);
   }
 };
 

@@ -63,6 +63,18 @@ void launch_fill(T *p, T v, int64_t n, hipStream_t s, const int32_t *stop) {
   gsk::k_fill<T><<<grid_for(n), kBlock, 0, s>>>(p, v, n, stop);
 }
 
+// Calls f with the transfer rows' entries per lane and round (2, 4, 8) as a
+// compile-time constant.
+template <class F>
+void with_k(int K, F &&f) {
+  switch (K) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    default: throw std::runtime_error("entries per lane must be 2, 4 or 8.");
+  }
+}
+
 // Calls f with the lane-group size G as a compile-time constant.
 template <class F>
 void with_group(int G, F &&f) {
@@ -152,6 +164,8 @@ class Engine final : public EngineBase {
   void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop);
   void encode_iteration(const int32_t *stop);
   void encode_residual(Level<TM> &lv, const int32_t *stop);
+  void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop);
+  void encode_prolong(int k, bool alias, const int32_t *stop);
   void capture_graph();
   void apply_preconditioner(const int32_t *stop);
 
@@ -384,10 +398,12 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
         throw std::runtime_error("MultigridSmoother: transfer operator shape mismatch.");
       const std::vector<int32_t> qf_inv = inverse_of(L_[k].q);
       const std::vector<int32_t> qc_inv = inverse_of(L_[k + 1].q);
+      // Level order: position p holds the internal row of level index p.
+      const bool lo = opt_.transfer_level_order;
       L_[k].P.build(relabel(Pk, L_[k].q, qc_inv), opt_.group_min_scalar,
-                    opt_.group_max_transfer);
+                    opt_.group_max_transfer, opt_.transfer_sigma, lo ? qf_inv.data() : nullptr);
       L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv), opt_.group_min_scalar,
-                    opt_.group_max_transfer);
+                    opt_.group_max_transfer, opt_.transfer_sigma, lo ? qc_inv.data() : nullptr);
       mark("transfers " + std::to_string(k));
     }
   }
@@ -606,28 +622,47 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
       !cl.direct && ((k + 1 == nlev_ - 1) ? opt_.coarse_steps > 0 : cl.pre > 0) &&
       (cl.kind == kSGS || cl.kind == kBlockGS);
   TM *corr_c = coarse_self_zero ? nullptr : cl.x;
+  encode_restrict(k, res, corr_c, stop);
+  encode_vcycle(k + 1, stop, true);
+  encode_prolong(k, opt_.reference_vcycle && k > 0, stop);
+  encode_smooth(lv, lv.post, false, stop);
+}
+
+// rhs_{k+1} = R_k res (and corr_c = 0 when given).
+template <class TC, class TM>
+void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop) {
+  Level<TM> &lv = L_[k];
+  Level<TM> &cl = L_[k + 1];
   if (lv.R.G)
     with_group(lv.R.G, [&](auto g) {
-      constexpr int G = decltype(g)::value;
-      gsk::k_restrict_w<TM, G><<<gsk::grid_rows<G>(cl.n), kBlock, 0, stream_>>>(
-          lv.R.w.view(), res, cl.rhs, corr_c, stop);
+      with_k(lv.R.K, [&](auto kk) {
+        constexpr int G = decltype(g)::value, K = decltype(kk)::value;
+        gsk::k_restrict_w<TM, G, K><<<gsk::grid_rows<G>(cl.n), kBlock, 0, stream_>>>(
+            lv.R.w.view(), res, cl.rhs, corr_c, stop);
+      });
     });
   else
     gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(lv.R.s.view(), res, cl.rhs,
                                                                 corr_c, stop);
-  encode_vcycle(k + 1, stop, true);
-  const bool alias = opt_.reference_vcycle && k > 0;
+}
+
+// x_k += P_k x_{k+1} (alias: the reference's coarse-level x = 2 P e).
+template <class TC, class TM>
+void Engine<TC, TM>::encode_prolong(int k, bool alias, const int32_t *stop) {
+  Level<TM> &lv = L_[k];
+  Level<TM> &cl = L_[k + 1];
   if (lv.P.G)
     with_group(lv.P.G, [&](auto g) {
-      constexpr int G = decltype(g)::value;
-      gsk::k_prolong_w<TM, G><<<gsk::grid_rows<G>(lv.n), kBlock, 0, stream_>>>(
-          lv.P.w.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+      with_k(lv.P.K, [&](auto kk) {
+        constexpr int G = decltype(g)::value, K = decltype(kk)::value;
+        gsk::k_prolong_w<TM, G, K><<<gsk::grid_rows<G>(lv.n), kBlock, 0, stream_>>>(
+            lv.P.w.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+      });
     });
   else if (alias)
     gsk::k_prolong_alias<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), cl.x, lv.x, stop);
   else
     gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), cl.x, lv.x, stop);
-  encode_smooth(lv, lv.post, false, stop);
 }
 
 template <class TC, class TM>
@@ -838,15 +873,31 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
       apply_preconditioner(nullptr);
     } else if (which == 2) {
       encode_iteration(nullptr);
+    } else if (which == 5) {
+      encode_restrict(0, L_[0].res.template as<TM>(), L_[1].x, nullptr);
+    } else if (which == 6) {
+      encode_prolong(0, false, nullptr);
     } else {
       encode_smooth(L_[which - 3], 1, false, nullptr);
     }
   };
+  if (which < 0 || which > 6) throw std::runtime_error("time_kernel: unknown kernel id.");
+  if ((which == 5 || which == 6) && (opt_.precond != kPrecMultigrid || nlev_ < 2))
+    throw std::runtime_error("time_kernel: no level-0 transfer.");
   if ((which == 3 || which == 4) &&
       (which - 3 >= nlev_ || L_[which - 3].direct ||
        (L_[which - 3].kind != kSGS && L_[which - 3].kind != kBlockGS)))
     throw std::runtime_error("time_kernel: that level has no Gauss-Seidel sweep.");
   if (which == 0) *bytes = Acg_.spmv_bytes();
+  if (which == 5 || which == 6) {
+    // Entries (value + column) + the input vector once + the output vector
+    // (restriction: rhs_c and the zeroed correction written; prolongation:
+    // x read and written).
+    const double s = sizeof(TM);
+    const auto &T = which == 5 ? L_[0].R : L_[0].P;
+    *bytes = T.nnz * (s + 4) + (which == 5 ? L_[0].n * s + 2.0 * L_[1].n * s
+                                           : L_[1].n * s + 2.0 * L_[0].n * s);
+  }
   if (which == 3 || which == 4) {
     const double s = sizeof(TM);
     const auto &lv = L_[which - 3];

@@ -43,6 +43,15 @@ struct Options {
   // 32 measured best for the ~190-entry rows of R = P^T (two rows per
   // wavefront, two load rounds per row).
   int group_max_transfer = 32;
+  // Thread-per-row transfer operators: rows sorted by length within
+  // windows of this many rows (SELL-C-sigma), cutting slice padding; 0 keeps
+  // the natural row order.
+  int64_t transfer_sigma = 1024;
+  // Execute the transfer operators' rows in the level's own (pre-colouring)
+  // order instead of the colour order, so that rows running together are
+  // spatial neighbours and share their gathers in cache; outputs are
+  // written through the row map, sums unchanged.
+  bool transfer_level_order = true;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };

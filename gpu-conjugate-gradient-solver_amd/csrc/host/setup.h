@@ -80,6 +80,8 @@ Csr permute_sorted(const Csr &A, const int32_t *row_old_to_new,
 // Keep-order relabelling: row i of the result is row rq[i] of A, column c
 // becomes cmap[c]; entry order inside each row is unchanged.
 Csr relabel(const Csr &A, const int32_t *rq, const int32_t *cmap);
+// Row p of the result is row rows[p] of A (entries unchanged), p < nrows.
+Csr take_rows(const Csr &A, const int32_t *rows, int64_t nrows);
 // Throws std::runtime_error(what) when a column index lies outside
 // [0, A.cols).
 void check_columns(const Csr &A, const char *what);
@@ -148,8 +150,16 @@ struct Sell64 {
   std::vector<int32_t> slice_w;    // slices
   std::vector<int32_t> col;        // slots
   std::vector<double> val;         // slots
+  // Stored position -> row of A (SELL-C-sigma); empty: position == row.
+  std::vector<int32_t> row_map;
 };
 Sell64 sell64_pack(const Csr &A);
+// SELL-C-sigma for non-square operators (transfers): rows are stably
+// sorted by decreasing length inside windows of `sigma` rows before the
+// SELL-64 packing, so a slice holds rows of similar length and little
+// padding; row_map gives each stored position's row. Entry order inside a
+// row is unchanged. Throws for square operators.
+Sell64 sell64_pack_sorted(const Csr &A, int64_t sigma);
 
 // BSR3-SELL-64: the device layout of 3x3-block operators. Block rows in
 // slices of 64; block slot j of block row b: bcol at slice_ptr[s] + 64 j +

@@ -196,6 +196,26 @@ Csr relabel(const Csr &A, const int32_t *rq, const int32_t *cmap) {
   return B;
 }
 
+Csr take_rows(const Csr &A, const int32_t *rows, int64_t nrows) {
+  Csr B;
+  B.rows = nrows;
+  B.cols = A.cols;
+  B.ptr.assign(nrows + 1, 0);
+  for (int64_t i = 0; i < nrows; ++i) {
+    if (rows[i] < 0 || rows[i] >= A.rows) throw std::runtime_error("take_rows: row out of range.");
+    B.ptr[i + 1] = B.ptr[i] + (A.ptr[rows[i] + 1] - A.ptr[rows[i]]);
+  }
+  B.col.resize(B.ptr[nrows]);
+  B.val.resize(B.ptr[nrows]);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nrows; ++i) {
+    const int32_t r = rows[i];
+    std::copy(A.col.begin() + A.ptr[r], A.col.begin() + A.ptr[r + 1], B.col.begin() + B.ptr[i]);
+    std::copy(A.val.begin() + A.ptr[r], A.val.begin() + A.ptr[r + 1], B.val.begin() + B.ptr[i]);
+  }
+  return B;
+}
+
 void check_columns(const Csr &A, const char *what) {
   bool bad = false;
 #pragma omp parallel for schedule(static) reduction(|| : bad)
@@ -539,6 +559,24 @@ Bsr3Sell64 bsr3_sell64_pack(const Bsr3Csr &B) {
         }
       }
     }
+  return out;
+}
+
+Sell64 sell64_pack_sorted(const Csr &A, int64_t sigma) {
+  if (A.rows == A.cols)
+    throw std::runtime_error("SELL-C-sigma packing is for non-square operators.");
+  if (sigma < 1) throw std::runtime_error("SELL-C-sigma: window must be positive.");
+  std::vector<int32_t> order(A.rows);
+  for (int64_t i = 0; i < A.rows; ++i) order[i] = static_cast<int32_t>(i);
+  for (int64_t w0 = 0; w0 < A.rows; w0 += sigma) {
+    const int64_t w1 = std::min(A.rows, w0 + sigma);
+    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int32_t a, int32_t b) {
+      return A.ptr[a + 1] - A.ptr[a] > A.ptr[b + 1] - A.ptr[b];
+    });
+  }
+  const Csr B = take_rows(A, order.data(), A.rows);
+  Sell64 out = sell64_pack(B);
+  out.row_map = std::move(order);
   return out;
 }
 

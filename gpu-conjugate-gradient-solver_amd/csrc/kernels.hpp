@@ -42,7 +42,16 @@ struct Sell {
   const int32_t *slice_w;    // width of each slice
   const int32_t *col;
   const T *val;
+  // Stored position -> row (rows sorted by length within windows, transfer
+  // operators only); null: position == row.
+  const int32_t *row_map = nullptr;
 };
+
+// Row held at stored position `pos` of a SELL operator.
+template <class T>
+__device__ __forceinline__ int64_t sell_row_of(const Sell<T> &A, int64_t pos) {
+  return A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
+}
 
 template <class T>
 struct Bsr3 {
@@ -362,9 +371,10 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict(Sell<T> R, const T *__restrict__ res,
                                                      T *__restrict__ rhs_c, T *__restrict__ corr_c,
                                                      const int32_t *stop) {
-  const int64_t row = xcd_block() * kBlock + threadIdx.x;
-  if (row >= R.rows) return;
-  const SliceHead h = sell_head(R, row);
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= R.rows) return;
+  const SliceHead h = sell_head(R, pos);
+  const int64_t row = sell_row_of(R, pos);
   if (stop_word(stop)) return;
   rhs_c[row] = sell_dot_h(R, h, res);
   if (corr_c) corr_c[row] = T(0);
@@ -374,9 +384,10 @@ __global__ __launch_bounds__(kBlock) void k_restrict(Sell<T> R, const T *__restr
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__restrict__ e,
                                                         T *__restrict__ x, const int32_t *stop) {
-  const int64_t row = xcd_block() * kBlock + threadIdx.x;
-  if (row >= P.rows) return;
-  const SliceHead h = sell_head(P, row);
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= P.rows) return;
+  const SliceHead h = sell_head(P, pos);
+  const int64_t row = sell_row_of(P, pos);
   const T xi = x[row];
   if (stop_word(stop)) return;
   const T c = sell_dot_h(P, h, e);
@@ -391,9 +402,10 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(Sell<T> P, const T *__re
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_prolong_alias(Sell<T> P, const T *__restrict__ e,
                                                           T *__restrict__ x, const int32_t *stop) {
-  const int64_t row = xcd_block() * kBlock + threadIdx.x;
-  if (row >= P.rows) return;
-  const SliceHead h = sell_head(P, row);
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= P.rows) return;
+  const SliceHead h = sell_head(P, pos);
+  const int64_t row = sell_row_of(P, pos);
   if (stop_word(stop)) return;
   const T c = sell_dot_h(P, h, e);
   x[row] = c + T(1) * c;

@@ -27,6 +27,9 @@ struct Rows {
   const int64_t *ptr;
   const int32_t *col;
   const T *val;
+  // Stored position -> row (transfer operators executed in another row
+  // order); null: position == row.
+  const int32_t *row_map = nullptr;
 };
 
 template <class T>
@@ -107,6 +110,37 @@ __device__ __forceinline__ T grp_row_dot(const Rows<T> &A, int64_t rb, int64_t r
         lp[lane + u * G] = c[u] == excl ? neg_zero<T>() : v[u] * xv;
       }
     }
+    wave_sync();
+    if (lane == 0) s = chain_add(lp, static_cast<int>(re - base < C ? re - base : C), s);
+    wave_sync();
+  }
+  return s;
+}
+
+// Transfer-operator rows: as grp_row_dot with K entries per lane and round
+// (a round covers G * K entries, chosen so most rows take one round) and
+// no excluded entry. Every lane loads from a clamped, in-bounds index, so
+// the loads carry no branch; entries past the row end are staged but not
+// summed.
+template <class T, int G, int K>
+__device__ __forceinline__ T grp_row_dot_k(const Rows<T> &A, int64_t rb, int64_t re, const T *x,
+                                           T *lp, int lane) {
+  constexpr int C = G * K;
+  T s = T(0);
+  for (int64_t base = rb; base < re; base += C) {
+    int32_t c[K];
+    T v[K], xv[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      int64_t k = base + lane + u * G;
+      k = k < re ? k : re - 1;
+      c[u] = A.col[k];
+      v[u] = A.val[k];
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < K; ++u) lp[lane + u * G] = v[u] * xv[u];
     wave_sync();
     if (lane == 0) s = chain_add(lp, static_cast<int>(re - base < C ? re - base : C), s);
     wave_sync();
@@ -207,20 +241,23 @@ __global__ __launch_bounds__(kBlock) void k_residual_w(Rows<T> A, const T *__res
 }
 
 // rhs_c = R res, corr_c = 0 (as k_restrict).
-template <class T, int G>
+template <class T, int G, int K>
 __global__ __launch_bounds__(kBlock) void k_restrict_w(Rows<T> R, const T *__restrict__ res,
                                                        T *__restrict__ rhs_c, T *__restrict__ corr_c,
                                                        const int32_t *stop) {
-  GSK_GROUP_SCALAR_LDS(G);
-  const int64_t row = grp_row<G>(0);
-  const bool active = row < R.rows;
-  int64_t rb = 0, re = 0;
+  __shared__ T lp[kBlock * K];
+  const int lane = threadIdx.x % G;
+  const int g0 = (threadIdx.x - lane) * K;
+  const int64_t pos = grp_row<G>(0);
+  const bool active = pos < R.rows;
+  int64_t rb = 0, re = 0, row = pos;
   if (active) {
-    rb = R.ptr[row];
-    re = R.ptr[row + 1];
+    rb = R.ptr[pos];
+    re = R.ptr[pos + 1];
+    if (R.row_map) row = R.row_map[pos];
   }
   if (stop_word(stop)) return;
-  const T s = grp_row_dot<T, G>(R, rb, re, res, -1, lp + g0, lane);
+  const T s = grp_row_dot_k<T, G, K>(R, rb, re, res, lp + g0, lane);
   if (active && lane == 0) {
     rhs_c[row] = s;
     if (corr_c) corr_c[row] = T(0);
@@ -229,22 +266,25 @@ __global__ __launch_bounds__(kBlock) void k_restrict_w(Rows<T> R, const T *__res
 
 // alias = 0: x += P e (as k_prolong_add); alias = 1: x = c + 1 * c with
 // c = P e (as k_prolong_alias).
-template <class T, int G>
+template <class T, int G, int K>
 __global__ __launch_bounds__(kBlock) void k_prolong_w(Rows<T> P, const T *__restrict__ e,
                                                       T *__restrict__ x, int alias,
                                                       const int32_t *stop) {
-  GSK_GROUP_SCALAR_LDS(G);
-  const int64_t row = grp_row<G>(0);
-  const bool active = row < P.rows;
-  int64_t rb = 0, re = 0;
+  __shared__ T lp[kBlock * K];
+  const int lane = threadIdx.x % G;
+  const int g0 = (threadIdx.x - lane) * K;
+  const int64_t pos = grp_row<G>(0);
+  const bool active = pos < P.rows;
+  int64_t rb = 0, re = 0, row = pos;
   T xi = T(0);
   if (active) {
-    rb = P.ptr[row];
-    re = P.ptr[row + 1];
+    rb = P.ptr[pos];
+    re = P.ptr[pos + 1];
+    if (P.row_map) row = P.row_map[pos];
     if (!alias) xi = x[row];
   }
   if (stop_word(stop)) return;
-  const T c = grp_row_dot<T, G>(P, rb, re, e, -1, lp + g0, lane);
+  const T c = grp_row_dot_k<T, G, K>(P, rb, re, e, lp + g0, lane);
   if (active && lane == 0) x[row] = alias ? c + T(1) * c : xi + T(1) * c;
 }
 

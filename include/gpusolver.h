@@ -168,6 +168,14 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     levels below unused (0)
  *   "group_max_transfer" largest lane group (16, 32, 64) for the rows of the
  *                     restriction / prolongation operators (32)
+ *   "transfer_sigma"  restriction / prolongation rows kept one per thread
+ *                     are sorted by length within windows of this many rows
+ *                     (SELL-C-sigma, less slice padding; same sums); 0: the
+ *                     natural order (1024)
+ *   "transfer_level_order" 1: restriction / prolongation rows run in the
+ *                     level's own (pre-colouring) row order, neighbouring
+ *                     rows together; outputs go through a row map, sums are
+ *                     unchanged; 0: the colour order (1)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed
@@ -210,8 +218,9 @@ int gs_solver_vcycle(gs_solver_h s, const double *r, double *z);
  * Backend.h:68-69), vectors in level-0 ordering. */
 int gs_solver_spmv(gs_solver_h s, const double *x, double *y);
 /* Average ms per launch of a hot kernel timed with HIP events on the
- * solver's stream: 0 CG SpMV, 1 V-cycle, 2 PCG iteration, 3 level-0 SGS /
- * block-GS sweep. *bytes: algorithmic bytes per launch. */
+ * solver's stream: 0 CG SpMV, 1 V-cycle, 2 PCG iteration, 3 / 4 level-0 /
+ * level-1 SGS or block-GS sweep, 5 level-0 restriction, 6 level-0
+ * prolongation. *bytes: algorithmic bytes per launch. */
 int gs_solver_time_kernel(gs_solver_h s, int32_t which, int32_t reps, double *ms, double *bytes);
 int gs_solver_describe(gs_solver_h s, char *buf, int64_t cap);
 int gs_solver_destroy(gs_solver_h s);

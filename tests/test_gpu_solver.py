@@ -229,6 +229,23 @@ def test_pcg_coarse_direct(cfg1):
     assert true <= 1.5e-8
 
 
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+def test_transfer_row_orders_bitexact(elast, prec):
+    """Transfer rows executed in another order (the level's own order, and
+    prolongation rows sorted by length within windows, SELL-C-sigma) give the
+    same V-cycle bit for bit as the colour order with no sorting."""
+    E, rhs, H = elast
+    r = gs.uniform_pm1(11, E.rows)
+    out = []
+    for lo, sigma in ((0, 0), (1, 0), (0, 1024), (1, 1024), (1, 37)):
+        s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2, transfer_sigma=sigma,
+                      transfer_level_order=lo)
+        s.setup(H)
+        out.append(s.vcycle(r))
+    for z in out[1:]:
+        np.testing.assert_array_equal(out[0], z)
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 333])
 def test_direct_level_tiled_product(n):
     """A one-level hierarchy with the direct coarse solve: the V-cycle is
