@@ -230,12 +230,14 @@ def main() -> None:
     if len(hdesc["levels"]) > 1:
         # Level-0 transfers of the headline solver, once each per V-cycle.
         l0 = hdesc["levels"][0]
+        unit = "block row" if l0["transfer_b3"] else "row"
+        store = "3x3 blocks" if l0["transfer_b3"] else "scalar"
         for kid, tid, what, grp in (("l0_restrict", 5, "restriction R0 (level 0 -> 1)", l0["r_group"]),
                                     ("l0_prolong", 6, "prolongation P0 (level 1 -> 0)", l0["p_group"])):
             t_ms, t_bytes = hs.time_kernel(tid, 20)
             cands.append({"id": kid,
-                          "kernel": f"level-0 {what} ("
-                                    f"{'lane groups of %d' % grp if grp else 'thread per row, SELL-C-sigma'}"
+                          "kernel": f"level-0 {what} ({store}, "
+                                    f"{'lane groups of %d' % grp if grp else 'thread per %s, SELL-C-sigma' % unit}"
                                     f", FP32, headline mode)",
                           "ms": t_ms, "bytes": t_bytes, "launches": 1, "per_iteration_ms": t_ms})
     dom = max(cands[1:] or cands, key=lambda c: c["per_iteration_ms"])

@@ -145,7 +145,8 @@ struct DBsr3 {
   int64_t nb = 0;
   int64_t nblocks = 0;  // stored blocks (without padding)
   int64_t slots = 0;
-  DevBuf slice_ptr, slice_w, bcol, val;
+  DevBuf slice_ptr, slice_w, bcol, val, row_map;
+  bool mapped = false;
 
   void build(const gsx::Bsr3Csr &B) {
     const gsx::Bsr3Sell64 P = gsx::bsr3_sell64_pack(B);
@@ -156,13 +157,36 @@ struct DBsr3 {
     slice_w.upload(P.slice_w);
     bcol.upload(P.bcol);
     upload_values<T>(val, P.val.data(), P.val.size());
+    mapped = false;
+    row_map.release();
+  }
+  // Transfer operators: block rows executed in `order` (null: natural
+  // order), then sorted by block count within windows of `sigma` (> 0).
+  void build(const gsx::Bsr3Csr &B, int64_t sigma, const int32_t *order) {
+    std::vector<int32_t> map;
+    if (order) map.assign(order, order + B.nb);
+    const gsx::Bsr3Csr B1 = order ? gsx::bsr3_take_rows(B, order, B.nb) : gsx::Bsr3Csr{};
+    const gsx::Bsr3Csr &Bo = order ? B1 : B;
+    if (sigma > 0) {
+      const std::vector<int32_t> srt = gsx::bsr3_sorted_order(Bo, sigma);
+      std::vector<int32_t> m2(B.nb);
+      for (int64_t p = 0; p < B.nb; ++p) m2[p] = order ? map[srt[p]] : srt[p];
+      build(gsx::bsr3_take_rows(Bo, srt.data(), B.nb));
+      map = std::move(m2);
+    } else {
+      build(Bo);
+    }
+    if (!map.empty()) {
+      row_map.upload(map);
+      mapped = true;
+    }
   }
   gsk::Bsr3<T> view() const {
     return gsk::Bsr3<T>{nb, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(), bcol.as<int32_t>(),
-                        val.as<T>()};
+                        val.as<T>(), mapped ? row_map.as<int32_t>() : nullptr};
   }
   size_t device_bytes() const {
-    return slice_ptr.bytes() + slice_w.bytes() + bcol.bytes() + val.bytes();
+    return slice_ptr.bytes() + slice_w.bytes() + bcol.bytes() + val.bytes() + row_map.bytes();
   }
 };
 
@@ -201,16 +225,27 @@ struct DRows {
 template <class T>
 struct DBRows {
   int64_t nb = 0, nblocks = 0;
-  DevBuf ptr, bcol, val;
-  void build(const gsx::Bsr3Csr &B) {
+  DevBuf ptr, bcol, val, row_map;
+  bool mapped = false;
+  // order: stored position p holds block row order[p] (null: natural).
+  void build(const gsx::Bsr3Csr &B, const int32_t *order = nullptr) {
+    if (order) {
+      build(gsx::bsr3_take_rows(B, order, B.nb));
+      row_map.upload(order, static_cast<size_t>(B.nb));
+      mapped = true;
+      return;
+    }
     nb = B.nb;
     nblocks = B.nblocks;
     ptr.upload(B.ptr);
     bcol.upload(B.bcol);
     upload_values<T>(val, B.val.data(), B.val.size());
+    mapped = false;
+    row_map.release();
   }
   gsk::BRows<T> view() const {
-    return gsk::BRows<T>{nb, ptr.as<int64_t>(), bcol.as<int32_t>(), val.as<T>()};
+    return gsk::BRows<T>{nb, ptr.as<int64_t>(), bcol.as<int32_t>(), val.as<T>(),
+                         mapped ? row_map.as<int32_t>() : nullptr};
   }
 };
 
@@ -234,19 +269,45 @@ inline int pick_block_group(double avg, double min_avg) {
   return 64;
 }
 
-// Scalar operator (transfers): SELL-64 or row-contiguous CSR.
+// Transfer operator: scalar (SELL-64 or row-contiguous CSR) or 3x3 blocks
+// (BSR3-SELL-64 or row-contiguous block CSR).
 template <class T>
 struct DMat {
   int G = 0;  // lanes per row; 0: SELL-64, one thread per row
   int K = 2;  // lane groups: entries per lane and load round (2, 4 or 8)
+  bool b3 = false;
   int64_t rows = 0, cols = 0, nnz = 0;
+  int64_t nblocks = 0;  // 3x3 blocks (block storage)
   DSell<T> s;
   DRows<T> w;
+  DBsr3<T> sb;
+  DBRows<T> bw;
+  // 3x3-block storage of a transfer whose row and column unknowns both come
+  // in node triples: key orders each block row's blocks (the CSR rows'
+  // column order), order[p] is the block row at position p (null: natural),
+  // group_min: average blocks per block row from which lane groups are used.
+  void build_blocks(const gsx::Csr &A, const int32_t *key, double group_min, int64_t sigma,
+                    const int32_t *order) {
+    gsx::Bsr3Csr B;
+    if (!gsx::bsr3_pack(A, B, key))
+      throw std::runtime_error("block transfer: dimensions are not multiples of 3.");
+    b3 = true;
+    rows = A.rows;
+    cols = A.cols;
+    nblocks = B.nblocks;
+    nnz = 9 * B.nblocks;
+    K = 2;
+    G = pick_block_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min);
+    if (G) bw.build(B, order);
+    else sb.build(B, sigma, order);
+  }
   // sigma: SELL-C-sigma window for the thread-per-row layout (0: none).
   // order (non-square operators only): rows are executed in this order,
   // order[p] being the row at position p (null: natural order).
   void build(const gsx::Csr &A, double group_min, int group_max = 64, int64_t sigma = 0,
              const int32_t *order = nullptr) {
+    b3 = false;
+    nblocks = 0;
     rows = A.rows;
     cols = A.cols;
     nnz = A.nnz();

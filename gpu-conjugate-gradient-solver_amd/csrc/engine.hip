@@ -51,6 +51,15 @@ std::vector<int32_t> inverse_of(const std::vector<int32_t> &q) {
   return inv;
 }
 
+// True when the ordering q moves unknowns in node triples: q[3j + l] =
+// q[3j] + l with q[3j] a multiple of 3 (block smoothers' orders, identity).
+bool keeps_triples(const std::vector<int32_t> &q) {
+  if (q.size() % 3 != 0) return false;
+  for (size_t j = 0; j < q.size(); j += 3)
+    if (q[j] % 3 != 0 || q[j + 1] != q[j] + 1 || q[j + 2] != q[j] + 2) return false;
+  return true;
+}
+
 std::vector<int32_t> iota_vec(int64_t n) {
   std::vector<int32_t> v(n);
   std::iota(v.begin(), v.end(), 0);
@@ -400,10 +409,31 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       const std::vector<int32_t> qc_inv = inverse_of(L_[k + 1].q);
       // Level order: position p holds the internal row of level index p.
       const bool lo = opt_.transfer_level_order;
-      L_[k].P.build(relabel(Pk, L_[k].q, qc_inv), opt_.group_min_scalar,
-                    opt_.group_max_transfer, opt_.transfer_sigma, lo ? qf_inv.data() : nullptr);
-      L_[k].R.build(relabel(Rk, L_[k + 1].q, qf_inv), opt_.group_min_scalar,
-                    opt_.group_max_transfer, opt_.transfer_sigma, lo ? qc_inv.data() : nullptr);
+      const gsx::Csr Pi = relabel(Pk, L_[k].q, qc_inv);
+      const gsx::Csr Ri = relabel(Rk, L_[k + 1].q, qf_inv);
+      if (opt_.block_transfers && keeps_triples(L_[k].q) && keeps_triples(L_[k + 1].q)) {
+        // 3x3 blocks: each block row's blocks in the CSR rows' column order
+        // (ascending level index), block rows in level order.
+        const int64_t nfb = L_[k].n / 3, ncb = L_[k + 1].n / 3;
+        std::vector<int32_t> key_c(ncb), key_f(nfb), ord_f(nfb), ord_c(ncb);
+        for (int64_t j = 0; j < ncb; ++j) {
+          key_c[j] = L_[k + 1].q[3 * j] / 3;
+          ord_c[j] = qc_inv[3 * j] / 3;
+        }
+        for (int64_t j = 0; j < nfb; ++j) {
+          key_f[j] = L_[k].q[3 * j] / 3;
+          ord_f[j] = qf_inv[3 * j] / 3;
+        }
+        L_[k].P.build_blocks(Pi, key_c.data(), opt_.group_min_block, opt_.transfer_sigma,
+                             lo ? ord_f.data() : nullptr);
+        L_[k].R.build_blocks(Ri, key_f.data(), opt_.group_min_block, opt_.transfer_sigma,
+                             lo ? ord_c.data() : nullptr);
+      } else {
+        L_[k].P.build(Pi, opt_.group_min_scalar, opt_.group_max_transfer, opt_.transfer_sigma,
+                      lo ? qf_inv.data() : nullptr);
+        L_[k].R.build(Ri, opt_.group_min_scalar, opt_.group_max_transfer, opt_.transfer_sigma,
+                      lo ? qc_inv.data() : nullptr);
+      }
       mark("transfers " + std::to_string(k));
     }
   }
@@ -633,6 +663,18 @@ template <class TC, class TM>
 void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop) {
   Level<TM> &lv = L_[k];
   Level<TM> &cl = L_[k + 1];
+  if (lv.R.b3) {
+    if (lv.R.G)
+      with_group(lv.R.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        gsk::k_restrict_bw<TM, G><<<gsk::grid_rows<G>(cl.n / 3), kBlock, 0, stream_>>>(
+            lv.R.bw.view(), res, cl.rhs, corr_c, stop);
+      });
+    else
+      gsk::k_restrict_b3<TM><<<grid_for(cl.n / 3), kBlock, 0, stream_>>>(lv.R.sb.view(), res,
+                                                                         cl.rhs, corr_c, stop);
+    return;
+  }
   if (lv.R.G)
     with_group(lv.R.G, [&](auto g) {
       with_k(lv.R.K, [&](auto kk) {
@@ -651,6 +693,18 @@ template <class TC, class TM>
 void Engine<TC, TM>::encode_prolong(int k, bool alias, const int32_t *stop) {
   Level<TM> &lv = L_[k];
   Level<TM> &cl = L_[k + 1];
+  if (lv.P.b3) {
+    if (lv.P.G)
+      with_group(lv.P.G, [&](auto g) {
+        constexpr int G = decltype(g)::value;
+        gsk::k_prolong_bw<TM, G><<<gsk::grid_rows<G>(lv.n / 3), kBlock, 0, stream_>>>(
+            lv.P.bw.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+      });
+    else
+      gsk::k_prolong_b3<TM><<<grid_for(lv.n / 3), kBlock, 0, stream_>>>(
+          lv.P.sb.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+    return;
+  }
   if (lv.P.G)
     with_group(lv.P.G, [&](auto g) {
       with_k(lv.P.K, [&](auto kk) {
@@ -895,8 +949,9 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
     // x read and written).
     const double s = sizeof(TM);
     const auto &T = which == 5 ? L_[0].R : L_[0].P;
-    *bytes = T.nnz * (s + 4) + (which == 5 ? L_[0].n * s + 2.0 * L_[1].n * s
-                                           : L_[1].n * s + 2.0 * L_[0].n * s);
+    const double entries = T.b3 ? T.nblocks * (9 * s + 4) : T.nnz * (s + 4);
+    *bytes = entries + (which == 5 ? L_[0].n * s + 2.0 * L_[1].n * s
+                                   : L_[1].n * s + 2.0 * L_[0].n * s);
   }
   if (which == 3 || which == 4) {
     const double s = sizeof(TM);
@@ -944,6 +999,7 @@ void Engine<TC, TM>::level_info(std::vector<LevelInfo> &out) const {
     li.group = lv.A.G;
     li.r_group = k + 1 < nlev_ ? lv.R.G : 0;
     li.p_group = k + 1 < nlev_ ? lv.P.G : 0;
+    li.transfer_b3 = k + 1 < nlev_ && lv.P.b3;
     li.ncolors = lv.ncolors;
     li.kind = lv.kind;
     li.direct = lv.direct;
@@ -967,7 +1023,8 @@ std::string Engine<TC, TM>::describe() const {
        << ", \"direct\": " << (li[k].direct ? "true" : "false") << ", \"omega\": " << li[k].omega
        << ", \"pre\": " << li[k].pre << ", \"post\": " << li[k].post
        << ", \"group\": " << li[k].group << ", \"r_group\": " << li[k].r_group
-       << ", \"p_group\": " << li[k].p_group << "}";
+       << ", \"p_group\": " << li[k].p_group
+       << ", \"transfer_b3\": " << (li[k].transfer_b3 ? "true" : "false") << "}";
   }
   os << "], \"setup_ms\": {";
   for (size_t i = 0; i < setup_ms_.size(); ++i)

@@ -52,6 +52,10 @@ struct Options {
   // spatial neighbours and share their gathers in cache; outputs are
   // written through the row map, sums unchanged.
   bool transfer_level_order = true;
+  // Transfers between two levels whose orderings keep node triples (block
+  // smoothers) are stored as 3x3 blocks, one column index per 9 values;
+  // the sums are those of the scalar rows.
+  bool block_transfers = true;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };
@@ -77,6 +81,7 @@ struct LevelInfo {
   int group = 0;    // lanes per row of the level operator (0: one thread)
   int r_group = 0;  // same for the restriction to the next level
   int p_group = 0;  // and the prolongation from it
+  bool transfer_b3 = false;  // the transfers to the next level are 3x3 blocks
 };
 
 class EngineBase {

@@ -113,17 +113,24 @@ int32_t max_row_length(const Csr &A);
 // its three scalar rows; blocks missing a scalar entry hold exact zeros.
 struct Bsr3Csr {
   int64_t nb = 0;            // block rows
+  int64_t ncb = 0;           // block columns
   int64_t nblocks = 0;       // stored blocks
   int32_t width = 0;         // max blocks per block row
   std::vector<int64_t> ptr;  // nb + 1
   std::vector<int32_t> bcol;
   RawVec<double> val;        // 9 * nblocks
 };
-// Returns false (and leaves out untouched) when A is not square with a
-// multiple of 3 rows. Blocks of a block row are stored in increasing
-// key[block column] (block column itself when key is null); duplicate
-// entries are summed in their CSR order.
+// Returns false (and leaves out untouched) unless A's rows and columns are
+// multiples of 3 (rectangular operators, the transfers, are allowed).
+// Blocks of a block row are stored in increasing key[block column] (block
+// column itself when key is null); duplicate entries are summed in their
+// CSR order.
 bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key = nullptr);
+// Block row p of the result is block row rows[p] of B (blocks unchanged).
+Bsr3Csr bsr3_take_rows(const Bsr3Csr &B, const int32_t *rows, int64_t nrows);
+// Stable order of block rows by decreasing block count within windows of
+// `sigma` rows (SELL-C-sigma): position p holds block row result[p].
+std::vector<int32_t> bsr3_sorted_order(const Bsr3Csr &B, int64_t sigma);
 // dst[i] = (float)src[i], in parallel.
 void to_float(const double *src, size_t n, float *dst);
 // Parallel permuted copies with a cast: gather dst[i] = src[idx[i]],

@@ -361,8 +361,8 @@ Ell ell_pack(const Csr &A, int32_t width) {
 }
 
 bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key) {
-  if (A.rows != A.cols || A.rows % 3 != 0) return false;
-  const int64_t nb = A.rows / 3;
+  if (A.rows % 3 != 0 || A.cols % 3 != 0) return false;
+  const int64_t nb = A.rows / 3, ncb = A.cols / 3;
   auto rank = [key](int32_t cb) { return key ? key[cb] : cb; };
   // Block columns of block row b: union over its three scalar rows, ordered
   // by rank, so that a row whose entries come in rank order is summed in
@@ -398,7 +398,7 @@ bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key) {
     std::vector<int32_t> bc;
     // slot[c]: slot of block column c in the current block row (only the
     // row's own columns are written before they are read).
-    std::vector<int32_t> slot(nb);
+    std::vector<int32_t> slot(ncb);
 #pragma omp for schedule(dynamic, 1024)
     for (int64_t b = 0; b < nb; ++b) {
       gather(b, bc);
@@ -417,12 +417,51 @@ bool bsr3_pack(const Csr &A, Bsr3Csr &out, const int32_t *key) {
     }
   }
   out.nb = nb;
+  out.ncb = ncb;
   out.nblocks = nblocks;
   out.width = width;
   out.ptr = std::move(ptr);
   out.bcol = std::move(bcol);
   out.val = std::move(val);
   return true;
+}
+
+Bsr3Csr bsr3_take_rows(const Bsr3Csr &B, const int32_t *rows, int64_t nrows) {
+  Bsr3Csr out;
+  out.nb = nrows;
+  out.ncb = B.ncb;
+  out.ptr.assign(nrows + 1, 0);
+  int64_t width = 0;
+  for (int64_t i = 0; i < nrows; ++i) {
+    if (rows[i] < 0 || rows[i] >= B.nb) throw std::runtime_error("bsr3_take_rows: row out of range.");
+    const int64_t len = B.ptr[rows[i] + 1] - B.ptr[rows[i]];
+    out.ptr[i + 1] = out.ptr[i] + len;
+    width = std::max(width, len);
+  }
+  out.width = static_cast<int32_t>(width);
+  out.nblocks = out.ptr[nrows];
+  out.bcol.resize(out.nblocks);
+  out.val.resize(9 * out.nblocks);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nrows; ++i) {
+    const int64_t s0 = B.ptr[rows[i]], s1 = B.ptr[rows[i] + 1], d = out.ptr[i];
+    std::copy(B.bcol.begin() + s0, B.bcol.begin() + s1, out.bcol.begin() + d);
+    std::copy(B.val.begin() + 9 * s0, B.val.begin() + 9 * s1, out.val.begin() + 9 * d);
+  }
+  return out;
+}
+
+std::vector<int32_t> bsr3_sorted_order(const Bsr3Csr &B, int64_t sigma) {
+  if (sigma < 1) throw std::runtime_error("SELL-C-sigma: window must be positive.");
+  std::vector<int32_t> order(B.nb);
+  for (int64_t i = 0; i < B.nb; ++i) order[i] = static_cast<int32_t>(i);
+  for (int64_t w0 = 0; w0 < B.nb; w0 += sigma) {
+    const int64_t w1 = std::min(B.nb, w0 + sigma);
+    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int32_t a, int32_t b) {
+      return B.ptr[a + 1] - B.ptr[a] > B.ptr[b + 1] - B.ptr[b];
+    });
+  }
+  return order;
 }
 
 void to_float(const double *src, size_t n, float *dst) {
@@ -554,7 +593,13 @@ Bsr3Sell64 bsr3_sell64_pack(const Bsr3Csr &B) {
           out.bcol[slot + l] = B.bcol[k];
           for (int e = 0; e < 9; ++e) v[e * S] = B.val[9 * k + e];
         } else {
-          out.bcol[slot + l] = static_cast<int32_t>(b < B.nb ? b : 0);
+          // Padding: a zero block on a valid column -- the row itself for
+          // square operators (the Gauss-Seidel kernels skip that column),
+          // else the row's last block column (0 for an empty row).
+          int32_t pc = 0;
+          if (B.nb == B.ncb) pc = static_cast<int32_t>(b < B.nb ? b : 0);
+          else if (b < B.nb && B.ptr[b + 1] > B.ptr[b]) pc = B.bcol[B.ptr[b + 1] - 1];
+          out.bcol[slot + l] = pc;
           for (int e = 0; e < 9; ++e) v[e * S] = 0.0;
         }
       }

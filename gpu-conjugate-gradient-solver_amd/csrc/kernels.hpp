@@ -60,6 +60,8 @@ struct Bsr3 {
   const int32_t *slice_w;    // width (blocks) of each slice
   const int32_t *bcol;       // block column per block slot
   const T *val;              // 9 planes of 64 per block slot
+  // Stored position -> block row (transfer operators); null: identity.
+  const int32_t *row_map = nullptr;
 };
 
 // Solver control block, one per solve, in device memory.
@@ -409,6 +411,50 @@ __global__ __launch_bounds__(kBlock) void k_prolong_alias(Sell<T> P, const T *__
   if (stop_word(stop)) return;
   const T c = sell_dot_h(P, h, e);
   x[row] = c + T(1) * c;
+}
+
+// 3x3-block transfer operators (both levels keep their unknowns in node
+// triples): the same row sums as the scalar CSR rows, block by block in
+// the rows' column order, three entries per block.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_restrict_b3(Bsr3<T> R, const T *__restrict__ res,
+                                                        T *__restrict__ rhs_c,
+                                                        T *__restrict__ corr_c,
+                                                        const int32_t *stop) {
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= R.nb) return;
+  const SliceHead h = bsr3_head(R, pos);
+  const int64_t b = R.row_map ? static_cast<int64_t>(R.row_map[pos]) : pos;
+  if (stop_word(stop)) return;
+  T v[3];
+  bsr3_dot_h(R, h, pos, res, v, false);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) {
+    rhs_c[3 * b + l] = v[l];
+    if (corr_c) corr_c[3 * b + l] = T(0);
+  }
+}
+
+// alias = 0: x += P e; alias = 1: x = c + 1 * c with c = P e.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_prolong_b3(Bsr3<T> P, const T *__restrict__ e,
+                                                       T *__restrict__ x, int alias,
+                                                       const int32_t *stop) {
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= P.nb) return;
+  const SliceHead h = bsr3_head(P, pos);
+  const int64_t b = P.row_map ? static_cast<int64_t>(P.row_map[pos]) : pos;
+  T xi[3] = {T(0), T(0), T(0)};
+  if (!alias) {
+    xi[0] = x[3 * b];
+    xi[1] = x[3 * b + 1];
+    xi[2] = x[3 * b + 2];
+  }
+  if (stop_word(stop)) return;
+  T c[3];
+  bsr3_dot_h(P, h, pos, e, c, false);
+#pragma unroll
+  for (int l = 0; l < 3; ++l) x[3 * b + l] = alias ? c[l] + T(1) * c[l] : xi[l] + T(1) * c[l];
 }
 
 // ---------------------------------------------------------------------------

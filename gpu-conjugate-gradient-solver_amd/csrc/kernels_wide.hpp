@@ -38,6 +38,8 @@ struct BRows {
   const int64_t *ptr;   // block offsets, nb + 1
   const int32_t *bcol;  // block column per block
   const T *val;         // 9 per block, row-major
+  // Stored position -> block row (transfer operators); null: identity.
+  const int32_t *row_map = nullptr;
 };
 
 // Orders the LDS traffic of one wavefront: stores issued by lanes of the
@@ -367,6 +369,52 @@ __global__ __launch_bounds__(kBlock) void k_residual_bw(BRows<T> A, const T *__r
   if (stop_word(stop)) return;
   const T s = grp_brow_dot<T, G>(A, rb, re, x, -1, lp + 9 * g0, lane);
   if (active && lane < 3) res[3 * b + lane] = bl - s;
+}
+
+// 3x3-block transfer rows, one G-lane group per block row (as
+// k_restrict_b3 / k_prolong_b3).
+template <class T, int G>
+__global__ __launch_bounds__(kBlock) void k_restrict_bw(BRows<T> R, const T *__restrict__ res,
+                                                        T *__restrict__ rhs_c,
+                                                        T *__restrict__ corr_c,
+                                                        const int32_t *stop) {
+  GSK_GROUP_BLOCK_LDS(G);
+  (void)glr;
+  const int64_t pos = grp_row<G>(0);
+  const bool active = pos < R.nb;
+  int64_t rb = 0, re = 0, b = pos;
+  if (active) {
+    rb = R.ptr[pos];
+    re = R.ptr[pos + 1];
+    if (R.row_map) b = R.row_map[pos];
+  }
+  if (stop_word(stop)) return;
+  const T s = grp_brow_dot<T, G>(R, rb, re, res, -1, lp + 9 * g0, lane);
+  if (active && lane < 3) {
+    rhs_c[3 * b + lane] = s;
+    if (corr_c) corr_c[3 * b + lane] = T(0);
+  }
+}
+
+template <class T, int G>
+__global__ __launch_bounds__(kBlock) void k_prolong_bw(BRows<T> P, const T *__restrict__ e,
+                                                       T *__restrict__ x, int alias,
+                                                       const int32_t *stop) {
+  GSK_GROUP_BLOCK_LDS(G);
+  (void)glr;
+  const int64_t pos = grp_row<G>(0);
+  const bool active = pos < P.nb;
+  int64_t rb = 0, re = 0, b = pos;
+  T xl = T(0);
+  if (active) {
+    rb = P.ptr[pos];
+    re = P.ptr[pos + 1];
+    if (P.row_map) b = P.row_map[pos];
+    if (!alias && lane < 3) xl = x[3 * b + lane];
+  }
+  if (stop_word(stop)) return;
+  const T c = grp_brow_dot<T, G>(P, rb, re, e, -1, lp + 9 * g0, lane);
+  if (active && lane < 3) x[3 * b + lane] = alias ? c + T(1) * c : xl + T(1) * c;
 }
 
 // nv_l = sum_c Dinv_b[l][c] r_c for lane l < 3 of the group, from the three

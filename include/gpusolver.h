@@ -176,6 +176,9 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     level's own (pre-colouring) row order, neighbouring
  *                     rows together; outputs go through a row map, sums are
  *                     unchanged; 0: the colour order (1)
+ *   "block_transfers" 1: restriction / prolongation between two levels
+ *                     whose orderings keep node triples (block smoothers)
+ *                     are stored as 3x3 blocks, same sums (1)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed

@@ -232,15 +232,18 @@ def test_pcg_coarse_direct(cfg1):
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
 def test_transfer_row_orders_bitexact(elast, prec):
     """Transfer rows executed in another order (the level's own order, and
-    prolongation rows sorted by length within windows, SELL-C-sigma) give the
-    same V-cycle bit for bit as the colour order with no sorting."""
+    prolongation rows sorted by length within windows, SELL-C-sigma) and
+    stored as 3x3 blocks give the same V-cycle bit for bit as scalar rows in
+    the colour order with no sorting."""
     E, rhs, H = elast
     r = gs.uniform_pm1(11, E.rows)
     out = []
-    for lo, sigma in ((0, 0), (1, 0), (0, 1024), (1, 1024), (1, 37)):
+    for blk, lo, sigma in ((0, 0, 0), (0, 1, 0), (0, 0, 1024), (0, 1, 1024), (0, 1, 37),
+                           (1, 0, 0), (1, 1, 0), (1, 1, 1024), (1, 0, 37)):
         s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2, transfer_sigma=sigma,
-                      transfer_level_order=lo)
+                      transfer_level_order=lo, block_transfers=blk)
         s.setup(H)
+        assert all(l["transfer_b3"] == bool(blk) for l in s.describe()["levels"][:-1])
         out.append(s.vcycle(r))
     for z in out[1:]:
         np.testing.assert_array_equal(out[0], z)
