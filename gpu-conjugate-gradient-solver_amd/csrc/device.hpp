@@ -344,8 +344,11 @@ struct DOp {
   // block row's blocks are stored and summed. group_min: average row length
   // (entries; blocks for block rows) from which rows are processed by lane
   // groups (< 0: never).
+  // order (3x3-block SELL storage only): block row order[p] is stored at
+  // position p (the CG operator in the level's spatial order).
   void build(const gsx::Csr &A, bool prefer_b3, const int32_t *block_key = nullptr,
-             double group_min_scalar = -1.0, double group_min_block = -1.0) {
+             double group_min_scalar = -1.0, double group_min_block = -1.0,
+             const int32_t *order = nullptr) {
     n = A.rows;
     b3 = false;
     G = 0;
@@ -360,6 +363,7 @@ struct DOp {
         nnz = 9 * B.nblocks;
         G = pick_block_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min_block);
         if (G) bw.build(B);
+        else if (order) b.build(B, 0, order);
         else b.build(B);
         return;
       }

@@ -461,7 +461,16 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       key.resize(n_ / 3);
       for (int64_t j = 0; j < n_ / 3; ++j) key[j] = int2orig_[3 * j] / 3;
     }
-    Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data());
+    // Block rows executed in level-0 (spatial) order when level 0 moves
+    // node triples together: neighbouring rows run together and share
+    // their x gathers in cache.
+    std::vector<int32_t> ord;
+    if (blocky && n_ % 3 == 0 && opt_.cg_level_order && keeps_triples(L_[0].q)) {
+      ord.resize(n_ / 3);
+      for (int64_t p = 0; p < n_ / 3; ++p) ord[p] = q0inv[3 * p] / 3;
+    }
+    Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data(), -1.0, -1.0,
+               ord.empty() ? nullptr : ord.data());
     mark("CG operator");
   }
   // CG vectors and control block.

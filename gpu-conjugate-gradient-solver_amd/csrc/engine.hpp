@@ -56,6 +56,8 @@ struct Options {
   // smoothers) are stored as 3x3 blocks, one column index per 9 values;
   // the sums are those of the scalar rows.
   bool block_transfers = true;
+  // The 3x3-block CG operator's rows run in level-0 order (row map).
+  bool cg_level_order = true;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };

@@ -329,10 +329,11 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_spmv_b3(Bsr3<T> A, const T *__restrict__ x,
                                                     T *__restrict__ y, const int32_t *stop) {
   if (stopped(stop)) return;
-  const int64_t b = xcd_block() * kBlock + threadIdx.x;
-  if (b >= A.nb) return;
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= A.nb) return;
   T v[3];
-  bsr3_row_dot(A, b, x, v, false);
+  bsr3_row_dot(A, pos, x, v, false);
+  const int64_t b = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
   y[3 * b] = v[0];
   y[3 * b + 1] = v[1];
   y[3 * b + 2] = v[2];
@@ -814,10 +815,11 @@ template <class TC, class TM>
 __global__ __launch_bounds__(kBlock) void k_cg_init_r_b3(Bsr3<TC> A, const TC *__restrict__ b,
                                                          const TC *__restrict__ x,
                                                          TC *__restrict__ r, TM *__restrict__ rm) {
-  const int64_t bb = xcd_block() * kBlock + threadIdx.x;
-  if (bb >= A.nb) return;
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  if (pos >= A.nb) return;
   TC v[3];
-  bsr3_row_dot(A, bb, x, v, false);
+  bsr3_row_dot(A, pos, x, v, false);
+  const int64_t bb = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     const TC t = b[3 * bb + l] - v[l];
@@ -912,11 +914,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3(Bsr3<TC> A, const TC 
                                                            TC *__restrict__ q, Ctl<TC> *ctl,
                                                            TC *partials) {
   if (ctl->stop) return;
-  const int64_t b = xcd_block() * kBlock + threadIdx.x;
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   TC v = TC(0);
-  if (b < A.nb) {
+  if (pos < A.nb) {
     TC y[3];
-    bsr3_row_dot(A, b, p, y, false);
+    bsr3_row_dot(A, pos, p, y, false);
+    const int64_t b = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
     q[3 * b] = y[0];
     q[3 * b + 1] = y[1];
     q[3 * b + 2] = y[2];
@@ -1042,11 +1045,12 @@ template <class TC>
 __global__ __launch_bounds__(kBlock) void k_true_residual_b3(Bsr3<TC> A, const TC *__restrict__ b,
                                                              const TC *__restrict__ x,
                                                              double *partials) {
-  const int64_t blk = xcd_block() * kBlock + threadIdx.x;
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   double rr = 0.0, bb = 0.0;
-  if (blk < A.nb) {
-    const int64_t s = blk >> 6;
-    const int64_t lane = blk & 63;
+  if (pos < A.nb) {
+    const int64_t blk = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
+    const int64_t s = pos >> 6;
+    const int64_t lane = pos & 63;
     const int w = A.slice_w[s];
     double y[3] = {0.0, 0.0, 0.0};
     for (int j = 0; j < w; ++j) {

@@ -179,6 +179,9 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *   "block_transfers" 1: restriction / prolongation between two levels
  *                     whose orderings keep node triples (block smoothers)
  *                     are stored as 3x3 blocks, same sums (1)
+ *   "cg_level_order"  1: the 3x3-block CG operator's rows run in level-0
+ *                     order through a row map (each row's sum unchanged;
+ *                     the reductions group rows differently) (1)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed

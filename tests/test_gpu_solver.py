@@ -62,6 +62,23 @@ def test_spmv_bitexact(cfg1, prec):
     np.testing.assert_array_equal(y, yo.astype(np.float64))
 
 
+@pytest.mark.parametrize("lo", [0, 1])
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
+def test_spmv_block_bitexact(elast, prec, lo):
+    """The 3x3-block CG operator (rows in colour order, or in level-0 order
+    through a row map): every row sum bit for bit the oracle's CSR row sum."""
+    E, rhs, H = elast
+    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, cg_block3=1, cg_level_order=lo)
+    s.setup(H)
+    x = gs.uniform_pm1(8, E.rows)
+    y = s.spmv(x)
+    if prec == gs.PREC_SINGLE:
+        yo = orc.spmv(E.ptr, E.col, E.val.astype(np.float32), x.astype(np.float32))
+    else:
+        yo = orc.spmv(E.ptr, E.col, E.val, x)
+    np.testing.assert_array_equal(y, yo.astype(np.float64))
+
+
 # Operator storage: -1 = one thread per row everywhere (SELL-64), 0 = lane
 # groups for every operator, None = the default selection by row length.
 GROUPS = [-1.0, 0.0, None]
