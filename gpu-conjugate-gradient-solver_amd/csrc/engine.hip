@@ -169,8 +169,11 @@ class Engine final : public EngineBase {
 
  private:
   void build_level(int k, const gsx::Csr &Ak_level, bool coarsest);
-  void encode_vcycle(int k, const int32_t *stop, bool zero_x);
-  void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop);
+  // zero_x: the level's x starts at zero; x_zeroed: and that zero is
+  // already in memory (written by the restriction).
+  void encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed = false);
+  void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
+                     bool x_zeroed = false);
   void encode_iteration(const int32_t *stop);
   void encode_residual(Level<TM> &lv, const int32_t *stop);
   void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop);
@@ -552,7 +555,8 @@ static std::vector<std::pair<int, int>> sweep_sequence(int ncolors, int iters) {
 }
 
 template <class TC, class TM>
-void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop) {
+void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
+                                   bool x_zeroed) {
   const int64_t n = lv.n;
   bool x_ready = !zero_x;
   if ((lv.kind == kSGS || lv.kind == kBlockGS) && iters > 0) {
@@ -599,7 +603,7 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
     }
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {
     if (!x_ready && iters > 0) {
-      launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
+      if (!x_zeroed) launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
       x_ready = true;
     }
     TM *dinv = lv.dinv.template as<TM>(), *tmp = lv.tmp.template as<TM>();
@@ -629,11 +633,11 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
       gsk::k_copy<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.tmp.template as<TM>(), lv.x, n, stop);
     }
   }
-  if (!x_ready) launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
+  if (!x_ready && !x_zeroed) launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
 }
 
 template <class TC, class TM>
-void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
+void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed) {
   Level<TM> &lv = L_[k];
   if (k == nlev_ - 1) {
     if (lv.direct) {
@@ -644,11 +648,11 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
       gsk::k_dense_sym_reduce<TM><<<static_cast<uint32_t>(lv.sym_nt), 1024, 0, stream_>>>(
           prow, pcol, lv.n, lv.sym_nt, lv.sym_kmax, lv.x, stop);
     } else {
-      encode_smooth(lv, opt_.coarse_steps, zero_x, stop);
+      encode_smooth(lv, opt_.coarse_steps, zero_x, stop, x_zeroed);
     }
     return;
   }
-  encode_smooth(lv, lv.pre, zero_x, stop);
+  encode_smooth(lv, lv.pre, zero_x, stop, x_zeroed);
   // With no pre-smoothing from x = 0 the residual is rhs bit for bit (the
   // row sums of A 0 are +0, and rhs - (+0) == rhs), so restrict rhs itself.
   const bool res_is_rhs = zero_x && lv.pre == 0;
@@ -662,7 +666,7 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x) {
       (cl.kind == kSGS || cl.kind == kBlockGS);
   TM *corr_c = coarse_self_zero ? nullptr : cl.x;
   encode_restrict(k, res, corr_c, stop);
-  encode_vcycle(k + 1, stop, true);
+  encode_vcycle(k + 1, stop, true, corr_c != nullptr);
   encode_prolong(k, opt_.reference_vcycle && k > 0, stop);
   encode_smooth(lv, lv.post, false, stop);
 }
