@@ -109,6 +109,11 @@ struct Level {
   TM omega = TM(1);
   int pre = 1, post = 1;
   DOp<TM> A;
+  // Gauss-Seidel levels: the entries of earlier colours only (column below
+  // the row's colour range), for the forward colours of a sweep that
+  // starts from x = 0, whose later-colour neighbours are still exactly 0.
+  DOp<TM> Alow;
+  bool has_low = false;
   std::vector<int64_t> coff;  // colour offsets (rows, or blocks for block GS)
   std::vector<int64_t> cunits;  // stored entries (blocks for block GS) per colour
   int ncolors = 0;
@@ -303,6 +308,41 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   }
   lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
              opt_.group_min_block);
+  lv.has_low = false;
+  if (!lv.direct && (lv.kind == kSGS || lv.kind == kBlockGS) && opt_.zero_start_lower &&
+      lv.ncolors > 1) {
+    // Colours are contiguous internal ranges: an entry belongs to an earlier
+    // colour exactly when its column (block column) lies below the row's
+    // colour range. Row order and entry order are kept.
+    const int unit = lv.kind == kBlockGS ? 3 : 1;
+    gsx::Csr Low;
+    Low.rows = Ai.rows;
+    Low.cols = Ai.cols;
+    Low.ptr.assign(Ai.rows + 1, 0);
+    std::vector<int64_t> lo_of(Ai.rows);
+    for (int c = 0; c < lv.ncolors; ++c)
+      for (int64_t u = lv.coff[c]; u < lv.coff[c + 1]; ++u)
+        for (int l = 0; l < unit; ++l) lo_of[unit * u + l] = unit * lv.coff[c];
+    for (int64_t r = 0; r < Ai.rows; ++r) {
+      int32_t cnt = 0;
+      for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e) cnt += Ai.col[e] < lo_of[r];
+      Low.ptr[r + 1] = Low.ptr[r] + cnt;
+    }
+    Low.col.resize(Low.ptr[Ai.rows]);
+    Low.val.resize(Low.ptr[Ai.rows]);
+    for (int64_t r = 0; r < Ai.rows; ++r) {
+      int64_t d = Low.ptr[r];
+      for (int32_t e = Ai.ptr[r]; e < Ai.ptr[r + 1]; ++e)
+        if (Ai.col[e] < lo_of[r]) {
+          Low.col[d] = Ai.col[e];
+          Low.val[d] = Ai.val[e];
+          ++d;
+        }
+    }
+    lv.Alow.build(Low, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
+                  opt_.group_min_block);
+    lv.has_low = lv.Alow.b3 == lv.A.b3;
+  }
   mark("level " + std::to_string(k) + " pack + upload");
   // Stored entries (3x3 blocks for block GS) per colour, for the bytes of a
   // colour launch.
@@ -564,10 +604,18 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
     if (block && !lv.A.b3)
       throw std::runtime_error("block Gauss-Seidel level is not 3x3-block representable.");
     const int64_t units = block ? n / 3 : n;
+    // From x = 0, the first sweep's forward colours 1 .. C-1 (steps 1 .. C-1
+    // of the sequence) see exact zeros at every later-colour neighbour, so
+    // they read the earlier-colour entries only; the products they leave out
+    // are +-0 and a row sum never is -0, so the sums are unchanged.
+    const bool low_start = !x_ready && lv.has_low;
+    int step_no = -1;
     for (const auto &step : sweep_sequence(lv.ncolors, iters)) {
+      ++step_no;
       const int c = step.first;
       int reps = step.second;
       const int64_t b = lv.coff[c], e = lv.coff[c + 1];
+      const DOp<TM> &M = (low_start && step_no >= 1 && step_no <= lv.ncolors - 1) ? lv.Alow : lv.A;
       if (!x_ready && c == 0) {
         // First update from x = 0: closed form for colour 0, zero elsewhere.
         if (block)
@@ -581,24 +629,24 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
         reps = 1;  // the second update of the pair reads the fresh values
       }
       if (e <= b) continue;
-      if (lv.A.G) {
-        with_group(lv.A.G, [&](auto g) {
+      if (M.G) {
+        with_group(M.G, [&](auto g) {
           constexpr int G = decltype(g)::value;
           if (block)
             gsk::k_block_gs_color_bw<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
-                lv.A.bw.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
+                M.bw.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
                 stop);
           else
             gsk::k_gs_color_w<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
-                lv.A.w.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
+                M.w.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
                 stop);
         });
       } else if (block) {
         gsk::k_block_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-            lv.A.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+            M.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
       } else {
         gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-            lv.A.s.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+            M.s.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
       }
     }
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {

@@ -266,6 +266,25 @@ def test_transfer_row_orders_bitexact(elast, prec):
         np.testing.assert_array_equal(out[0], z)
 
 
+@pytest.mark.parametrize("grp", GROUPS)
+def test_zero_start_lower_bitexact(cfg1, elast, grp):
+    """The forward colours of a sweep from x = 0 read only earlier-colour
+    entries: the same V-cycle bit for bit as reading whole rows, for scalar
+    SGS and 3x3 block GS, one and two sweeps."""
+    A, H = cfg1
+    E, rhs, HE = elast
+    for mat, hier, kind, om in ((A, H, gs.SGS, 1.1), (E, HE, gs.BLOCK_GS, 0.7)):
+        r = gs.uniform_pm1(21, mat.rows)
+        for sweeps in (1, 2):
+            out = []
+            for low in (0, 1):
+                s = gs.Solver(gs.PREC_DOUBLE, smoother=kind, omega=om, pre=sweeps, post=sweeps,
+                              coarse_steps=2, zero_start_lower=low, **group_opts(grp))
+                s.setup(hier)
+                out.append(s.vcycle(r))
+            assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+
+
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 333])
 def test_direct_level_tiled_product(n):
     """A one-level hierarchy with the direct coarse solve: the V-cycle is
