@@ -413,6 +413,14 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       throw std::runtime_error("MultigridSmoother: permutation size does not match level size.");
     perms[k].check();
   }
+  // Transfer shapes before any permutation is applied to them.
+  for (int k = 0; k + 1 < hier_levels; ++k) {
+    H.P[k].check();
+    H.R[k].check();
+    if (H.P[k].rows != H.A[k].rows || H.P[k].cols != H.A[k + 1].rows ||
+        H.R[k].rows != H.A[k + 1].rows || H.R[k].cols != H.A[k].rows)
+      throw std::runtime_error("MultigridSmoother: transfer operator shape mismatch.");
+  }
   if (opt_.precond == kPrecIdentity && !std::is_same<TC, TM>::value)
     throw std::runtime_error("identity preconditioner requires one precision for CG and V-cycle.");
   // Direct solve at the first level small enough for the dense inverse; the
