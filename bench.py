@@ -166,7 +166,7 @@ def main() -> None:
     # (the inverse is built on the device); deeper levels are not used.
     mixed_kw["coarse_direct"] = 1
     mixed_kw["direct_truncate"] = 1
-    # FP64 throughout with the FP64 coarsest Cholesky: separates the gain of
+    # FP64 throughout with the FP64 direct coarsest solve: separates the gain of
     # the direct coarsest solve from that of the FP32 V-cycle.
     fd = gs.Solver(gs.PREC_DOUBLE, graph_iters=a.graph_iters, **mixed_kw)
     fd.setup(H)

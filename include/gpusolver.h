@@ -154,7 +154,9 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *   "coarse_steps"    coarsest-level sweeps (default 2 = pre + post,
  *                     src/Preconditioners/MultigridGaussSeidelPreconditioner.cc:21)
  *   "vcycles"         V-cycles per application (default 1)
- *   "coarse_direct"   1: FP64 Cholesky solve on the coarsest level
+ *   "coarse_direct"   1: FP64 direct solve on the coarsest level (explicit
+ *                     inverse built on the device, applied as a product
+ *                     over its lower 64 x 64 tiles)
  *   "graph_iters"     PCG iterations per captured hipGraph (0: no graph)
  *   "cg_block3"       1: store the CG operator as 3x3 blocks
  *   "reference_vcycle" 1: reproduce the reference's coarse-level correction
