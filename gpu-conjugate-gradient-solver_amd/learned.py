@@ -125,8 +125,11 @@ def make_solver(H, spec, omega, sweeps):
 ACCEPT = 0.98
 
 
-def sweep_labels(H, b, spec, maxit=400):
-    """Coordinate search over (w, sweeps) per smoothed level."""
+def sweep_labels(H, b, spec, maxit=400, passes=3):
+    """Coordinate search over (w, sweeps) per smoothed level, repeated until
+    a pass changes nothing (at most `passes`): a level's best w depends on
+    the sweeps chosen below it (with level 1 unsmoothed, level 0 wants a
+    larger w than with level 1 smoothed)."""
     L = H.levels
     omega = [spec["plain_omega"]] * L
     sweeps = [1] * L
@@ -145,14 +148,21 @@ def sweep_labels(H, b, spec, maxit=400):
 
     best, it0 = cost(omega, sweeps)
     plain_cost = best
-    for k in range(used - 1):  # the last level in use is solved directly
-        for om in spec["omegas"]:
-            for sw in (spec["sweeps"] if k == 0 else spec["coarse_sweeps"]):
-                o2, s2 = list(omega), list(sweeps)
-                o2[k], s2[k] = om, sw
-                c, _ = cost(o2, s2)
-                if c < ACCEPT * best:
-                    best, omega, sweeps = c, o2, s2
+    for _ in range(passes):
+        changed = False
+        for k in range(used - 1):  # the last level in use is solved directly
+            for om in spec["omegas"]:
+                for sw in (spec["sweeps"] if k == 0 else spec["coarse_sweeps"]):
+                    if om == omega[k] and sw == sweeps[k]:
+                        continue
+                    o2, s2 = list(omega), list(sweeps)
+                    o2[k], s2[k] = om, sw
+                    c, _ = cost(o2, s2)
+                    if c < ACCEPT * best:
+                        best, omega, sweeps = c, o2, s2
+                        changed = True
+        if not changed:
+            break
     return np.array(omega[:used - 1]), np.array(sweeps[:used - 1]), plain_cost, best
 
 
