@@ -444,7 +444,10 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "transfer_level_order") o.transfer_level_order = v != 0;
     else if (k == "block_transfers") o.block_transfers = v != 0;
     else if (k == "cg_level_order") o.cg_level_order = v != 0;
-    else if (k == "zero_start_lower") o.zero_start_lower = v != 0;
+    else if (k == "zero_start_lower") {
+      if (v < 0) throw std::runtime_error("zero_start_lower must be non-negative.");
+      o.zero_start_lower = static_cast<int>(v);
+    }
     else if (k == "transfer_sigma") {
       if (v < 0) throw std::runtime_error("transfer_sigma must be non-negative.");
       o.transfer_sigma = v;

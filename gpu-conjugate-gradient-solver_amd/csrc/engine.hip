@@ -309,7 +309,7 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
              opt_.group_min_block);
   lv.has_low = false;
-  if (!lv.direct && (lv.kind == kSGS || lv.kind == kBlockGS) && opt_.zero_start_lower &&
+  if (!lv.direct && (lv.kind == kSGS || lv.kind == kBlockGS) && k < opt_.zero_start_lower &&
       lv.ncolors > 1) {
     // Colours are contiguous internal ranges: an entry belongs to an earlier
     // colour exactly when its column (block column) lies below the row's
@@ -339,8 +339,12 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
           ++d;
         }
     }
-    lv.Alow.build(Low, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
-                  opt_.group_min_block);
+    // Same kernel family as the full operator (lane groups when it uses
+    // them; its half-length rows would otherwise fall back to one latency-
+    // bound thread per row on the coarse levels); the group size follows
+    // the lower rows' own length.
+    const double gmin = lv.A.G ? 0.0 : -1.0;
+    lv.Alow.build(Low, blocky, key.empty() ? nullptr : key.data(), gmin, gmin);
     lv.has_low = lv.Alow.b3 == lv.A.b3;
   }
   mark("level " + std::to_string(k) + " pack + upload");

@@ -58,9 +58,10 @@ struct Options {
   bool block_transfers = true;
   // The 3x3-block CG operator's rows run in level-0 order (row map).
   bool cg_level_order = true;
-  // Gauss-Seidel levels keep a second operator with the earlier-colour
-  // entries only, read by the forward colours of a sweep from x = 0.
-  bool zero_start_lower = true;
+  // Gauss-Seidel levels below this index keep a second operator with the
+  // earlier-colour entries only, read by the forward colours of a sweep
+  // from x = 0 (0: none).
+  int zero_start_lower = 1;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };

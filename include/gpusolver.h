@@ -184,10 +184,12 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *   "cg_level_order"  1: the 3x3-block CG operator's rows run in level-0
  *                     order through a row map (each row's sum unchanged;
  *                     the reductions group rows differently) (1)
- *   "zero_start_lower" 1: Gauss-Seidel levels keep a second operator with
- *                     only the entries of earlier colours; the forward
- *                     colours of a sweep from x = 0 read it (the entries it
- *                     leaves out multiply exact zeros; same results) (1)
+ *   "zero_start_lower" k: Gauss-Seidel levels below index k keep a second
+ *                     operator with only the entries of earlier colours; the
+ *                     forward colours of a sweep from x = 0 read it (the
+ *                     entries it leaves out multiply exact zeros; same
+ *                     results); 0: none (1: level 0, where the sweeps are
+ *                     bandwidth-bound)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed

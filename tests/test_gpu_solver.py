@@ -277,12 +277,13 @@ def test_zero_start_lower_bitexact(cfg1, elast, grp):
         r = gs.uniform_pm1(21, mat.rows)
         for sweeps in (1, 2):
             out = []
-            for low in (0, 1):
+            for low in (0, 1, 10):  # none, level 0, every level
                 s = gs.Solver(gs.PREC_DOUBLE, smoother=kind, omega=om, pre=sweeps, post=sweeps,
                               coarse_steps=2, zero_start_lower=low, **group_opts(grp))
                 s.setup(hier)
                 out.append(s.vcycle(r))
-            assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+            for z in out[1:]:
+                assert np.array_equal(out[0].view(np.uint64), z.view(np.uint64))
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 333])
