@@ -655,7 +655,8 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
         });
       } else if (block) {
         constexpr int CB = gsk::kColorBlock;
-        gsk::k_block_gs_color<TM><<<static_cast<uint32_t>((e - b + CB - 1) / CB), CB, 0, stream_>>>(
+        const int64_t b0 = b & ~int64_t(gsk::kSlice - 1);  // slice-aligned start
+        gsk::k_block_gs_color<TM><<<static_cast<uint32_t>((e - b0 + CB - 1) / CB), CB, 0, stream_>>>(
             M.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
       } else {
         gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(

@@ -537,18 +537,22 @@ __global__ __launch_bounds__(kBlock) void k_gs_color_zero(const T *__restrict__ 
 // [begin, end) (src/CPU/CPUBackend.cc:350-387): r_l = b_l - sum over
 // off-diagonal blocks, x_b = (1 - w) x_b + w D_b^-1 r.
 // `reps` as for k_gs_color.
-// Workgroup size of the thread-per-row block Gauss-Seidel colour launches:
-// a colour holds ~50k-165k block rows, and 64-thread workgroups spread them
-// over the 256 CUs more evenly than 256-thread ones (measured: level-0
-// sweep 0.249 -> 0.239 ms; 128 threads no gain).
+// Workgroup size of the thread-per-row block Gauss-Seidel colour launches
+// (one wavefront, one SELL slice): a colour holds ~50k-165k block rows, and
+// 64-thread workgroups spread them over the 256 CUs more evenly than
+// 256-thread ones (measured: level-0 sweep 0.249 -> 0.239 ms; 128 threads
+// no gain).
 constexpr int kColorBlock = 64;
 
 template <class T>
 __global__ __launch_bounds__(kColorBlock) void k_block_gs_color(
     Bsr3<T> A, const T *__restrict__ dinv, int64_t begin, int64_t end, const T *__restrict__ rhs,
     T *x, T omega, int reps, const int32_t *stop) {
-  const int64_t b = begin + xcd_block() * kColorBlock + threadIdx.x;
-  if (b >= end) return;
+  // Threads start at the SELL slice boundary below `begin`, so every
+  // wavefront covers exactly one slice (one head, one width); rows before
+  // `begin` belong to another colour and exit.
+  const int64_t b = (begin & ~int64_t(kSlice - 1)) + xcd_block() * kColorBlock + threadIdx.x;
+  if (b < begin || b >= end) return;
   const SliceHead h = bsr3_head(A, b);
   T bl[3], xl[3], bi[9];
 #pragma unroll
