@@ -394,6 +394,34 @@ int gs_hierarchy_aggregates(gs_hier_h h, int32_t k, int32_t *agg) {
   });
 }
 
+int gs_hierarchy_level_rows(gs_hier_h h, int32_t k, int64_t *rows) {
+  return guard([&] {
+    need(h, "hierarchy");
+    need(rows, "rows");
+    if (k < 0 || k >= static_cast<int32_t>(h->H.A.size()))
+      throw std::runtime_error("hierarchy index out of range.");
+    *rows = h->H.A[k].rows;
+  });
+}
+
+int gs_hierarchical_reordering(gs_hier_h h, int32_t *o2n, int32_t *n2o, int64_t *orphans) {
+  return guard([&] {
+    need(h, "hierarchy");
+    need(o2n, "old_to_new");
+    need(n2o, "new_to_old");
+    std::vector<int64_t> orph;
+    const std::vector<gsx::Perm> perms = gsx::hierarchical_reordering(h->H, &orph);
+    int64_t off = 0;
+    for (const gsx::Perm &p : perms) {
+      const size_t n = p.old_to_new.size();
+      std::memcpy(o2n + off, p.old_to_new.data(), n * sizeof(int32_t));
+      std::memcpy(n2o + off, p.new_to_old.data(), n * sizeof(int32_t));
+      off += static_cast<int64_t>(n);
+    }
+    if (orphans && !orph.empty()) std::memcpy(orphans, orph.data(), orph.size() * sizeof(int64_t));
+  });
+}
+
 int gs_hierarchy_features(gs_hier_h h, int32_t block_size, double *features) {
   return guard([&] {
     need(h, "hierarchy");

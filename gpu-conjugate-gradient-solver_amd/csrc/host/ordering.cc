@@ -1,11 +1,16 @@
-// Greedy multicolouring, reverse Cuthill-McKee, block graphs and block
-// permutation expansion.
+// Greedy multicolouring, reverse Cuthill-McKee, hierarchy-aware
+// reordering, block graphs and block permutation expansion.
 #include "setup.h"
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <cmath>
+#include <cstring>
 #include <limits>
+#include <memory>
 #include <queue>
+#include <stdexcept>
 
 namespace gsx {
 
@@ -90,6 +95,113 @@ Perm rcm(int64_t n, const int32_t *ptr, const int32_t *col) {
   for (int64_t i = 0; i < n; ++i) p.new_to_old[i] = cm[n - 1 - i];
   for (int64_t i = 0; i < n; ++i) p.old_to_new[p.new_to_old[i]] = static_cast<int32_t>(i);
   return p;
+}
+
+// Strongest coarse parent of every fine DOF
+// (src/ReorderingStrategies/HierarchicalReordering.cc:13-52): the coarse row
+// whose entry in the DOF's column has the largest |value| in FP32 (the
+// reference's hierarchy type), the first such entry in row-major order on a
+// tie, and -1 for an empty or all-NaN column. Every entry bids a 64-bit key,
+// |value|'s bits above the complement of its position, and one atomic max
+// per column keeps the winner whatever order the threads run in; the
+// winning position then maps back to its row.
+static std::vector<int32_t> strongest_parents(const Csr &R) {
+  const int64_t nf = R.cols;
+  std::unique_ptr<std::atomic<uint64_t>[]> key(new std::atomic<uint64_t>[nf]);
+#pragma omp parallel for schedule(static)
+  for (int64_t f = 0; f < nf; ++f) key[f].store(0, std::memory_order_relaxed);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t c = 0; c < R.rows; ++c)
+    for (int32_t e = R.ptr[c]; e < R.ptr[c + 1]; ++e) {
+      const float w = std::fabs(static_cast<float>(R.val[e]));
+      if (std::isnan(w)) continue;
+      uint32_t bits;
+      std::memcpy(&bits, &w, sizeof bits);
+      const uint64_t k = (static_cast<uint64_t>(bits) << 32) | (0xFFFFFFFFu - static_cast<uint32_t>(e));
+      std::atomic<uint64_t> &slot = key[R.col[e]];
+      uint64_t cur = slot.load(std::memory_order_relaxed);
+      while (k > cur && !slot.compare_exchange_weak(cur, k, std::memory_order_relaxed)) {
+      }
+    }
+  std::vector<int32_t> parent(nf);
+  const int32_t *pb = R.ptr.data(), *pe = pb + R.rows + 1;
+#pragma omp parallel for schedule(static)
+  for (int64_t f = 0; f < nf; ++f) {
+    const uint64_t k = key[f].load(std::memory_order_relaxed);
+    if (k == 0) {
+      parent[f] = -1;
+      continue;
+    }
+    const int32_t e = static_cast<int32_t>(0xFFFFFFFFu - static_cast<uint32_t>(k));
+    parent[f] = static_cast<int32_t>(std::upper_bound(pb, pe, e) - pb - 1);
+  }
+  return parent;
+}
+
+// src/ReorderingStrategies/HierarchicalReordering.cc:217-278 (driver) and
+// :74-200 (one level). The coarsest level takes its RCM order; each finer
+// level is grouped by the final rank of its DOFs' strongest parents, DOFs
+// without a parent last, and ordered within a group by the level's own RCM
+// rank. The reference sorts on that (parent rank, RCM rank) key, which is a
+// total order; a counting sort over the parent ranks, fed in RCM order, gives
+// the same permutation in linear time.
+std::vector<Perm> hierarchical_reordering(const Hierarchy &H, std::vector<int64_t> *orphans) {
+  const int64_t L = static_cast<int64_t>(H.A.size());
+  if (L == 0) throw std::runtime_error("HierarchicalReordering: empty hierarchy.");
+  if (static_cast<int64_t>(H.R.size()) + 1 != L)
+    throw std::runtime_error(
+        "HierarchicalReordering: expected one restriction matrix between consecutive levels.");
+  for (int64_t k = L - 2; k >= 0; --k) {
+    if (H.A[k].rows != H.A[k].cols)
+      throw std::runtime_error("HierarchicalReordering: fine matrix must be square.");
+    if (H.R[k].cols != H.A[k].rows)
+      throw std::runtime_error("HierarchicalReordering: restriction dimensions do not match fine matrix.");
+    // No counterpart in the reference, which indexes the coarse permutation
+    // with R's row numbers unchecked.
+    if (H.R[k].rows != H.A[k + 1].rows)
+      throw std::runtime_error("HierarchicalReordering: restriction rows do not match the coarse level.");
+  }
+  // No counterpart either: the reference runs RCM on a non-square coarsest
+  // matrix and reads past its degree table.
+  if (H.A[L - 1].rows != H.A[L - 1].cols)
+    throw std::runtime_error("HierarchicalReordering: coarsest matrix must be square.");
+  // Every level's own RCM order; the levels are independent.
+  std::vector<Perm> own(L);
+  std::vector<std::string> err(L);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t k = 0; k < L; ++k) {
+    try {
+      own[k] = rcm(H.A[k].rows, H.A[k].ptr.data(), H.A[k].col.data());
+      own[k].check();
+    } catch (const std::exception &e) {
+      err[k] = e.what();
+    }
+  }
+  for (int64_t k = L - 1; k >= 0; --k)
+    if (!err[k].empty()) throw std::runtime_error(err[k]);
+  std::vector<Perm> out(L);
+  out[L - 1] = own[L - 1];
+  if (orphans) orphans->assign(L - 1, 0);
+  for (int64_t k = L - 2; k >= 0; --k) {
+    const int64_t nf = H.A[k].rows, nc = H.A[k + 1].rows;
+    const std::vector<int32_t> parent = strongest_parents(H.R[k]);
+    const int32_t *crank = out[k + 1].old_to_new.data();
+    std::vector<int64_t> start(nc + 2, 0);
+    for (int64_t f = 0; f < nf; ++f) ++start[(parent[f] >= 0 ? crank[parent[f]] : nc) + 1];
+    if (orphans) (*orphans)[k] = start[nc + 1];
+    for (int64_t r = 0; r <= nc; ++r) start[r + 1] += start[r];
+    Perm &p = out[k];
+    p.new_to_old.resize(nf);
+    p.old_to_new.resize(nf);
+    for (int64_t i = 0; i < nf; ++i) {
+      const int32_t f = own[k].new_to_old[i];
+      const int64_t r = parent[f] >= 0 ? crank[parent[f]] : nc;
+      const int64_t pos = start[r]++;
+      p.new_to_old[pos] = f;
+      p.old_to_new[f] = static_cast<int32_t>(pos);
+    }
+  }
+  return out;
 }
 
 // src/BlockUtils.cc:126-216: every scalar entry coupling two different

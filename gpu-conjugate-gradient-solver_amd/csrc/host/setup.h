@@ -236,6 +236,12 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
                              const std::vector<double> &nullspace,
                              int32_t ncols);
 
+// HierarchicalReordering::compute
+// (src/ReorderingStrategies/HierarchicalReordering.cc:217-278): one
+// permutation per level, finest first. orphans (may be null) receives, per
+// transfer, the number of fine DOFs that R gives no parent.
+std::vector<Perm> hierarchical_reordering(const Hierarchy &H, std::vector<int64_t> *orphans);
+
 // Per-level features for the learned smoother weights, 5 per level:
 // mean row nnz, mean diagonal-dominance ratio |a_ii| / sum_{j!=i} |a_ij|
 // (capped at 10), coarsening ratio n_k / n_{k-1} (1 on level 0),

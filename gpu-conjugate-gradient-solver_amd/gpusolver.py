@@ -76,6 +76,8 @@ _SIGS = {
     "gs_hierarchy_levels": (C.c_int, [_vp, C.POINTER(_i32)]),
     "gs_hierarchy_get": (C.c_int, [_vp, _i32, _i32, C.POINTER(_vp)]),
     "gs_hierarchy_aggregates": (C.c_int, [_vp, _i32, _i32p]),
+    "gs_hierarchy_level_rows": (C.c_int, [_vp, _i32, C.POINTER(_i64)]),
+    "gs_hierarchical_reordering": (C.c_int, [_vp, _i32p, _i32p, _vp]),
     "gs_hierarchy_destroy": (C.c_int, [_vp]),
     "gs_solver_create": (C.c_int, [_i32, C.POINTER(_vp)]),
     "gs_solver_set_int": (C.c_int, [_vp, C.c_char_p, _i64]),
@@ -442,11 +444,35 @@ class Hierarchy:
         check(lib.gs_hierarchy_features(self.h, block_size, out))
         return out.reshape(self.levels, 5)
 
+    def level_rows(self, k: int) -> int:
+        n = _i64()
+        check(lib.gs_hierarchy_level_rows(self.h, k, C.byref(n)))
+        return n.value
+
     def aggregates(self, k: int) -> np.ndarray:
-        n = self.get("A", k).rows
+        n = self.level_rows(k)
         out = np.empty(n, np.int32)
         check(lib.gs_hierarchy_aggregates(self.h, k, out))
         return out
+
+
+def hierarchical_reordering(H: Hierarchy):
+    """HierarchicalReordering::compute
+    (src/ReorderingStrategies/HierarchicalReordering.cc:217-278): returns the
+    list of per-level (old_to_new, new_to_old) pairs, finest first, and the
+    per-transfer counts of fine DOFs without a parent in R (which the
+    reference prints)."""
+    sizes = [H.level_rows(k) for k in range(H.levels)]
+    total = int(sum(sizes))
+    o2n = np.empty(total, np.int32)
+    n2o = np.empty(total, np.int32)
+    orph = np.zeros(max(len(sizes) - 1, 1), np.int64)
+    check(lib.gs_hierarchical_reordering(H.h, o2n, n2o, orph.ctypes.data_as(_vp)))
+    perms, off = [], 0
+    for n in sizes:
+        perms.append((o2n[off:off + n].copy(), n2o[off:off + n].copy()))
+        off += n
+    return perms, orph[:len(sizes) - 1].copy()
 
 
 # ---------------------------------------------------------------------------

@@ -138,6 +138,23 @@ int gs_hierarchy_levels(gs_hier_h h, int32_t *nlev);
 /* which: 0 = A[k], 1 = P[k], 2 = R[k]; returns a new host CSR copy. */
 int gs_hierarchy_get(gs_hier_h h, int32_t which, int32_t k, gs_csr_h *out);
 int gs_hierarchy_aggregates(gs_hier_h h, int32_t k, int32_t *aggregate_of_dof);
+/* Rows of A[k] (hierarchy.A[k].rows(), include/GPUSolver/AMGHierarchy.h:
+ * 13-21) without copying the matrix. */
+int gs_hierarchy_level_rows(gs_hier_h h, int32_t k, int64_t *rows);
+/* HierarchicalReordering::compute (include/GPUSolver/ReorderingStrategies/
+ * HierarchicalReordering.h:10-16, src/ReorderingStrategies/
+ * HierarchicalReordering.cc:217-278): one permutation per level, the
+ * coarsest by RCM, every finer level grouped by the rank of each DOF's
+ * strongest coarse parent in R (largest FP32 |value|, first on a tie; DOFs
+ * without one last) and by its own level's RCM rank within a group.
+ * old_to_new / new_to_old receive the levels back to back, finest first
+ * (sum of the level sizes each), ready to pass level by level to
+ * gs_solver_setup. orphans (levels - 1 entries, may be NULL) receives per
+ * transfer the number of fine DOFs without a parent, which the reference
+ * prints to standard output instead. Errors are the reference's messages,
+ * plus two checks it lacks (R's rows against the coarse level, a square
+ * coarsest matrix). */
+int gs_hierarchical_reordering(gs_hier_h h, int32_t *old_to_new, int32_t *new_to_old, int64_t *orphans);
 int gs_hierarchy_destroy(gs_hier_h h);
 
 /* ------------------------------------------------------------------ */

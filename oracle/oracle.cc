@@ -4,7 +4,8 @@
 // (src/CPU/CPUBackend.cc), its PCG loop (src/CGSolver.cc), its V-cycle and
 // smoothers (src/Smoothers/*.cc), its preconditioner wrappers
 // (src/Preconditioners/*.cc) and its setup helpers (src/GraphColoring.cc,
-// src/ReorderingStrategies/RCMReordering.cc, src/HostELLMatrix.cc,
+// src/ReorderingStrategies/RCMReordering.cc,
+// src/ReorderingStrategies/HierarchicalReordering.cc, src/HostELLMatrix.cc,
 // src/HostSparseMatrix.cc, src/SparseMatrixUtils.cc, src/BlockUtils.cc),
 // templated on the value type so the same code serves as the FP32
 // "as-reference" path and the FP64 oracle. Only the test suite, the smoke
@@ -562,6 +563,66 @@ extern "C" int orc_rcm(int64_t n, const int32_t *ptr, const int32_t *col, int32_
     }
     for (int64_t i = 0; i < n; ++i) n2o[i] = cm[n - 1 - i];
     for (int64_t i = 0; i < n; ++i) o2n[n2o[i]] = static_cast<int32_t>(i);
+  });
+}
+
+// src/ReorderingStrategies/HierarchicalReordering.cc:13-52,74-200,217-278.
+// Level k's matrix is a_ptr[k] / a_col[k] (n[k] rows, square), transfer k's
+// restriction r_ptr[k] / r_col[k] / r_val[k] (n[k+1] x n[k]); the values are
+// rounded to FP32 first, the reference's hierarchy type. o2n / n2o receive
+// every level's permutation back to back, finest first; orphans[k] counts
+// the DOFs of level k with no parent in R_k (the reference prints it).
+extern "C" int orc_hierarchical_reordering(int32_t nlev, const int64_t *n, const int32_t *const *a_ptr,
+                                           const int32_t *const *a_col, const int32_t *const *r_ptr,
+                                           const int32_t *const *r_col, const double *const *r_val,
+                                           int32_t *o2n, int32_t *n2o, int64_t *orphans) {
+  ORC_TRY({
+    if (nlev <= 0) throw std::runtime_error("HierarchicalReordering: empty hierarchy.");
+    std::vector<int64_t> off(nlev + 1, 0);
+    for (int32_t k = 0; k < nlev; ++k) off[k + 1] = off[k] + n[k];
+    auto rcm_of = [&](int32_t k, std::vector<int32_t> &po, std::vector<int32_t> &pn) {
+      po.resize(n[k]);
+      pn.resize(n[k]);
+      if (orc_rcm(n[k], a_ptr[k], a_col[k], po.data(), pn.data()) != 0) throw std::runtime_error(g_err);
+    };
+    // Coarsest level: plain RCM.
+    std::vector<int32_t> co, cn;
+    rcm_of(nlev - 1, co, cn);
+    std::memcpy(o2n + off[nlev - 1], co.data(), n[nlev - 1] * sizeof(int32_t));
+    std::memcpy(n2o + off[nlev - 1], cn.data(), n[nlev - 1] * sizeof(int32_t));
+    for (int32_t k = nlev - 2; k >= 0; --k) {
+      const int64_t nf = n[k], nc = n[k + 1];
+      // Strongest parent: scan R row by row, keep the first largest |value|.
+      std::vector<int> parent(nf, -1);
+      std::vector<float> strongest(nf, -1.0f);
+      for (int64_t c = 0; c < nc; ++c)
+        for (int32_t e = r_ptr[k][c]; e < r_ptr[k][c + 1]; ++e) {
+          const int32_t f = r_col[k][e];
+          const float w = std::abs(static_cast<float>(r_val[k][e]));
+          if (w > strongest[f]) {
+            strongest[f] = w;
+            parent[f] = static_cast<int>(c);
+          }
+        }
+      int64_t orph = 0;
+      for (int64_t f = 0; f < nf; ++f) orph += parent[f] < 0;
+      orphans[k] = orph;
+      std::vector<int32_t> fo, fn;
+      rcm_of(k, fo, fn);
+      const int32_t *cperm = o2n + off[k + 1];  // the coarse level's final permutation
+      std::vector<int32_t> order(nf);
+      for (int64_t i = 0; i < nf; ++i) order[i] = static_cast<int32_t>(i);
+      std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        const int64_t ra = parent[a] >= 0 ? cperm[parent[a]] : nc;
+        const int64_t rb = parent[b] >= 0 ? cperm[parent[b]] : nc;
+        if (ra != rb) return ra < rb;
+        return fo[a] < fo[b];
+      });
+      for (int64_t i = 0; i < nf; ++i) {
+        n2o[off[k] + i] = order[i];
+        o2n[off[k] + order[i]] = static_cast<int32_t>(i);
+      }
+    }
   });
 }
 

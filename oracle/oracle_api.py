@@ -57,6 +57,8 @@ def _lib() -> C.CDLL:
         "orc_coloring": (C.c_int, [_i64, _i32p, _i32p, _i32p, _i32p, _i32p,
                                    C.POINTER(C.c_int32)]),
         "orc_rcm": (C.c_int, [_i64, _i32p, _i32p, _i32p, _i32p]),
+        "orc_hierarchical_reordering": (C.c_int, [C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _i32p,
+                                                  _i32p, _vp]),
         "orc_ell_pack_d": (C.c_int, [_i64, _i32p, _i32p, _f64p, C.c_int32, _vp, _vp, _vp, _vp, _vp,
                                      C.POINTER(_i64)]),
         "orc_permute_keep_order": (C.c_int, [_i64, _i32p, _i32p, _f64p, _i32p, _i32p, _i32p, _i32p,
@@ -146,6 +148,38 @@ def rcm(ptr, col):
     n2o = np.empty(n, np.int32)
     _check(lib.orc_rcm(n, ptr, col, o2n, n2o))
     return o2n, n2o
+
+
+def hierarchical_reordering(A, R):
+    """A: per-level (ptr, col) of square matrices, finest first; R: per-
+    transfer (ptr, col, val) restrictions (n[k+1] x n[k]). Returns the
+    per-level (old_to_new, new_to_old) pairs and the orphan counts."""
+    L = len(A)
+    n = np.array([len(a[0]) - 1 for a in A], np.int64)
+    keep = []
+
+    def arr(xs, dt):
+        xs = [np.ascontiguousarray(x, dt) for x in xs]
+        keep.extend(xs)
+        p = (_vp * max(len(xs), 1))(*[x.ctypes.data for x in xs])
+        keep.append(p)
+        return p
+
+    ap, ac = arr([a[0] for a in A], np.int32), arr([a[1] for a in A], np.int32)
+    rp = arr([r[0] for r in R], np.int32)
+    rc = arr([r[1] for r in R], np.int32)
+    rv = arr([r[2] for r in R], np.float64)
+    tot = int(n.sum())
+    o2n = np.empty(tot, np.int32)
+    n2o = np.empty(tot, np.int32)
+    orph = np.zeros(max(L - 1, 1), np.int64)
+    _check(lib.orc_hierarchical_reordering(L, n.ctypes.data, ap, ac, rp, rc, rv, o2n, n2o,
+                                           orph.ctypes.data))
+    out, off = [], 0
+    for k in range(L):
+        out.append((o2n[off:off + n[k]].copy(), n2o[off:off + n[k]].copy()))
+        off += int(n[k])
+    return out, orph[:L - 1].copy()
 
 
 def ell_pack(ptr, col, val, width):

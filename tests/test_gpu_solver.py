@@ -216,14 +216,16 @@ def test_pcg_single_precision_matches_reference_semantics(cfg1):
     assert np.linalg.norm(x - xo) <= 1e-4 * np.linalg.norm(xo)
 
 
-def test_pcg_with_rcm_permutations(cfg1):
+@pytest.mark.parametrize("ordering", ["rcm", "hierarchical"])
+def test_pcg_with_level_permutations(cfg1, ordering):
     """Level permutations as the reference drivers pass them (RCM on every
-    level; the CG operator uses level 0's, order kept)."""
+    level, or HierarchicalReordering's per-level list; the CG operator uses
+    level 0's, order kept)."""
     A, H = cfg1
-    perms = []
-    for k in range(H.levels):
-        Ak = H.get("A", k)
-        perms.append(gs.rcm(Ak)[0])
+    if ordering == "rcm":
+        perms = [gs.rcm(H.get("A", k))[0] for k in range(H.levels)]
+    else:
+        perms = [o2n for o2n, _ in gs.hierarchical_reordering(H)[0]]
     s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS)
     s.setup(H, perms)
     hv, _ = hier_view(H, perms)
