@@ -66,7 +66,7 @@ def build_workload(name: str):
 
 def measured_traffic(config: str, kernel_id: str):
     """HBM bytes per launch of kernel `kernel_id` (cg_spmv, l0_sweep,
-    l1_sweep, l0_restrict, l0_prolong) from the committed rocprofv3 counter
+    l1_sweep, l0_restrict, l0_prolong, l0_residual) from the committed rocprofv3 counter
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor the same
     counter reads for a copy of known size), or None when none was taken
     for this workload and kernel."""
@@ -240,6 +240,16 @@ def main() -> None:
                                     f"{'lane groups of %d' % grp if grp else 'thread per %s, SELL-C-sigma' % unit}"
                                     f", FP32, headline mode)",
                           "ms": t_ms, "bytes": t_bytes, "launches": 1, "per_iteration_ms": t_ms})
+    if len(hdesc["levels"]) > 1 and not hdesc["levels"][0]["direct"]:
+        # Level-0 residual rhs - A x of the V-cycle (the headline mode's
+        # operator SpMV, once per V-cycle).
+        r_ms, r_bytes = hs.time_kernel(7, 20)
+        grp = hdesc["levels"][0]["group"]
+        cands.append({"id": "l0_residual",
+                      "kernel": f"level-0 residual SpMV rhs - A x ("
+                                f"{'lane groups of %d' % grp if grp else 'thread per row'}"
+                                f", FP32, headline mode)",
+                      "ms": r_ms, "bytes": r_bytes, "launches": 1, "per_iteration_ms": r_ms})
     dom = max(cands[1:] or cands, key=lambda c: c["per_iteration_ms"])
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     desc_plain = plain.describe()

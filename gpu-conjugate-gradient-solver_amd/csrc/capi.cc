@@ -472,6 +472,11 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "transfer_level_order") o.transfer_level_order = v != 0;
     else if (k == "block_transfers") o.block_transfers = v != 0;
     else if (k == "cg_level_order") o.cg_level_order = v != 0;
+    else if (k == "cg_vcycle_operator") o.cg_vcycle_operator = v != 0;
+    else if (k == "cg_replace_every") {
+      if (v < 1) throw std::runtime_error("cg_replace_every must be at least 1.");
+      o.cg_replace_every = static_cast<int>(v);
+    }
     else if (k == "zero_start_lower") {
       if (v < 0) throw std::runtime_error("zero_start_lower must be non-negative.");
       o.zero_start_lower = static_cast<int>(v);

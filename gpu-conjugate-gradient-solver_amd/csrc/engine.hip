@@ -206,6 +206,11 @@ class Engine final : public EngineBase {
   int64_t hist_cap_ = 0;
   std::vector<double> hist_host_;
   bool setup_done_ = false;
+  // The CG SpMV reads level 0's V-cycle-type operator (cg_vcycle_operator,
+  // when the types differ and level 0 is stored one row or block row per
+  // thread); r is then replaced from the FP64 operator.
+  bool cg_vop_ = false;
+  int64_t replace_every() const { return cg_vop_ ? opt_.cg_replace_every : 0; }
   // Host wall time of the setup phases (ms), in order, for describe().
   std::vector<std::pair<std::string, double>> setup_ms_;
   std::chrono::steady_clock::time_point tmark_;
@@ -543,6 +548,8 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   }
   L_[0].rhs = rvc_;
   L_[0].x = z_.template as<TM>();
+  cg_vop_ = opt_.cg_vcycle_operator && !std::is_same<TC, TM>::value &&
+            opt_.precond == kPrecMultigrid && !L_[0].direct && L_[0].A.G == 0;
   partials_.alloc((3 * static_cast<size_t>(grid_for(n_)) + 64) * sizeof(double));
   ctl_.alloc(sizeof(gsk::Ctl<TC>));
   tr_.alloc(2 * sizeof(double));
@@ -808,8 +815,16 @@ template <class TC, class TM>
 void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
   TC *parts = partials_.template as<TC>();
-  const uint32_t g = grid_for(Acg_.b3 ? n_ / 3 : n_);
-  if (Acg_.b3)
+  const DOp<TM> &Av = L_[0].A;
+  const bool vb3 = cg_vop_ && Av.b3;
+  const uint32_t g = grid_for((cg_vop_ ? vb3 : Acg_.b3) ? n_ / 3 : n_);
+  if (cg_vop_ && vb3)
+    gsk::k_cg_spmv_pap_b3<TC, TM><<<g, kBlock, 0, stream_>>>(Av.b.view(), p_.template as<TC>(),
+                                                              q_.template as<TC>(), ctl, parts);
+  else if (cg_vop_)
+    gsk::k_cg_spmv_pap<TC, TM><<<g, kBlock, 0, stream_>>>(Av.s.view(), p_.template as<TC>(),
+                                                           q_.template as<TC>(), ctl, parts);
+  else if (Acg_.b3)
     gsk::k_cg_spmv_pap_b3<TC><<<g, kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>(),
                                                           ctl, parts);
   else
@@ -821,6 +836,17 @@ void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   gsk::k_cg_update_xr<TC, TM><<<gn, kBlock, 0, stream_>>>(
       x_.template as<TC>(), r_.template as<TC>(), rm, p_.template as<TC>(), q_.template as<TC>(), n_, ctl, parts);
   gsk::k_cg_rr_final<TC><<<1, kBlock, 0, stream_>>>(ctl, parts, gn);
+  if (cg_vop_) {
+    // Replacement (skipped on the device unless due): r = b - A x in FP64.
+    const uint32_t gr = grid_for(Acg_.b3 ? n_ / 3 : n_);
+    if (Acg_.b3)
+      gsk::k_cg_replace_b3<TC, TM><<<gr, kBlock, 0, stream_>>>(
+          Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm, ctl, parts);
+    else
+      gsk::k_cg_replace<TC, TM><<<gr, kBlock, 0, stream_>>>(
+          Acg_.s.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm, ctl, parts);
+    gsk::k_cg_replace_final<TC><<<1, kBlock, 0, stream_>>>(ctl, parts, gr);
+  }
   apply_preconditioner(stop);
   gsk::k_cg_rz<TC, TM><<<gn, kBlock, 0, stream_>>>(r_.template as<TC>(), z_.template as<TM>(), n_, ctl,
                                                    parts);
@@ -867,6 +893,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   c0.tol2 = static_cast<TC>(tol) * static_cast<TC>(tol);
   c0.maxit = maxit;
   c0.hist = hist_.template as<TC>();
+  c0.replace_every = replace_every();
   GS_HIP(hipMemcpyAsync(ctl_.get(), &c0, sizeof(c0), hipMemcpyHostToDevice, stream_));
   GS_HIP(hipStreamSynchronize(stream_));
 
@@ -1006,11 +1033,15 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
       encode_restrict(0, L_[0].res.template as<TM>(), L_[1].x, nullptr);
     } else if (which == 6) {
       encode_prolong(0, false, nullptr);
+    } else if (which == 7) {
+      encode_residual(L_[0], nullptr);
     } else {
       encode_smooth(L_[which - 3], 1, false, nullptr);
     }
   };
-  if (which < 0 || which > 6) throw std::runtime_error("time_kernel: unknown kernel id.");
+  if (which < 0 || which > 7) throw std::runtime_error("time_kernel: unknown kernel id.");
+  if (which == 7 && (opt_.precond != kPrecMultigrid || nlev_ < 2 || L_[0].direct))
+    throw std::runtime_error("time_kernel: no level-0 residual.");
   if ((which == 5 || which == 6) && (opt_.precond != kPrecMultigrid || nlev_ < 2))
     throw std::runtime_error("time_kernel: no level-0 transfer.");
   if ((which == 3 || which == 4) &&
@@ -1018,6 +1049,8 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
        (L_[which - 3].kind != kSGS && L_[which - 3].kind != kBlockGS)))
     throw std::runtime_error("time_kernel: that level has no Gauss-Seidel sweep.");
   if (which == 0) *bytes = Acg_.spmv_bytes();
+  // Level-0 residual: the operator's SpMV bytes plus the right-hand side read.
+  if (which == 7) *bytes = L_[0].A.spmv_bytes() + L_[0].n * static_cast<double>(sizeof(TM));
   if (which == 5 || which == 6) {
     // Entries (value + column) + the input vector once + the output vector
     // (restriction: rhs_c and the zeroed correction written; prolongation:
@@ -1049,6 +1082,7 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
     c0.rho = TC(1);
     c0.bb = TC(1);
     c0.hist = nullptr;
+    c0.replace_every = replace_every();
     GS_HIP(hipMemcpyAsync(ctl_.get(), &c0, sizeof(c0), hipMemcpyHostToDevice, stream_));
   }
   body();  // warm-up
@@ -1104,7 +1138,8 @@ std::string Engine<TC, TM>::describe() const {
   os << "], \"setup_ms\": {";
   for (size_t i = 0; i < setup_ms_.size(); ++i)
     os << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
-  os << "}, \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
+  os << "}, \"cg_vcycle_operator\": " << (cg_vop_ ? "true" : "false")
+     << ", \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
      << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters
      << "}";
   return os.str();

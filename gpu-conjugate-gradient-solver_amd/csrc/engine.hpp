@@ -62,6 +62,12 @@ struct Options {
   // earlier-colour entries only, read by the forward colours of a sweep
   // from x = 0 (0: none).
   int zero_start_lower = 1;
+  // Mixed precision: the CG SpMV reads level 0's V-cycle-type operator
+  // (FP32 values, products and sums in FP64) instead of the FP64 copy; the
+  // FP64 operator recomputes r = b - A x every cg_replace_every iterations
+  // and before every stop on convergence.
+  bool cg_vcycle_operator = false;
+  int cg_replace_every = 8;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };

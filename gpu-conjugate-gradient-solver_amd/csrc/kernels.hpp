@@ -82,6 +82,12 @@ struct Ctl {
   int32_t stop;       // 0 run, 1 converged, 2 maxit, 3 non-finite, 4 breakdown
   int32_t breakdown;  // 1: p.Ap <= 0, 2: r.z <= 0 (recorded, not fatal)
   T *hist;            // relres^2 per iteration (length maxit + 1) or null
+  // Residual replacement (CG SpMV on the V-cycle-type operator): every
+  // replace_every iterations, and whenever the recursive residual passes
+  // the tolerance, r is recomputed as b - A x with the full-precision
+  // operator and the stop test uses it. 0: off.
+  int64_t replace_every;
+  int32_t replace;    // set for the current iteration
 };
 
 // Blocks are dispatched round-robin over the eight XCDs; remap so each XCD
@@ -161,42 +167,46 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // ---------------------------------------------------------------------------
 // Row kernels (device helpers).
 // ---------------------------------------------------------------------------
-template <class T>
-__device__ __forceinline__ T sell_row_dot(const Sell<T> &A, int64_t row, const T *x) {
+// X: the type of x and of the products and sum (the stored values are
+// widened to it; X = T for every operator but the CG one read in the V-cycle
+// type).
+template <class T, class X = T>
+__device__ __forceinline__ X sell_row_dot(const Sell<T> &A, int64_t row, const X *x) {
   const int64_t s = row >> 6;
   const int64_t base = A.slice_ptr[s] + (row & 63);
   const int w = A.slice_w[s];
-  T sum = T(0);
+  X sum = X(0);
 #pragma unroll 4
   for (int j = 0; j < w; ++j) {
     const int64_t e = base + static_cast<int64_t>(j) * kSlice;
-    sum += ld_stream(A.val + e) * x[ld_stream(A.col + e)];
+    sum += static_cast<X>(ld_stream(A.val + e)) * x[ld_stream(A.col + e)];
   }
   return sum;
 }
 
 // y(3) = A(3 rows of block row b) x, scalar-row order kept: blocks in slot
 // order, within a block columns 0, 1, 2.
-template <class T>
-__device__ __forceinline__ void bsr3_row_dot(const Bsr3<T> &A, int64_t b, const T *x, T y[3],
+// X as in sell_row_dot.
+template <class T, class X = T>
+__device__ __forceinline__ void bsr3_row_dot(const Bsr3<T> &A, int64_t b, const X *x, X y[3],
                                              bool skip_diag) {
   const int64_t s = b >> 6;
   const int64_t lane = b & 63;
   const int64_t base = A.slice_ptr[s];
   const int w = A.slice_w[s];
-  T y0 = T(0), y1 = T(0), y2 = T(0);
+  X y0 = X(0), y1 = X(0), y2 = X(0);
 #pragma unroll 2
   for (int j = 0; j < w; ++j) {
     const int64_t slot = base + static_cast<int64_t>(j) * kSlice;
     const int32_t bc = ld_stream(A.bcol + slot + lane);
     if (skip_diag && bc == b) continue;
     const T *v = A.val + slot * 9 + lane;
-    const T x0 = x[3 * static_cast<int64_t>(bc)];
-    const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
-    const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
-    T vv[9];
+    const X x0 = x[3 * static_cast<int64_t>(bc)];
+    const X x1 = x[3 * static_cast<int64_t>(bc) + 1];
+    const X x2 = x[3 * static_cast<int64_t>(bc) + 2];
+    X vv[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) vv[e] = ld_stream(v + e * kSlice);
+    for (int e = 0; e < 9; ++e) vv[e] = static_cast<X>(ld_stream(v + e * kSlice));
     y0 += vv[0] * x0;
     y0 += vv[1] * x1;
     y0 += vv[2] * x2;
@@ -921,8 +931,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_alpha(Ctl<TC> *ctl, const TC *par
   }
 }
 
-template <class TC>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(Sell<TC> A, const TC *__restrict__ p,
+// TA: the stored operator's type (TC, or the V-cycle type with products and
+// sums in TC).
+template <class TC, class TA = TC>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(Sell<TA> A, const TC *__restrict__ p,
                                                         TC *__restrict__ q, Ctl<TC> *ctl,
                                                         TC *partials) {
   if (ctl->stop) return;
@@ -936,8 +948,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap(Sell<TC> A, const TC *__
   block_partial(v, partials);
 }
 
-template <class TC>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3(Bsr3<TC> A, const TC *__restrict__ p,
+template <class TC, class TA = TC>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3(Bsr3<TA> A, const TC *__restrict__ p,
                                                            TC *__restrict__ q, Ctl<TC> *ctl,
                                                            TC *partials) {
   if (ctl->stop) return;
@@ -993,12 +1005,84 @@ __global__ __launch_bounds__(kBlock) void k_cg_rr_final(Ctl<TC> *ctl, const TC *
     const int64_t it = ctl->iter;
     if (ctl->hist) ctl->hist[it + 1] = rel;
     ctl->iter = it + 1;
+    const int64_t every = ctl->replace_every;
+    if (!finite_val(rel)) {
+      ctl->stop = 3;
+    } else if (rel < ctl->tol2 && every == 0) {
+      ctl->stop = 1;
+      ctl->iters_out = it;
+    } else if (rel < ctl->tol2) {
+      ctl->replace = 1;  // confirm with the replaced residual
+    } else if (it + 1 >= ctl->maxit) {
+      ctl->stop = 2;
+      ctl->iters_out = ctl->maxit;
+    }
+    if (every > 0 && (it + 1) % every == 0) ctl->replace = 1;
+  }
+}
+
+// Residual replacement, when ctl->replace is set: r = b - A x with the
+// full-precision operator, its V-cycle-type copy, and partials of r.r.
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_replace(Sell<TC> A, const TC *__restrict__ b,
+                                                       const TC *__restrict__ x,
+                                                       TC *__restrict__ r, TM *__restrict__ rm,
+                                                       const Ctl<TC> *ctl, TC *partials) {
+  if (ctl->stop || !ctl->replace) return;
+  const int64_t row = xcd_block() * kBlock + threadIdx.x;
+  TC v = TC(0);
+  if (row < A.rows) {
+    const TC t = b[row] - sell_row_dot(A, row, x);
+    r[row] = t;
+    if (rm) rm[row] = static_cast<TM>(t);
+    v = t * t;
+  }
+  block_partial(v, partials);
+}
+
+template <class TC, class TM>
+__global__ __launch_bounds__(kBlock) void k_cg_replace_b3(Bsr3<TC> A, const TC *__restrict__ b,
+                                                          const TC *__restrict__ x,
+                                                          TC *__restrict__ r, TM *__restrict__ rm,
+                                                          const Ctl<TC> *ctl, TC *partials) {
+  if (ctl->stop || !ctl->replace) return;
+  const int64_t pos = xcd_block() * kBlock + threadIdx.x;
+  TC v = TC(0);
+  if (pos < A.nb) {
+    TC y[3];
+    bsr3_row_dot(A, pos, x, y, false);
+    const int64_t bb = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      const TC t = b[3 * bb + l] - y[l];
+      r[3 * bb + l] = t;
+      if (rm) rm[3 * bb + l] = static_cast<TM>(t);
+      v += t * t;
+    }
+  }
+  block_partial(v, partials);
+}
+
+// rr, rel and the stop test from the replaced residual (the iteration was
+// counted by k_cg_rr_final; its history entry is overwritten).
+template <class TC>
+__global__ __launch_bounds__(kBlock) void k_cg_replace_final(Ctl<TC> *ctl, const TC *partials,
+                                                             uint32_t G) {
+  if (ctl->stop || !ctl->replace) return;
+  const TC rr = fixed_sum(partials, G);
+  if (threadIdx.x == 0) {
+    const TC rel = rr / ctl->bb;
+    const int64_t it = ctl->iter;  // iterations done
+    ctl->rr = rr;
+    ctl->rel = rel;
+    ctl->replace = 0;
+    if (ctl->hist) ctl->hist[it] = rel;
     if (!finite_val(rel)) {
       ctl->stop = 3;
     } else if (rel < ctl->tol2) {
       ctl->stop = 1;
-      ctl->iters_out = it;
-    } else if (it + 1 >= ctl->maxit) {
+      ctl->iters_out = it - 1;
+    } else if (it >= ctl->maxit) {
       ctl->stop = 2;
       ctl->iters_out = ctl->maxit;
     }

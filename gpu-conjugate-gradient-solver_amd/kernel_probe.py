@@ -1,11 +1,11 @@
 """Runs the benchmark workload's hot kernels in isolation, for counter
 collection: `reps` launches of the CG-operator SpMV (FP64), `reps`
 level-0 and `reps` level-1 symmetric smoothing sweeps, `reps` level-0
-restrictions and `reps` level-0 prolongations in the V-cycle precision of
-the headline (mixed) mode, then `reps` launches of a plain
-device copy of known
-size (the calibration for FETCH_SIZE / WRITE_SIZE). Prints the algorithmic
-bytes of each so a profiler summary can be scaled.
+restrictions, `reps` level-0 prolongations and `reps` level-0 residuals in
+the V-cycle precision of the headline (mixed) mode, then `reps` launches of
+a plain device copy of known size (the calibration for FETCH_SIZE /
+WRITE_SIZE). Prints the algorithmic bytes of each so a profiler summary can
+be scaled.
 
 Usage (under rocprofv3):
   rocprofv3 --pmc FETCH_SIZE -d OUT -- python gpu-conjugate-gradient-solver_amd/kernel_probe.py
@@ -50,6 +50,7 @@ def main() -> None:
     sw1_ms, sw1_by = m.time_kernel(4, a.reps)
     r_ms, r_by = m.time_kernel(5, a.reps)
     p_ms, p_by = m.time_kernel(6, a.reps)
+    res_ms, res_by = m.time_kernel(7, a.reps)
     # Calibration copy: n doubles read and written once.
     n = a.copy_mib * (1 << 20) // 8
     be = gs.Backend(np.float64)
@@ -64,6 +65,7 @@ def main() -> None:
                       "sweep1_ms": sw1_ms, "sweep1_algorithmic_bytes": sw1_by,
                       "restrict_ms": r_ms, "restrict_algorithmic_bytes": r_by,
                       "prolong_ms": p_ms, "prolong_algorithmic_bytes": p_by,
+                      "residual_ms": res_ms, "residual_algorithmic_bytes": res_by,
                       "copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
                       "reps": a.reps}))
 

@@ -207,6 +207,12 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     entries it leaves out multiply exact zeros; same
  *                     results); 0: none (1: level 0, where the sweeps are
  *                     bandwidth-bound)
+ *   "cg_vcycle_operator" 1 (GS_PREC_MIXED): the CG SpMV reads level 0's
+ *                     FP32 V-cycle operator, products and sums in FP64; the
+ *                     FP64 operator recomputes r = b - A x every
+ *                     "cg_replace_every" iterations (8) and confirms every
+ *                     stop on convergence; needs level 0 smoothed and
+ *                     stored one row / block row per thread, else ignored (0)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed
@@ -251,7 +257,8 @@ int gs_solver_spmv(gs_solver_h s, const double *x, double *y);
 /* Average ms per launch of a hot kernel timed with HIP events on the
  * solver's stream: 0 CG SpMV, 1 V-cycle, 2 PCG iteration, 3 / 4 level-0 /
  * level-1 SGS or block-GS sweep, 5 level-0 restriction, 6 level-0
- * prolongation. *bytes: algorithmic bytes per launch. */
+ * prolongation, 7 level-0 residual (rhs - A x in the V-cycle type).
+ * *bytes: algorithmic bytes per launch. */
 int gs_solver_time_kernel(gs_solver_h s, int32_t which, int32_t reps, double *ms, double *bytes);
 int gs_solver_describe(gs_solver_h s, char *buf, int64_t cap);
 int gs_solver_destroy(gs_solver_h s);
