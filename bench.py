@@ -88,6 +88,27 @@ def learned_weights(config: str, H):
     return learned.weights_for(config, H)
 
 
+def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
+    """GB/s of a plain device copy (one FP64 element per thread, the kernel
+    the counter calibration uses), `reps` copies of `mib` MiB back to back
+    on one stream, host-timed around launch and completion: a practical
+    streaming ceiling next to the published peak."""
+    n = mib * (1 << 20) // 8
+    be = gs.Backend(np.float64)
+    x = be.create_vector(np.ones(n))
+    y = be.create_vector(n)
+    e = be.create_encoder()
+    be.encode_copy(e, x, y)
+    be.submit_and_wait(e)
+    e = be.create_encoder()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        be.encode_copy(e, x, y)
+    be.submit_and_wait(e)
+    t = time.perf_counter() - t0
+    return 2.0 * n * 8 * reps / t / 1e9
+
+
 def time_solves(s, b, steps, warmup, tol=1e-8, maxit=2000):
     infos = []
     for _ in range(warmup):
@@ -251,6 +272,7 @@ def main() -> None:
                                 f", FP32, headline mode)",
                       "ms": r_ms, "bytes": r_bytes, "launches": 1, "per_iteration_ms": r_ms})
     dom = max(cands[1:] or cands, key=lambda c: c["per_iteration_ms"])
+    copy_gbs = copy_bandwidth()
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     desc_plain = plain.describe()
 
@@ -298,6 +320,8 @@ def main() -> None:
                      "per_iteration_ms": round(c["per_iteration_ms"], 4),
                      "traffic": measured_traffic(a.config, c["id"])} for c in cands],
         "cpu_baseline": cpu,
+        "hbm_copy": {"GBs": round(copy_gbs, 1), "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
+                     "kernel": "k_copy<double>, 10 x 512 MiB, host-timed"},
         "setup_s": {"hierarchy": round(t_hier, 2), "plain_upload": round(t_setup_plain, 2)},
         "levels": desc_plain["levels"],
     }
