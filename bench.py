@@ -88,6 +88,17 @@ def learned_weights(config: str, H):
     return learned.weights_for(config, H)
 
 
+def progress(msg: str) -> None:
+    """One progress line per finished phase on stderr (stdout carries only
+    the result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def progress_mode(modes: dict, name: str) -> None:
+    m = modes[name]
+    progress(f"{name}: {m['device_ms']:.2f} ms, {m['iterations']} iterations")
+
+
 def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
     """GB/s of a plain device copy (one FP64 element per thread, the kernel
     the counter calibration uses), `reps` copies of `mib` MiB back to back
@@ -171,6 +182,7 @@ def main() -> None:
     A, b, hk, sk, desc = build_workload(a.config)
     H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
     t_hier = time.perf_counter() - t0
+    progress(f"hierarchy built: {H.levels} levels, {t_hier:.1f} s")
 
     modes = {}
     # Plain FP64 AMG-PCG, the reference's method in FP64.
@@ -182,6 +194,7 @@ def main() -> None:
     modes["plain_fp64"] = dict(ms=ms_plain, iterations=ip[-1].iterations,
                                true_relres=ip[-1].true_relres,
                                device_ms=float(np.mean([i.solve_ms for i in ip])))
+    progress_mode(modes, "plain_fp64")
     mixed_kw = dict(sk)
     # FP64 dense direct solve at the first level within max_direct unknowns
     # (the inverse is built on the device); deeper levels are not used.
@@ -195,6 +208,7 @@ def main() -> None:
     modes["fp64_direct"] = dict(ms=ms_fd, iterations=idd[-1].iterations,
                                 true_relres=idd[-1].true_relres,
                                 device_ms=float(np.mean([i.solve_ms for i in idd])))
+    progress_mode(modes, "fp64_direct")
     del fd
     # Mixed: FP32 V-cycle in FP64 CG, FP64 dense direct coarse solve.
     mixed = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
@@ -203,6 +217,7 @@ def main() -> None:
     modes["mixed"] = dict(ms=ms_mixed, iterations=im[-1].iterations,
                           true_relres=im[-1].true_relres,
                           device_ms=float(np.mean([i.solve_ms for i in im])))
+    progress_mode(modes, "mixed")
     head = "mixed"
     ml = None
     lw = learned_weights(a.config, H)
@@ -218,6 +233,7 @@ def main() -> None:
                                       fell_back=ml.fell_back(),
                                       omega=[round(float(w), 3) for w in lw[0]],
                                       sweeps=[int(v) for v in lw[1]])
+        progress_mode(modes, "mixed_learned")
         head = "mixed_learned"
     hm = modes[head]
     sols = {"plain_fp64": xp, "fp64_direct": xd, "mixed": xm}
@@ -278,6 +294,7 @@ def main() -> None:
 
     cpu = None
     if not a.no_cpu:
+        progress("kernels timed; CPU baseline running")
         xo, cpu = cpu_baseline(H, b, sk["smoother"], sk["omega"], sk["coarse_steps"])
         no = float(np.linalg.norm(xo))
         for name, xs in sols.items():
