@@ -104,6 +104,7 @@ def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
     the counter calibration uses), `reps` copies of `mib` MiB back to back
     on one stream, host-timed around launch and completion: a practical
     streaming ceiling next to the published peak."""
+    import gpusolver as gs
     n = mib * (1 << 20) // 8
     be = gs.Backend(np.float64)
     x = be.create_vector(np.ones(n))
