@@ -33,7 +33,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unknown-pragmas", "-Wno-unused-result"]
 
-HOST_SOURCES = ["host/sparse.cc", "host/ordering.cc", "host/amg.cc", "host/problems.cc"]
+HOST_SOURCES = ["host/sparse.cc", "host/ordering.cc", "host/amg.cc", "host/problems.cc",
+                "host/io.cc"]
 HIP_SOURCES = ["engine.hip", "backend.hip", "capi.cc", "mlp.hip", "dense.hip"]
 
 

@@ -344,6 +344,42 @@ int gs_hierarchy_build_sa(gs_csr_h A, int32_t bs, double eps, int32_t max_levels
   });
 }
 
+int gs_csr_save(gs_csr_h A, const char *path) {
+  return guard([&] {
+    need(A, "matrix");
+    need(path, "path");
+    gsx::save_csr(A->A, path);
+  });
+}
+
+int gs_csr_load(const char *path, gs_csr_h *out) {
+  return guard([&] {
+    need(path, "path");
+    need(out, "out");
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = gsx::load_csr(path);
+    *out = h.release();
+  });
+}
+
+int gs_hierarchy_save(gs_hier_h h, const char *path) {
+  return guard([&] {
+    need(h, "hierarchy");
+    need(path, "path");
+    gsx::save_hierarchy(h->H, path);
+  });
+}
+
+int gs_hierarchy_load(const char *path, gs_hier_h *out) {
+  return guard([&] {
+    need(path, "path");
+    need(out, "out");
+    auto h = std::make_unique<gs_hier_s>();
+    h->H = gsx::load_hierarchy(path);
+    *out = h.release();
+  });
+}
+
 int gs_hierarchy_create(gs_hier_h *out) {
   return guard([&] { *out = new gs_hier_s(); });
 }

@@ -236,6 +236,13 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
                              const std::vector<double> &nullspace,
                              int32_t ncols);
 
+// Binary files (layout in io.cc): one CSR matrix, or a whole hierarchy
+// (A, P, R and the aggregate maps). Loading validates every record.
+void save_csr(const Csr &A, const std::string &path);
+Csr load_csr(const std::string &path);
+void save_hierarchy(const Hierarchy &H, const std::string &path);
+Hierarchy load_hierarchy(const std::string &path);
+
 // HierarchicalReordering::compute
 // (src/ReorderingStrategies/HierarchicalReordering.cc:217-278): one
 // permutation per level, finest first. orphans (may be null) receives, per

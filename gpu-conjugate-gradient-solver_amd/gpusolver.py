@@ -71,6 +71,10 @@ _SIGS = {
     "gs_hierarchy_build_sa": (C.c_int, [_vp, _i32, C.c_double, _i32, _i64, _vp, _i32,
                                         C.POINTER(_vp)]),
     "gs_hierarchy_create": (C.c_int, [C.POINTER(_vp)]),
+    "gs_csr_save": (C.c_int, [_vp, C.c_char_p]),
+    "gs_csr_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "gs_hierarchy_save": (C.c_int, [_vp, C.c_char_p]),
+    "gs_hierarchy_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "gs_hierarchy_push_level": (C.c_int, [_vp, _vp]),
     "gs_hierarchy_push_transfer": (C.c_int, [_vp, _vp, _vp]),
     "gs_hierarchy_levels": (C.c_int, [_vp, C.POINTER(_i32)]),
@@ -211,6 +215,16 @@ class Csr:
                                 np.ascontiguousarray(self.col, np.int32),
                                 np.ascontiguousarray(self.val, np.float64), C.byref(h)))
         return CsrHandle(h)
+
+    def save(self, path) -> None:
+        """Binary CSR file (gs_csr_save)."""
+        hd = self.handle()  # keeps the C object alive during the call
+        check(lib.gs_csr_save(hd.h, os.fsencode(path)))
+
+
+def load_csr(path) -> Csr:
+    """Reads a file written by Csr.save (gs_csr_load)."""
+    return _from_handle_fn(lib.gs_csr_load, os.fsencode(path))
 
 
 class CsrHandle:
@@ -414,6 +428,17 @@ class Hierarchy:
         check(lib.gs_hierarchy_build_sa(ah.h, block_size, eps_strong, max_levels, min_dofs, nsp,
                                         ncols, C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def load(cls, path) -> "Hierarchy":
+        """Reads a file written by save (gs_hierarchy_load)."""
+        h = _vp()
+        check(lib.gs_hierarchy_load(os.fsencode(path), C.byref(h)))
+        return cls(h)
+
+    def save(self, path) -> None:
+        """Binary file of A, P, R and the aggregate maps (gs_hierarchy_save)."""
+        check(lib.gs_hierarchy_save(self.h, os.fsencode(path)))
 
     @classmethod
     def from_levels(cls, A: list, P: list, R: list) -> "Hierarchy":

@@ -156,6 +156,21 @@ int gs_hierarchy_level_rows(gs_hier_h h, int32_t k, int64_t *rows);
  * coarsest matrix). */
 int gs_hierarchical_reordering(gs_hier_h h, int32_t *old_to_new, int32_t *new_to_old, int64_t *orphans);
 int gs_hierarchy_destroy(gs_hier_h h);
+/* Binary files, no counterpart in the reference (whose drivers rebuild the
+ * hierarchy with amgcl on every run): one CSR matrix, or everything an
+ * AMGHierarchy holds (include/GPUSolver/AMGHierarchy.h:13-21: A, P, R) plus
+ * the aggregate maps. Little-endian: an 8-byte tag ("GSCSR001" /
+ * "GSHIER01"); for a hierarchy int64 levels L and transfers T, then L A
+ * records, T P records, T R records, int64 count of aggregate maps and each
+ * as int64 n + int32[n]; a matrix record is int64 rows, cols, nnz, int32
+ * ptr[rows + 1], int32 col[nnz], float64 val[nnz]. Loading checks sizes
+ * against the file before allocating and validates every CSR record
+ * ("truncated file", "corrupt ... header", "is not a ... file", "trailing
+ * bytes", or the CSR check's own message). */
+int gs_csr_save(gs_csr_h A, const char *path);
+int gs_csr_load(const char *path, gs_csr_h *out);
+int gs_hierarchy_save(gs_hier_h h, const char *path);
+int gs_hierarchy_load(const char *path, gs_hier_h *out);
 
 /* ------------------------------------------------------------------ */
 /* Solver: CGSolver (src/CGSolver.cc:18-308) with one of the reference's */
