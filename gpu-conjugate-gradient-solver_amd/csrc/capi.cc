@@ -312,6 +312,24 @@ int gs_problem_elasticity(int32_t nv, double E, double nu, double hs, double hsc
   });
 }
 
+int gs_problem_elasticity_mesh(int64_t nverts, const double *V, int64_t ntets, const int32_t *T,
+                               double young, double poisson, const double *young_per_tet,
+                               double clamp_rel, double density, double gravity, gs_csr_h *out,
+                               double *rhs, double *nullspace) {
+  return guard([&] {
+    need(V, "vertices");
+    need(T, "tetrahedra");
+    need(out, "out");
+    gsx::ElasticityProblem P = gsx::elasticity_mesh(nverts, V, ntets, T, young, poisson,
+                                                    young_per_tet, clamp_rel, density, gravity);
+    if (rhs) std::memcpy(rhs, P.rhs.data(), P.rhs.size() * sizeof(double));
+    if (nullspace) std::memcpy(nullspace, P.nullspace.data(), P.nullspace.size() * sizeof(double));
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = std::move(P.A);
+    *out = h.release();
+  });
+}
+
 int gs_uniform_pm1(uint64_t seed, int64_t n, double *out) {
   return guard([&] { gsx::uniform_pm1(seed, n, out); });
 }

@@ -4,6 +4,7 @@
 // reference reads from its (absent) data/ directory.
 #include "setup.h"
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 
@@ -405,6 +406,143 @@ ElasticityProblem elasticity(int nv, double E0, double nu, double hetero_sigma,
         B[0 * 6 + 4] = Z;  B[2 * 6 + 4] = -X;
         B[0 * 6 + 5] = -Y; B[1 * 6 + 5] = X;
       }
+  return P;
+}
+
+// examples/elasticity/cantileverBeam.cc:275-395 on a caller's tet mesh:
+// lambda and mu from E and nu as there, element matrices as
+// LinearElasticity.ipp:239-266 (a tet's matrix does not depend on its
+// vertex order, so the reference's orientation fix at :53-56 changes
+// nothing), vertices with x <= x_min + clamp_rel (x_max - x_min) clamped
+// and eliminated (constrainSimulation, :379-413; the free vertices keep
+// their input order), duplicates summed in element order (:325-348), and
+// rhs = density * lumped mass * (0, 0, gravity) per free vertex (:20-79,
+// cantileverBeam.cc:341-348). The rigid-body modes are taken about the
+// bounding-box centre. A zero-volume tet is an error here (the reference
+// would divide by its zero determinant).
+ElasticityProblem elasticity_mesh(int64_t nverts, const double *V, int64_t ntets,
+                                  const int32_t *T, double young, double nu,
+                                  const double *young_per_tet, double clamp_rel,
+                                  double density, double gravity) {
+  if (nverts <= 0 || ntets <= 0) throw std::runtime_error("elasticity mesh: empty mesh.");
+  if (nverts > INT32_MAX / 3) throw std::runtime_error("elasticity mesh: too many vertices.");
+  for (int64_t i = 0; i < 4 * ntets; ++i)
+    if (T[i] < 0 || T[i] >= nverts)
+      throw std::runtime_error("elasticity mesh: tetrahedron vertex index out of range.");
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) lo[d] = hi[d] = V[d];
+  for (int64_t v = 0; v < nverts; ++v)
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = std::min(lo[d], V[3 * v + d]);
+      hi[d] = std::max(hi[d], V[3 * v + d]);
+    }
+  const double clamp_x = lo[0] + clamp_rel * (hi[0] - lo[0]);
+  std::vector<int64_t> fidx(nverts, -1);
+  int64_t nfree = 0;
+  for (int64_t v = 0; v < nverts; ++v)
+    if (!(V[3 * v] <= clamp_x)) fidx[v] = nfree++;
+  if (nfree == nverts) throw std::runtime_error("No vertices found on clamped side.");
+  if (nfree == 0) throw std::runtime_error("elasticity mesh: every vertex is clamped.");
+  // Block pattern: every pair of free vertices sharing a tet.
+  std::vector<uint64_t> keys;
+  keys.reserve(static_cast<size_t>(ntets) * 16);
+  for (int64_t t = 0; t < ntets; ++t)
+    for (int a = 0; a < 4; ++a) {
+      const int64_t fa = fidx[T[4 * t + a]];
+      if (fa < 0) continue;
+      for (int b = 0; b < 4; ++b) {
+        const int64_t fb = fidx[T[4 * t + b]];
+        if (fb >= 0) keys.push_back((static_cast<uint64_t>(fa) << 32) | static_cast<uint64_t>(fb));
+      }
+    }
+  std::sort(keys.begin(), keys.end());
+  keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+  if (keys.size() * 9 > static_cast<size_t>(INT32_MAX))
+    throw std::runtime_error("elasticity mesh: more than 2^31 nonzeros.");
+  std::vector<int32_t> bptr(nfree + 1, 0), bcol(keys.size());
+  for (size_t i = 0; i < keys.size(); ++i) {
+    ++bptr[(keys[i] >> 32) + 1];
+    bcol[i] = static_cast<int32_t>(keys[i] & 0xFFFFFFFFu);
+  }
+  for (int64_t f = 0; f < nfree; ++f) bptr[f + 1] += bptr[f];
+  std::vector<double> bval(keys.size() * 9, 0.0), mass(nverts, 0.0);
+  const double mu0 = young / (2.0 * (1.0 + nu));
+  const double lam0 = young * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  for (int64_t t = 0; t < ntets; ++t) {
+    double X[4][3];
+    for (int a = 0; a < 4; ++a)
+      for (int d = 0; d < 3; ++d) X[a][d] = V[3 * static_cast<int64_t>(T[4 * t + a]) + d];
+    double lam = lam0, mu = mu0;
+    if (young_per_tet) {
+      const double E = young_per_tet[t];
+      mu = E / (2.0 * (1.0 + nu));
+      lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    }
+    double e[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) e[c][d] = X[c + 1][d] - X[0][d];
+    const double det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                       e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                       e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+    if (det == 0.0 || !std::isfinite(det))
+      throw std::runtime_error("elasticity mesh: degenerate tetrahedron.");
+    // Lumped mass: a quarter of the signed volume per vertex; inverted tets
+    // (negative volume) add none, as the reference skips them
+    // (LinearElasticity.ipp:31-47).
+    if (det > 0.0)
+      for (int a = 0; a < 4; ++a) mass[T[4 * t + a]] += det / 6.0 / 4.0;
+    const Mat12 H = tet_matrix(X, lam, mu);
+    for (int a = 0; a < 4; ++a) {
+      const int64_t fa = fidx[T[4 * t + a]];
+      if (fa < 0) continue;
+      for (int b = 0; b < 4; ++b) {
+        const int64_t fb = fidx[T[4 * t + b]];
+        if (fb < 0) continue;
+        const int32_t *lo_c = bcol.data() + bptr[fa], *hi_c = bcol.data() + bptr[fa + 1];
+        const int64_t slot = std::lower_bound(lo_c, hi_c, static_cast<int32_t>(fb)) - bcol.data();
+        double *blk = &bval[static_cast<size_t>(slot) * 9];
+        for (int k = 0; k < 3; ++k)
+          for (int l = 0; l < 3; ++l) blk[3 * k + l] += H[(3 * a + k) * 12 + 3 * b + l];
+      }
+    }
+  }
+  ElasticityProblem P;
+  P.free_vertices = nfree;
+  P.tets = ntets;
+  Csr &A = P.A;
+  A.rows = A.cols = 3 * nfree;
+  A.ptr.assign(A.rows + 1, 0);
+  for (int64_t f = 0; f < nfree; ++f)
+    for (int k = 0; k < 3; ++k) A.ptr[3 * f + k + 1] = 3 * (bptr[f + 1] - bptr[f]);
+  for (int64_t r = 0; r < A.rows; ++r) A.ptr[r + 1] += A.ptr[r];
+  A.col.resize(A.ptr[A.rows]);
+  A.val.resize(A.ptr[A.rows]);
+  for (int64_t f = 0; f < nfree; ++f)
+    for (int k = 0; k < 3; ++k) {
+      int64_t d = A.ptr[3 * f + k];
+      for (int32_t b = bptr[f]; b < bptr[f + 1]; ++b)
+        for (int l = 0; l < 3; ++l) {
+          A.col[d] = 3 * bcol[b] + l;
+          A.val[d] = bval[static_cast<size_t>(b) * 9 + 3 * k + l];
+          ++d;
+        }
+    }
+  P.rhs.assign(A.rows, 0.0);
+  P.nullspace.assign(static_cast<size_t>(A.rows) * 6, 0.0);
+  const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+  for (int64_t v = 0; v < nverts; ++v) {
+    const int64_t f = fidx[v];
+    if (f < 0) continue;
+    P.rhs[3 * f + 2] = density * mass[v] * gravity;
+    const double X = V[3 * v] - cx, Y = V[3 * v + 1] - cy, Z = V[3 * v + 2] - cz;
+    double *B = &P.nullspace[static_cast<size_t>(3 * f) * 6];
+    B[0 * 6 + 0] = 1.0;
+    B[1 * 6 + 1] = 1.0;
+    B[2 * 6 + 2] = 1.0;
+    B[1 * 6 + 3] = -Z; B[2 * 6 + 3] = Y;
+    B[0 * 6 + 4] = Z;  B[2 * 6 + 4] = -X;
+    B[0 * 6 + 5] = -Y; B[1 * 6 + 5] = X;
+  }
   return P;
 }
 

@@ -290,6 +290,14 @@ struct ElasticityProblem {
 // E = E0 * exp(hetero_scale * g) per hex from a Gaussian random field.
 ElasticityProblem elasticity(int nv, double E0, double nu, double hetero_sigma,
                              double hetero_scale, uint64_t seed);
+// The reference's cantilever driver on a caller's tet mesh (V: nverts x 3,
+// T: ntets x 4, both row-major, 0-based): clamp x <= x_min + clamp_rel *
+// length, element matrices with E (or young_per_tet[t]) and nu, duplicates
+// summed in element order, rhs = density * lumped mass * gravity in z.
+ElasticityProblem elasticity_mesh(int64_t nverts, const double *V, int64_t ntets,
+                                  const int32_t *T, double young, double nu,
+                                  const double *young_per_tet, double clamp_rel,
+                                  double density, double gravity);
 
 // Dense SPD inverse through Cholesky in FP64 (coarsest-level direct solve).
 // a: n x n row-major, overwritten with the inverse. Throws if not SPD.

@@ -71,6 +71,9 @@ _SIGS = {
     "gs_hierarchy_build_sa": (C.c_int, [_vp, _i32, C.c_double, _i32, _i64, _vp, _i32,
                                         C.POINTER(_vp)]),
     "gs_hierarchy_create": (C.c_int, [C.POINTER(_vp)]),
+    "gs_problem_elasticity_mesh": (C.c_int, [_i64, _f64p, _i64, _i32p, C.c_double, C.c_double, _vp,
+                                             C.c_double, C.c_double, C.c_double, C.POINTER(_vp),
+                                             _f64p, _f64p]),
     "gs_csr_save": (C.c_int, [_vp, C.c_char_p]),
     "gs_csr_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "gs_hierarchy_save": (C.c_int, [_vp, C.c_char_p]),
@@ -277,6 +280,89 @@ def elasticity(nv: int, young: float = 1.0, poisson: float = 0.3, hetero_sigma: 
     A = CsrHandle(h).to_csr()
     assert A.rows == n
     return A, rhs, ns.reshape(n, 6)
+
+
+def elasticity_mesh(V, T, young: float = 1.0e5, poisson: float = 0.35, young_per_tet=None,
+                    clamp_rel: float = 1.0e-4, density: float = 1.0, gravity: float = -9.81):
+    """The cantilever driver's system on a tet mesh (V: n x 3 vertices, T:
+    m x 4 zero-based tets; gs_problem_elasticity_mesh). Defaults are the
+    driver's (examples/elasticity/cantileverBeam.cc:291-348). Returns
+    (A, rhs, nullspace[n x 6])."""
+    V = np.ascontiguousarray(V, np.float64)
+    T = np.ascontiguousarray(T, np.int32)
+    if V.ndim != 2 or V.shape[1] != 3 or T.ndim != 2 or T.shape[1] != 4:
+        raise ValueError("V must be n x 3 and T m x 4")
+    ypt = None
+    if young_per_tet is not None:
+        ypt = np.ascontiguousarray(young_per_tet, np.float64)
+        if ypt.shape != (T.shape[0],):
+            raise ValueError("young_per_tet must hold one value per tet")
+    rhs = np.empty(3 * V.shape[0], np.float64)
+    ns = np.empty(18 * V.shape[0], np.float64)
+    h = _vp()
+    check(lib.gs_problem_elasticity_mesh(V.shape[0], V, T.shape[0], T, young, poisson,
+                                         None if ypt is None else ypt.ctypes.data_as(_vp),
+                                         clamp_rel, density, gravity, C.byref(h), rhs, ns))
+    A = CsrHandle(h).to_csr()
+    n = A.rows
+    return A, rhs[:n].copy(), ns[:6 * n].reshape(n, 6).copy()
+
+
+def read_mesh(path):
+    """Tetrahedral mesh in the MEDIT ASCII format (.mesh: the format the
+    reference's drivers load with igl::readMESH): returns V (n x 3), T (m x 4,
+    zero-based) and the boundary triangles F (k x 3, zero-based)."""
+    with open(path) as fh:
+        tok = [t for line in fh for t in line.split("#", 1)[0].split()]
+    V = np.zeros((0, 3))
+    T = np.zeros((0, 4), np.int32)
+    F = np.zeros((0, 3), np.int32)
+    i = 0
+    while i < len(tok):
+        key = tok[i]
+        i += 1
+        if key in ("MeshVersionFormatted", "Dimension"):
+            i += 1
+        elif key in ("Vertices", "Triangles", "Tetrahedra", "Edges", "Corners", "Ridges",
+                     "RequiredVertices"):
+            n = int(tok[i])
+            i += 1
+            width = {"Vertices": 4, "Triangles": 4, "Tetrahedra": 5, "Edges": 3}.get(key, 1)
+            if n < 0 or i + n * width > len(tok):
+                raise ValueError(f"{path}: truncated {key} section")
+            rows = np.array(tok[i:i + n * width], dtype=np.float64).reshape(n, width)
+            i += n * width
+            if key == "Vertices":
+                V = rows[:, :3].copy()
+            elif key == "Triangles":
+                F = rows[:, :3].astype(np.int32) - 1
+            elif key == "Tetrahedra":
+                T = rows[:, :4].astype(np.int32) - 1
+        elif key == "End":
+            break
+        else:
+            raise ValueError(f"{path}: unsupported section {key!r}")
+    if T.size and (T.min() < 0 or T.max() >= len(V)):
+        raise ValueError(f"{path}: tetrahedron vertex index out of range")
+    return V, T, F
+
+
+def write_mesh(path, V, T, F=None) -> None:
+    """Writes V, T (zero-based) and optional triangles F in the MEDIT ASCII
+    format read by read_mesh."""
+    with open(path, "w") as fh:
+        fh.write("MeshVersionFormatted 1\nDimension 3\n")
+        fh.write(f"Vertices\n{len(V)}\n")
+        for x, y, z in np.asarray(V, np.float64).tolist():
+            fh.write(f"{x!r} {y!r} {z!r} 0\n")
+        if F is not None and len(F):
+            fh.write(f"Triangles\n{len(F)}\n")
+            for a, b, c in (np.asarray(F) + 1).tolist():
+                fh.write(f"{a} {b} {c} 0\n")
+        fh.write(f"Tetrahedra\n{len(T)}\n")
+        for a, b, c, d in (np.asarray(T) + 1).tolist():
+            fh.write(f"{a} {b} {c} {d} 0\n")
+        fh.write("End\n")
 
 
 def uniform_pm1(seed: int, n: int) -> np.ndarray:

@@ -117,6 +117,22 @@ int gs_problem_varcoef3d(int32_t n, uint64_t seed, double sigma, double scale, g
 int gs_problem_elasticity(int32_t nv, double young, double poisson, double hetero_sigma,
                           double hetero_scale, uint64_t seed, gs_csr_h *out, double *rhs,
                           double *nullspace);
+/* The cantilever driver's system on a caller's tet mesh
+ * (examples/elasticity/cantileverBeam.cc:275-348 with
+ * examples/elasticity/LinearElasticity.ipp:20-79,115-147,239-348,379-413):
+ * V nverts x 3 and T ntets x 4 (0-based), row-major; young_per_tet may be
+ * NULL (young everywhere; the driver uses 1e5 with poisson 0.35); vertices
+ * with x <= x_min + clamp_rel * (x_max - x_min) are clamped (the driver:
+ * 1e-4); rhs = density * lumped mass * (0, 0, gravity) (the driver: 1,
+ * -9.81). rhs and nullspace (rigid-body modes, rows x 6) may be NULL, else
+ * must hold 3 * nverts and 18 * nverts values; the first rows of the
+ * returned matrix are written. Errors: "No vertices found on clamped
+ * side." (the driver's), empty mesh, vertex index out of range, degenerate
+ * tetrahedron. */
+int gs_problem_elasticity_mesh(int64_t nverts, const double *V, int64_t ntets, const int32_t *T,
+                               double young, double poisson, const double *young_per_tet,
+                               double clamp_rel, double density, double gravity, gs_csr_h *out,
+                               double *rhs, double *nullspace);
 int gs_uniform_pm1(uint64_t seed, int64_t n, double *out);
 int gs_gaussian_field(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double sigma, double *out);
 
