@@ -3,6 +3,13 @@ path): gs_problem_elasticity_mesh's system solved on the GPU with the
 driver's preconditioner (SA(3, 0.1), multigrid block Gauss-Seidel 1/1,
 omega 0.7, no coarsest-level sweeps, examples/elasticity/cantileverBeam.cc:
 413-415,572-573) matches the oracle's PCG on the same hierarchy.
+
+FP64: iterations within +-2 of the oracle. Mixed precision (FP32 V-cycle):
+this weakly preconditioned beam needs 130 iterations in FP64 and ~154 with
+the FP32 V-cycle, and there the count follows rounding: relative
+perturbations of 1e-15 in b move the oracle's own mixed count over 151-155
+on the CPU. The mixed bar is therefore the FP64-verified residual, the
+solution against the FP64 oracle, and the count within 5% of the oracle's.
 """
 import numpy as np
 import pytest
@@ -37,9 +44,15 @@ def test_cantilever_mesh_pcg(tmp_path, prec):
     x, info = s.solve(b, tol=1e-8, maxit=1000)
     xo, ito, _, _ = orc.pcg(hier_view(H), b, tol=1e-8, maxit=1000, kinds=gs.BLOCK_GS, omegas=0.7,
                             coarse_steps=0, mg_double=prec == gs.PREC_DOUBLE)
-    assert info.converged and abs(info.iterations - ito) <= 2, (info.iterations, ito)
+    assert info.converged
+    if prec == gs.PREC_DOUBLE:
+        assert abs(info.iterations - ito) <= 2, (info.iterations, ito)
+    else:
+        assert abs(info.iterations - ito) <= 0.05 * ito, (info.iterations, ito)
     assert info.true_relres <= 1.5e-8
-    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+    x64 = xo if prec == gs.PREC_DOUBLE else orc.pcg(
+        hier_view(H), b, tol=1e-8, maxit=1000, kinds=gs.BLOCK_GS, omegas=0.7, coarse_steps=0)[0]
+    assert np.linalg.norm(x - x64) <= 1e-6 * np.linalg.norm(x64)
     # The beam sags under gravity: the free end moves down.
     tip = np.flatnonzero(V[V[:, 0] > 1e-4 * np.ptp(V[:, 0]), 0] == V[:, 0].max())
     assert (x[3 * tip + 2] < 0).all()
