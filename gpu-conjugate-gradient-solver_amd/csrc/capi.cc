@@ -527,6 +527,7 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "block_transfers") o.block_transfers = v != 0;
     else if (k == "cg_level_order") o.cg_level_order = v != 0;
     else if (k == "cg_vcycle_operator") o.cg_vcycle_operator = v != 0;
+    else if (k == "cg_pair_layout") o.cg_pair_layout = v != 0;
     else if (k == "cg_replace_every") {
       if (v < 1) throw std::runtime_error("cg_replace_every must be at least 1.");
       o.cg_replace_every = static_cast<int>(v);

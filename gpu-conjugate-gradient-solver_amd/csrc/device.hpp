@@ -147,9 +147,14 @@ struct DBsr3 {
   int64_t slots = 0;
   DevBuf slice_ptr, slice_w, bcol, val, row_map;
   bool mapped = false;
+  // Values as four planes of pairs and one plane per 64-slot group
+  // (gsx::bsr3_pair_planes); set before build, read by the kernels that
+  // take the PAIRS layout.
+  bool pairs = false;
 
   void build(const gsx::Bsr3Csr &B) {
-    const gsx::Bsr3Sell64 P = gsx::bsr3_sell64_pack(B);
+    gsx::Bsr3Sell64 P = gsx::bsr3_sell64_pack(B);
+    if (pairs) gsx::bsr3_pair_planes(P);
     nb = P.nb;
     nblocks = P.nblocks;
     slots = P.slots;
@@ -185,6 +190,7 @@ struct DBsr3 {
     return gsk::Bsr3<T>{nb, slice_ptr.as<int64_t>(), slice_w.as<int32_t>(), bcol.as<int32_t>(),
                         val.as<T>(), mapped ? row_map.as<int32_t>() : nullptr};
   }
+  int64_t slices() const { return (nb + 63) / 64; }
   size_t device_bytes() const {
     return slice_ptr.bytes() + slice_w.bytes() + bcol.bytes() + val.bytes() + row_map.bytes();
   }

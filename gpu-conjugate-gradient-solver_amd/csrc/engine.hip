@@ -210,6 +210,8 @@ class Engine final : public EngineBase {
   // when the types differ and level 0 is stored one row or block row per
   // thread); r is then replaced from the FP64 operator.
   bool cg_vop_ = false;
+  // The CG operator is stored as planes of value pairs (FP64, block SELL).
+  bool cg_pairs_ = false;
   int64_t replace_every() const { return cg_vop_ ? opt_.cg_replace_every : 0; }
   // Host wall time of the setup phases (ms), in order, for describe().
   std::vector<std::pair<std::string, double>> setup_ms_;
@@ -529,8 +531,10 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       ord.resize(n_ / 3);
       for (int64_t p = 0; p < n_ / 3; ++p) ord[p] = q0inv[3 * p] / 3;
     }
+    Acg_.b.pairs = opt_.cg_pair_layout && std::is_same<TC, double>::value;
     Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data(), -1.0, -1.0,
                ord.empty() ? nullptr : ord.data());
+    cg_pairs_ = Acg_.b3 && Acg_.b.pairs;
     mark("CG operator");
   }
   // CG vectors and control block.
@@ -550,7 +554,10 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   L_[0].x = z_.template as<TM>();
   cg_vop_ = opt_.cg_vcycle_operator && !std::is_same<TC, TM>::value &&
             opt_.precond == kPrecMultigrid && !L_[0].direct && L_[0].A.G == 0;
-  partials_.alloc((3 * static_cast<size_t>(grid_for(n_)) + 64) * sizeof(double));
+  // Partials: one per workgroup of the widest reduction (three segments of
+  // the n-thread grid, or one per workgroup of a persistent launch).
+  partials_.alloc((std::max<size_t>(3 * static_cast<size_t>(grid_for(n_)), gsk::kPersistGrid) + 64) *
+                  sizeof(double));
   ctl_.alloc(sizeof(gsk::Ctl<TC>));
   tr_.alloc(2 * sizeof(double));
   if (!host_ctl_) GS_HIP(hipHostMalloc(reinterpret_cast<void **>(&host_ctl_), sizeof(gsk::Ctl<TC>)));
@@ -817,8 +824,12 @@ void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   TC *parts = partials_.template as<TC>();
   const DOp<TM> &Av = L_[0].A;
   const bool vb3 = cg_vop_ && Av.b3;
-  const uint32_t g = grid_for((cg_vop_ ? vb3 : Acg_.b3) ? n_ / 3 : n_);
-  if (cg_vop_ && vb3)
+  const uint32_t g = (!cg_vop_ && cg_pairs_) ? gsk::kPersistGrid
+                                              : grid_for((cg_vop_ ? vb3 : Acg_.b3) ? n_ / 3 : n_);
+  if (!cg_vop_ && cg_pairs_)
+    gsk::k_cg_spmv_pap_b3_persist<TC, true><<<g, kBlock, 0, stream_>>>(
+        Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>(), ctl, parts);
+  else if (cg_vop_ && vb3)
     gsk::k_cg_spmv_pap_b3<TC, TM><<<g, kBlock, 0, stream_>>>(Av.b.view(), p_.template as<TC>(),
                                                               q_.template as<TC>(), ctl, parts);
   else if (cg_vop_)
@@ -839,7 +850,10 @@ void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   if (cg_vop_) {
     // Replacement (skipped on the device unless due): r = b - A x in FP64.
     const uint32_t gr = grid_for(Acg_.b3 ? n_ / 3 : n_);
-    if (Acg_.b3)
+    if (cg_pairs_)
+      gsk::k_cg_replace_b3<TC, TM, true><<<gr, kBlock, 0, stream_>>>(
+          Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm, ctl, parts);
+    else if (Acg_.b3)
       gsk::k_cg_replace_b3<TC, TM><<<gr, kBlock, 0, stream_>>>(
           Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm, ctl, parts);
     else
@@ -901,7 +915,10 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   auto t0 = std::chrono::steady_clock::now();
   auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
   TM *rm = std::is_same<TC, TM>::value ? nullptr : rm_.template as<TM>();
-  if (Acg_.b3)
+  if (cg_pairs_)
+    gsk::k_cg_init_r_b3<TC, TM, true><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(
+        Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm);
+  else if (Acg_.b3)
     gsk::k_cg_init_r_b3<TC, TM><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(
         Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), r_.template as<TC>(), rm);
   else
@@ -958,7 +975,10 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   gsx::scatter_cast(xi, int2orig_.data(), n_, x);
   // True residual of the CG operator in FP64 arithmetic.
   const uint32_t gt = grid_for(Acg_.b3 ? n_ / 3 : n_);
-  if (Acg_.b3)
+  if (cg_pairs_)
+    gsk::k_true_residual_b3<TC, true><<<gt, kBlock, 0, stream_>>>(
+        Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), partials_.template as<double>());
+  else if (Acg_.b3)
     gsk::k_true_residual_b3<TC><<<gt, kBlock, 0, stream_>>>(
         Acg_.b.view(), b_.template as<TC>(), x_.template as<TC>(), partials_.template as<double>());
   else
@@ -994,7 +1014,10 @@ void Engine<TC, TM>::spmv_level_order(const double *x, double *y) {
   std::vector<TC> xi(n_), yi(n_);
   for (int64_t i = 0; i < n_; ++i) xi[i] = static_cast<TC>(x[int2lvl0_[i]]);
   GS_HIP(hipMemcpy(p_.get(), xi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice));
-  if (Acg_.b3)
+  if (cg_pairs_)
+    gsk::k_spmv_b3_persist<TC, true><<<gsk::kPersistGrid, kBlock, 0, stream_>>>(
+        Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>());
+  else if (Acg_.b3)
     gsk::k_spmv_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(),
                                                                  q_.template as<TC>(), nullptr);
   else
@@ -1019,7 +1042,10 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
   *bytes = 0.0;
   auto body = [&]() {
     if (which == 0) {
-      if (Acg_.b3)
+      if (cg_pairs_)
+        gsk::k_spmv_b3_persist<TC, true><<<gsk::kPersistGrid, kBlock, 0, stream_>>>(
+            Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>());
+      else if (Acg_.b3)
         gsk::k_spmv_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(),
                                                                      q_.template as<TC>(), nullptr);
       else
@@ -1139,6 +1165,7 @@ std::string Engine<TC, TM>::describe() const {
   for (size_t i = 0; i < setup_ms_.size(); ++i)
     os << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
   os << "}, \"cg_vcycle_operator\": " << (cg_vop_ ? "true" : "false")
+     << ", \"cg_pair_layout\": " << (cg_pairs_ ? "true" : "false")
      << ", \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
      << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters
      << "}";

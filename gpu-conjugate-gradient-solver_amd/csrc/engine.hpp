@@ -67,6 +67,9 @@ struct Options {
   // FP64 operator recomputes r = b - A x every cg_replace_every iterations
   // and before every stop on convergence.
   bool cg_vcycle_operator = false;
+  // The FP64 3x3-block CG operator's values as planes of value pairs
+  // (16-byte loads) and its SpMV as a persistent launch walking slices.
+  bool cg_pair_layout = true;
   int cg_replace_every = 8;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)

@@ -180,6 +180,11 @@ struct Bsr3Sell64 {
   RawVec<double> val;
 };
 Bsr3Sell64 bsr3_sell64_pack(const Bsr3Csr &B);
+// Regroups each 64-slot group's nine value planes into four planes of value
+// pairs (lane l holds values 2q and 2q + 1 of its block side by side, 16
+// bytes in FP64) followed by the ninth plane: offsets 128 q + 2 l + (e & 1)
+// for e = 2q, 2q + 1, and 512 + l for e = 8.
+void bsr3_pair_planes(Bsr3Sell64 &P);
 
 // ---------------------------------------------------------------------------
 // Colouring and orderings.

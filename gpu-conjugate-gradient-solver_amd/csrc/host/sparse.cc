@@ -563,6 +563,19 @@ Sell64 sell64_pack(const Csr &A) {
   return out;
 }
 
+void bsr3_pair_planes(Bsr3Sell64 &P) {
+  const int64_t groups = P.slots / 64;
+#pragma omp parallel for schedule(static)
+  for (int64_t g = 0; g < groups; ++g) {
+    double t[576];
+    double *v = P.val.data() + g * 576;
+    for (int e = 0; e < 9; ++e)
+      for (int l = 0; l < 64; ++l)
+        t[e < 8 ? 128 * (e >> 1) + 2 * l + (e & 1) : 512 + l] = v[64 * e + l];
+    for (int i = 0; i < 576; ++i) v[i] = t[i];
+  }
+}
+
 Bsr3Sell64 bsr3_sell64_pack(const Bsr3Csr &B) {
   constexpr int64_t S = 64;
   Bsr3Sell64 out;

@@ -186,8 +186,30 @@ __device__ __forceinline__ X sell_row_dot(const Sell<T> &A, int64_t row, const X
 
 // y(3) = A(3 rows of block row b) x, scalar-row order kept: blocks in slot
 // order, within a block columns 0, 1, 2.
-// X as in sell_row_dot.
-template <class T, class X = T>
+// The nine values of the block at `slot` for lane `lane`: nine planes of 64
+// (PAIRS false), or four planes of value pairs then one plane (PAIRS true,
+// gsx::bsr3_pair_planes: one 16-byte load per pair in FP64).
+template <bool PAIRS, class T>
+__device__ __forceinline__ void bsr3_block_values(const T *val, int64_t slot, int64_t lane, T vv[9]) {
+  if constexpr (PAIRS) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const T *v = val + slot * 9;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const T2 t = ld_stream(reinterpret_cast<const T2 *>(v + q * 2 * kSlice) + lane);
+      vv[2 * q] = t.x;
+      vv[2 * q + 1] = t.y;
+    }
+    vv[8] = ld_stream(v + 8 * kSlice + lane);
+  } else {
+    const T *v = val + slot * 9 + lane;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) vv[e] = ld_stream(v + e * kSlice);
+  }
+}
+
+// X as in sell_row_dot; PAIRS as in bsr3_block_values.
+template <class T, class X = T, bool PAIRS = false>
 __device__ __forceinline__ void bsr3_row_dot(const Bsr3<T> &A, int64_t b, const X *x, X y[3],
                                              bool skip_diag) {
   const int64_t s = b >> 6;
@@ -200,13 +222,14 @@ __device__ __forceinline__ void bsr3_row_dot(const Bsr3<T> &A, int64_t b, const 
     const int64_t slot = base + static_cast<int64_t>(j) * kSlice;
     const int32_t bc = ld_stream(A.bcol + slot + lane);
     if (skip_diag && bc == b) continue;
-    const T *v = A.val + slot * 9 + lane;
     const X x0 = x[3 * static_cast<int64_t>(bc)];
     const X x1 = x[3 * static_cast<int64_t>(bc) + 1];
     const X x2 = x[3 * static_cast<int64_t>(bc) + 2];
+    T tv[9];
+    bsr3_block_values<PAIRS>(A.val, slot, lane, tv);
     X vv[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) vv[e] = static_cast<X>(ld_stream(v + e * kSlice));
+    for (int e = 0; e < 9; ++e) vv[e] = static_cast<X>(tv[e]);
     y0 += vv[0] * x0;
     y0 += vv[1] * x1;
     y0 += vv[2] * x2;
@@ -353,18 +376,52 @@ __global__ __launch_bounds__(kBlock) void k_spmv(Sell<T> A, const T *__restrict_
   y[row] = sell_row_dot(A, row, x);
 }
 
-template <class T>
+template <class T, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_spmv_b3(Bsr3<T> A, const T *__restrict__ x,
                                                     T *__restrict__ y, const int32_t *stop) {
   if (stopped(stop)) return;
   const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   if (pos >= A.nb) return;
   T v[3];
-  bsr3_row_dot(A, pos, x, v, false);
+  bsr3_row_dot<T, T, PAIRS>(A, pos, x, v, false);
   const int64_t b = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
   y[3 * b] = v[0];
   y[3 * b + 1] = v[1];
   y[3 * b + 2] = v[2];
+}
+
+// Persistent launches over the SELL slices of the CG operator: the grid is
+// a multiple of 8 workgroups; XCD x (workgroups x, x + 8, ...) walks slices
+// [x ns / 8, (x + 1) ns / 8), its wavefronts striding by the number it
+// holds, one block row per lane.
+constexpr uint32_t kPersistGrid = 4096;
+struct SliceWalk {
+  int64_t s, end, stride;
+};
+__device__ __forceinline__ SliceWalk slice_walk(int64_t ns) {
+  const int64_t x = blockIdx.x & 7;
+  const int64_t per = gridDim.x >> 3;
+  constexpr int64_t kWaves = kBlock / 64;
+  return SliceWalk{x * ns / 8 + (blockIdx.x >> 3) * kWaves + (threadIdx.x >> 6), (x + 1) * ns / 8,
+                   per * kWaves};
+}
+
+template <class T, bool PAIRS>
+__global__ __launch_bounds__(kBlock) void k_spmv_b3_persist(Bsr3<T> A, const T *__restrict__ x,
+                                                            T *__restrict__ y) {
+  const int64_t lane = threadIdx.x & 63;
+  const SliceWalk w = slice_walk((A.nb + 63) >> 6);
+  for (int64_t s = w.s; s < w.end; s += w.stride) {
+    const int64_t pos = (s << 6) + lane;
+    if (pos < A.nb) {
+      T v[3];
+      bsr3_row_dot<T, T, PAIRS>(A, pos, x, v, false);
+      const int64_t b = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
+      y[3 * b] = v[0];
+      y[3 * b + 1] = v[1];
+      y[3 * b + 2] = v[2];
+    }
+  }
 }
 
 // res = rhs - A x, as rhs + (-1) * (A x) (src/Smoothers/MultigridSmoother.cc:367-371).
@@ -848,14 +905,14 @@ __global__ __launch_bounds__(kBlock) void k_cg_init_r(Sell<TC> A, const TC *__re
   if (rm) rm[row] = static_cast<TM>(v);
 }
 
-template <class TC, class TM>
+template <class TC, class TM, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_cg_init_r_b3(Bsr3<TC> A, const TC *__restrict__ b,
                                                          const TC *__restrict__ x,
                                                          TC *__restrict__ r, TM *__restrict__ rm) {
   const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   if (pos >= A.nb) return;
   TC v[3];
-  bsr3_row_dot(A, pos, x, v, false);
+  bsr3_row_dot<TC, TC, PAIRS>(A, pos, x, v, false);
   const int64_t bb = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
@@ -969,6 +1026,34 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3(Bsr3<TA> A, const TC 
   block_partial(v, partials);
 }
 
+// Persistent form of k_cg_spmv_pap_b3 for the CG operator (one partial per
+// workgroup, gridDim.x partials).
+template <class TC, bool PAIRS>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv_pap_b3_persist(Bsr3<TC> A,
+                                                                   const TC *__restrict__ p,
+                                                                   TC *__restrict__ q,
+                                                                   Ctl<TC> *ctl, TC *partials) {
+  if (ctl->stop) return;
+  const int64_t lane = threadIdx.x & 63;
+  const SliceWalk w = slice_walk((A.nb + 63) >> 6);
+  TC v = TC(0);
+  for (int64_t s = w.s; s < w.end; s += w.stride) {
+    const int64_t pos = (s << 6) + lane;
+    if (pos < A.nb) {
+      TC y[3];
+      bsr3_row_dot<TC, TC, PAIRS>(A, pos, p, y, false);
+      const int64_t b = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
+      q[3 * b] = y[0];
+      q[3 * b + 1] = y[1];
+      q[3 * b + 2] = y[2];
+      v += p[3 * b] * y[0];
+      v += p[3 * b + 1] * y[1];
+      v += p[3 * b + 2] * y[2];
+    }
+  }
+  block_partial(v, partials);
+}
+
 // x += alpha p; r = r + (-alpha) q; partials of r.r (k_cg_rr_final then
 // forms rel = rr / bb and the stop test, src/CGSolver.cc:218-230, 252-268).
 // Writes the V-cycle-type copy of r.
@@ -1040,7 +1125,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace(Sell<TC> A, const TC *__r
   block_partial(v, partials);
 }
 
-template <class TC, class TM>
+template <class TC, class TM, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_cg_replace_b3(Bsr3<TC> A, const TC *__restrict__ b,
                                                           const TC *__restrict__ x,
                                                           TC *__restrict__ r, TM *__restrict__ rm,
@@ -1050,7 +1135,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace_b3(Bsr3<TC> A, const TC *
   TC v = TC(0);
   if (pos < A.nb) {
     TC y[3];
-    bsr3_row_dot(A, pos, x, y, false);
+    bsr3_row_dot<TC, TC, PAIRS>(A, pos, x, y, false);
     const int64_t bb = A.row_map ? static_cast<int64_t>(A.row_map[pos]) : pos;
 #pragma unroll
     for (int l = 0; l < 3; ++l) {
@@ -1152,7 +1237,7 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(Sell<TC> A, const TC *
   block_partial(bb, partials + gridDim.x);
 }
 
-template <class TC>
+template <class TC, bool PAIRS = false>
 __global__ __launch_bounds__(kBlock) void k_true_residual_b3(Bsr3<TC> A, const TC *__restrict__ b,
                                                              const TC *__restrict__ x,
                                                              double *partials) {
@@ -1167,10 +1252,11 @@ __global__ __launch_bounds__(kBlock) void k_true_residual_b3(Bsr3<TC> A, const T
     for (int j = 0; j < w; ++j) {
       const int64_t slot = A.slice_ptr[s] + static_cast<int64_t>(j) * kSlice;
       const int64_t bc = A.bcol[slot + lane];
-      const TC *v = A.val + slot * 9 + lane;
+      TC v[9];
+      bsr3_block_values<PAIRS>(A.val, slot, lane, v);
       for (int l = 0; l < 3; ++l)
         for (int c = 0; c < 3; ++c)
-          y[l] += static_cast<double>(v[(3 * l + c) * kSlice]) * static_cast<double>(x[3 * bc + c]);
+          y[l] += static_cast<double>(v[3 * l + c]) * static_cast<double>(x[3 * bc + c]);
     }
     for (int l = 0; l < 3; ++l) {
       const double bi = static_cast<double>(b[3 * blk + l]);
