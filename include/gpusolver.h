@@ -232,6 +232,10 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *   "cg_level_order"  1: the 3x3-block CG operator's rows run in level-0
  *                     order through a row map (each row's sum unchanged;
  *                     the reductions group rows differently) (1)
+ *   "cg_pair_layout"  1: the FP64 3x3-block CG operator stores its values
+ *                     as planes of value pairs (16-byte loads) and its SpMV
+ *                     runs as a persistent launch over the slices (row sums
+ *                     unchanged; p.Ap grouped by workgroup) (1)
  *   "zero_start_lower" k: Gauss-Seidel levels below index k keep a second
  *                     operator with only the entries of earlier colours; the
  *                     forward colours of a sweep from x = 0 read it (the

@@ -62,14 +62,19 @@ def test_spmv_bitexact(cfg1, prec):
     np.testing.assert_array_equal(y, yo.astype(np.float64))
 
 
+@pytest.mark.parametrize("pairs", [0, 1])
 @pytest.mark.parametrize("lo", [0, 1])
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_SINGLE])
-def test_spmv_block_bitexact(elast, prec, lo):
+def test_spmv_block_bitexact(elast, prec, lo, pairs):
     """The 3x3-block CG operator (rows in colour order, or in level-0 order
-    through a row map): every row sum bit for bit the oracle's CSR row sum."""
+    through a row map; FP64 values as nine planes or as planes of value pairs
+    read by the persistent kernel): every row sum bit for bit the oracle's
+    CSR row sum."""
     E, rhs, H = elast
-    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, cg_block3=1, cg_level_order=lo)
+    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, cg_block3=1, cg_level_order=lo,
+                  cg_pair_layout=pairs)
     s.setup(H)
+    assert s.describe()["cg_pair_layout"] == (pairs == 1 and prec == gs.PREC_DOUBLE)
     x = gs.uniform_pm1(8, E.rows)
     y = s.spmv(x)
     if prec == gs.PREC_SINGLE:
@@ -314,10 +319,12 @@ def test_direct_level_tiled_product(n):
     assert np.linalg.norm(zm - ref) <= 1e-6 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("pairs", [0, 1])
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
-def test_pcg_parity_elasticity_block_gs(elast, prec):
+def test_pcg_parity_elasticity_block_gs(elast, prec, pairs):
     E, rhs, H = elast
-    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2)
+    s = gs.Solver(prec, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2, cg_block3=1,
+                  cg_pair_layout=pairs)
     s.setup(H)
     hv, _ = hier_view(H)
     x, xo, info, true = _pcg_parity(s, hv, E, rhs, prec, kinds=gs.BLOCK_GS, omegas=0.7)
