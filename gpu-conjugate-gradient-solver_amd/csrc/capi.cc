@@ -27,11 +27,20 @@ struct gs_solver_s {
   std::unique_ptr<gse::EngineBase> eng;
   gse::SolveResult last;
   // Learned-weight guard: the plain solver restarts the solve when the
-  // learned one breaks down (p.Ap <= 0 or r.z <= 0) or lags the plain
-  // rate (more than lag * plain iterations).
+  // learned one breaks down (p.Ap <= 0 or r.z <= 0), produces a non-finite
+  // residual, or lags the plain rate (checked at every host check of the
+  // stop word, and at most lag * the plain iteration count in total).
   gs_solver_s *fallback = nullptr;
   double lag = 1.5;
   bool fell_back = false;
+  // Why the last solve fell back (0 none, 1 lagging the plain rate, 2 CG
+  // breakdown, 3 non-finite residual, 4 budget spent) and how many PCG
+  // iterations the guarded attempt ran.
+  int fallback_reason = 0;
+  int64_t first_iterations = 0;
+  // Convergence rate of this solver's last converged solve: mean log
+  // reduction of relres^2 per PCG iteration (0: none yet).
+  double log_rate = 0.0;
 };
 
 namespace gsc {
@@ -528,6 +537,11 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "cg_level_order") o.cg_level_order = v != 0;
     else if (k == "cg_vcycle_operator") o.cg_vcycle_operator = v != 0;
     else if (k == "cg_pair_layout") o.cg_pair_layout = v != 0;
+    else if (k == "persist_grid") {
+      if (v < 8 || v % 8 != 0 || v > (int64_t(1) << 20))
+        throw std::runtime_error("persist_grid must be a positive multiple of 8.");
+      o.persist_grid = static_cast<int>(v);
+    }
     else if (k == "cg_replace_every") {
       if (v < 1) throw std::runtime_error("cg_replace_every must be at least 1.");
       o.cg_replace_every = static_cast<int>(v);
@@ -599,43 +613,86 @@ int gs_solver_setup(gs_solver_h s, gs_hier_h h, const int32_t *const *o2n) {
       if (o2n && o2n[k]) perms.push_back(perm_from_o2n(n, o2n[k]));
       else perms.push_back(gsx::Perm::identity(n));
     }
+    s->eng.reset();
+    s->last = gse::SolveResult{};
+    s->log_rate = 0.0;
     s->eng = gse::make_engine(s->precision, s->opt);
     s->eng->setup(h->H, perms);
   });
 }
 
-int gs_solver_solve(gs_solver_h s, const double *b, double *x, double tol, int64_t maxit,
-                    int64_t *iters, double *relres2, double *true_relres, double *solve_ms) {
+namespace {
+// Rate of a converged solve from its relres^2 history (first entry: the
+// initial residual).
+double solve_log_rate(const gse::EngineBase &e, const gse::SolveResult &r) {
+  const auto &h = e.history();
+  if (!r.converged || r.loop_iterations <= 0 || h.size() < 2 || !(h.front() > 0.0) ||
+      !(h.back() > 0.0))
+    return 0.0;
+  return std::log(h.back() / h.front()) / static_cast<double>(r.loop_iterations);
+}
+}  // namespace
+
+int gs_solver_solve(gs_solver_h s, int64_t b_len, const double *b, int64_t x_len, double *x,
+                    double tol, int64_t maxit, int64_t *iters, double *relres2,
+                    double *true_relres, double *solve_ms) {
   return guard([&] {
     need(s, "solver");
     if (!s->eng) throw std::runtime_error("CGSolver: solver is not set up.");
+    // src/CGSolver.cc:90-98.
+    if (b_len != s->eng->size())
+      throw std::runtime_error("CGSolver::solve: RHS size does not match matrix.");
+    if (x_len != s->eng->size())
+      throw std::runtime_error("CGSolver::solve: initial guess size does not match matrix.");
     need(b, "b");
     need(x, "x");
     if (maxit < 0) throw std::runtime_error("maxIterations must be non-negative.");
     s->fell_back = false;
-    if (s->fallback && s->fallback->eng) {
-      // Budget from the plain solver's last iteration count.
-      int64_t budget = maxit;
-      const int64_t ref = s->fallback->last.loop_iterations;
-      if (ref > 0)
-        budget = std::min<int64_t>(maxit, static_cast<int64_t>(std::ceil(s->lag * ref)));
+    s->fallback_reason = 0;
+    s->first_iterations = 0;
+    s->last = gse::SolveResult{};
+    if (s->fallback) {
+      gs_solver_s *pl = s->fallback;
+      if (!pl->eng) throw std::runtime_error("fallback solver is not set up.");
+      if (pl->eng->size() != s->eng->size())
+        throw std::runtime_error("fallback solver was set up for another matrix size.");
+      if (pl->last.loop_iterations <= 0 || !(pl->log_rate < 0.0))
+        throw std::runtime_error(
+            "fallback solver has no converged solve to take the plain rate from; solve with it "
+            "first.");
+      // Budget from the plain solver's last iteration count, and a rate
+      // check at every host check of the stop word.
+      const int64_t budget = std::min<int64_t>(
+          maxit, static_cast<int64_t>(std::ceil(s->lag * pl->last.loop_iterations)));
+      gse::LagGuard lg;
+      lg.log_rate = pl->log_rate;
+      lg.lag = s->lag;
       const std::vector<double> x0(x, x + s->eng->size());
-      bool bad = false;
+      int reason = 0;
+      double first_ms = 0.0;
       try {
-        s->last = s->eng->solve(b, x, tol, budget);
-        bad = !s->last.zero_rhs && (!s->last.converged || s->last.breakdown != 0);
-      } catch (const std::runtime_error &) {
-        bad = true;  // non-finite residual in the learned solve
+        s->last = s->eng->solve(b, x, tol, budget, &lg);
+        first_ms = s->last.solve_ms;
+        s->first_iterations = s->last.loop_iterations;
+        if (!s->last.zero_rhs) {
+          if (s->last.breakdown != 0) reason = 2;
+          else if (s->last.lagging) reason = 1;
+          else if (!s->last.converged) reason = 4;
+        }
+      } catch (const gse::NonFiniteResidual &) {
+        reason = 3;  // non-finite residual in the learned solve
       }
-      if (bad) {
+      if (reason != 0) {
+        s->fallback_reason = reason;
         std::copy(x0.begin(), x0.end(), x);
-        const double first_ms = s->last.solve_ms;
-        s->last = s->fallback->eng->solve(b, x, tol, maxit);
+        s->last = pl->eng->solve(b, x, tol, maxit);
         s->last.solve_ms += first_ms;
         s->fell_back = true;
       }
     } else {
       s->last = s->eng->solve(b, x, tol, maxit);
+      const double r = solve_log_rate(*s->eng, s->last);
+      if (r < 0.0) s->log_rate = r;
     }
     if (iters) *iters = s->last.iterations;
     if (relres2) *relres2 = s->last.relres2;
@@ -655,6 +712,10 @@ int gs_solver_result(gs_solver_h s, const char *key, double *value) {
     else if (k == "wall_ms") *value = r.wall_ms;
     else if (k == "zero_rhs") *value = r.zero_rhs ? 1.0 : 0.0;
     else if (k == "fell_back") *value = s->fell_back ? 1.0 : 0.0;
+    else if (k == "lagging") *value = r.lagging ? 1.0 : 0.0;
+    else if (k == "fallback_reason") *value = s->fallback_reason;
+    else if (k == "first_iterations") *value = static_cast<double>(s->first_iterations);
+    else if (k == "log_rate") *value = s->log_rate;
     else throw std::runtime_error("unknown result key: " + k);
   });
 }
@@ -670,19 +731,36 @@ int gs_solver_history(gs_solver_h s, double *out, int64_t cap, int64_t *len) {
   });
 }
 
-int gs_solver_vcycle(gs_solver_h s, const double *r, double *z) {
+int gs_solver_vcycle(gs_solver_h s, int64_t n, const double *r, double *z) {
   return guard([&] {
     need(s, "solver");
     if (!s->eng) throw std::runtime_error("solver is not set up.");
+    if (n != s->eng->size())
+      throw std::runtime_error("MultigridSmoother: vector size does not match the finest level.");
+    need(r, "r");
+    need(z, "z");
     s->eng->vcycle_level_order(r, z);
   });
 }
 
-int gs_solver_spmv(gs_solver_h s, const double *x, double *y) {
+int gs_solver_spmv(gs_solver_h s, int64_t n, const double *x, double *y) {
   return guard([&] {
     need(s, "solver");
     if (!s->eng) throw std::runtime_error("solver is not set up.");
+    // src/CPU/CPUBackend.cc:220-222.
+    if (n != s->eng->size()) throw std::runtime_error("CPUBackend::encodeSpmv: size mismatch.");
+    need(x, "x");
+    need(y, "y");
     s->eng->spmv_level_order(x, y);
+  });
+}
+
+int gs_solver_size(gs_solver_h s, int64_t *n) {
+  return guard([&] {
+    need(s, "solver");
+    need(n, "n");
+    if (!s->eng) throw std::runtime_error("solver is not set up.");
+    *n = s->eng->size();
   });
 }
 

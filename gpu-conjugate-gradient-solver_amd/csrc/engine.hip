@@ -146,7 +146,8 @@ class Engine final : public EngineBase {
   }
 
   void setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm> &perms) override;
-  SolveResult solve(const double *b, double *x, double tol, int64_t maxit) override;
+  SolveResult solve(const double *b, double *x, double tol, int64_t maxit,
+                    const LagGuard *guard) override;
   void vcycle_level_order(const double *r, double *z) override;
   void spmv_level_order(const double *x, double *y) override;
   double time_kernel(int which, int reps, double *bytes) override;
@@ -155,21 +156,26 @@ class Engine final : public EngineBase {
   const std::vector<double> &history() const override { return hist_host_; }
   void level_info(std::vector<LevelInfo> &out) const override;
   void set_level_weights(const std::vector<double> &omega, const std::vector<int> &sweeps) override {
-    for (int k = 0; k < nlev_ && k < static_cast<int>(L_.size()); ++k) {
-      Level<TM> &lv = L_[k];
-      if (k < static_cast<int>(omega.size()) && omega[k] > 0.0) {
-        if ((lv.kind == kSGS || lv.kind == kBlockGS) && !(omega[k] < 2.0))
-          throw std::runtime_error("Gauss-Seidel smoother: omega must be in (0, 2).");
-        lv.omega = static_cast<TM>(omega[k]);
-      }
-      if (k < static_cast<int>(sweeps.size()) && sweeps[k] >= 0) {
-        if (sweeps[k] == 0 && lv.kind == kSGS && !lv.direct && k + 1 < nlev_)
-          throw std::runtime_error("Preconditioner requires at least one smoothing step.");
-        lv.pre = lv.post = sweeps[k];
-      }
+    const int nl = std::min<int>(nlev_, static_cast<int>(L_.size()));
+    // Validate every entry first, so a rejected entry leaves the weights
+    // (and the captured graph) as they were.
+    for (int k = 0; k < nl; ++k) {
+      const Level<TM> &lv = L_[k];
+      if (k < static_cast<int>(omega.size()) && omega[k] > 0.0 &&
+          (lv.kind == kSGS || lv.kind == kBlockGS) && !(omega[k] < 2.0))
+        throw std::runtime_error("Gauss-Seidel smoother: omega must be in (0, 2).");
+      if (k < static_cast<int>(sweeps.size()) && sweeps[k] == 0 && lv.kind == kSGS &&
+          !lv.direct && k + 1 < nlev_)
+        throw std::runtime_error("Preconditioner requires at least one smoothing step.");
     }
     if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    for (int k = 0; k < nl; ++k) {
+      Level<TM> &lv = L_[k];
+      if (k < static_cast<int>(omega.size()) && omega[k] > 0.0)
+        lv.omega = static_cast<TM>(omega[k]);
+      if (k < static_cast<int>(sweeps.size()) && sweeps[k] >= 0) lv.pre = lv.post = sweeps[k];
+    }
   }
 
  private:
@@ -556,7 +562,10 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
             opt_.precond == kPrecMultigrid && !L_[0].direct && L_[0].A.G == 0;
   // Partials: one per workgroup of the widest reduction (three segments of
   // the n-thread grid, or one per workgroup of a persistent launch).
-  partials_.alloc((std::max<size_t>(3 * static_cast<size_t>(grid_for(n_)), gsk::kPersistGrid) + 64) *
+  if (opt_.persist_grid < 8 || opt_.persist_grid % 8 != 0)
+    throw std::runtime_error("persist_grid must be a positive multiple of 8.");
+  partials_.alloc((std::max<size_t>(3 * static_cast<size_t>(grid_for(n_)),
+                                    static_cast<size_t>(opt_.persist_grid)) + 64) *
                   sizeof(double));
   ctl_.alloc(sizeof(gsk::Ctl<TC>));
   tr_.alloc(2 * sizeof(double));
@@ -824,7 +833,7 @@ void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   TC *parts = partials_.template as<TC>();
   const DOp<TM> &Av = L_[0].A;
   const bool vb3 = cg_vop_ && Av.b3;
-  const uint32_t g = (!cg_vop_ && cg_pairs_) ? gsk::kPersistGrid
+  const uint32_t g = (!cg_vop_ && cg_pairs_) ? static_cast<uint32_t>(opt_.persist_grid)
                                               : grid_for((cg_vop_ ? vb3 : Acg_.b3) ? n_ / 3 : n_);
   if (!cg_vop_ && cg_pairs_)
     gsk::k_cg_spmv_pap_b3_persist<TC, true><<<g, kBlock, 0, stream_>>>(
@@ -881,7 +890,8 @@ void Engine<TC, TM>::capture_graph() {
 }
 
 template <class TC, class TM>
-SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_t maxit) {
+SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_t maxit,
+                                  const LagGuard *guard) {
   if (!setup_done_) throw std::runtime_error("CGSolver: solver is not set up.");
   SolveResult res;
   // src/CGSolver.cc:113-118: a right-hand side below 1e-12 in norm returns
@@ -930,6 +940,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   gsk::k_cg_init_final<TC><<<1, kBlock, 0, stream_>>>(ctl, partials_.template as<TC>(), grid_for(n_));
   const int32_t *stop = &ctl->stop;
   int64_t launched = 0;
+  double log_rel0 = 0.0;
   if (maxit > 0) {
     if (opt_.graph_iters > 0) capture_graph();
     const int per = opt_.graph_iters > 0 ? opt_.graph_iters : 1;
@@ -938,6 +949,19 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
                             stream_));
       GS_HIP(hipStreamSynchronize(stream_));
       if (host_ctl_->stop != 0 || launched >= maxit) break;
+      if (guard) {
+        // The first check sees the initial residual; later ones compare the
+        // reduction so far with the guard's rate stretched by its lag.
+        const double rel = static_cast<double>(host_ctl_->rel);
+        const int64_t it = host_ctl_->iter;
+        if (launched == 0) {
+          log_rel0 = std::log(rel);
+        } else if (it >= guard->min_iter && rel > 0.0 &&
+                   std::log(rel) - log_rel0 > guard->log_rate * static_cast<double>(it) / guard->lag) {
+          res.lagging = true;
+          break;
+        }
+      }
       if (graph_exec_) GS_HIP(hipGraphLaunch(graph_exec_, stream_));
       else encode_iteration(stop);
       launched += per;
@@ -956,8 +980,8 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   const gsk::Ctl<TC> &hc = *host_ctl_;
   if (hc.stop == 3) {
     if (hc.iter == 0)
-      throw std::runtime_error("CGSolver: initial residual is NaN/Inf - check matrix and RHS.");
-    throw std::runtime_error("CGSolver: solver diverged (NaN/Inf residual).");
+      throw NonFiniteResidual("CGSolver: initial residual is NaN/Inf - check matrix and RHS.");
+    throw NonFiniteResidual("CGSolver: solver diverged (NaN/Inf residual).");
   }
   res.iterations = (maxit == 0) ? 0 : hc.iters_out;
   res.loop_iterations = hc.iter;
@@ -1015,7 +1039,7 @@ void Engine<TC, TM>::spmv_level_order(const double *x, double *y) {
   for (int64_t i = 0; i < n_; ++i) xi[i] = static_cast<TC>(x[int2lvl0_[i]]);
   GS_HIP(hipMemcpy(p_.get(), xi.data(), n_ * sizeof(TC), hipMemcpyHostToDevice));
   if (cg_pairs_)
-    gsk::k_spmv_b3_persist<TC, true><<<gsk::kPersistGrid, kBlock, 0, stream_>>>(
+    gsk::k_spmv_b3_persist<TC, true><<<static_cast<uint32_t>(opt_.persist_grid), kBlock, 0, stream_>>>(
         Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>());
   else if (Acg_.b3)
     gsk::k_spmv_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(),
@@ -1043,7 +1067,7 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
   auto body = [&]() {
     if (which == 0) {
       if (cg_pairs_)
-        gsk::k_spmv_b3_persist<TC, true><<<gsk::kPersistGrid, kBlock, 0, stream_>>>(
+        gsk::k_spmv_b3_persist<TC, true><<<static_cast<uint32_t>(opt_.persist_grid), kBlock, 0, stream_>>>(
             Acg_.b.view(), p_.template as<TC>(), q_.template as<TC>());
       else if (Acg_.b3)
         gsk::k_spmv_b3<TC><<<grid_for(n_ / 3), kBlock, 0, stream_>>>(Acg_.b.view(), p_.template as<TC>(),

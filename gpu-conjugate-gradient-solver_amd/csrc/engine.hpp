@@ -3,6 +3,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -70,9 +71,31 @@ struct Options {
   // The FP64 3x3-block CG operator's values as planes of value pairs
   // (16-byte loads) and its SpMV as a persistent launch walking slices.
   bool cg_pair_layout = true;
+  // Workgroups of the persistent CG SpMV (a multiple of 8: one share per
+  // XCD); each wavefront walks one SELL slice per pass.
+  int persist_grid = static_cast<int>(gsk::kPersistGrid);
   int cg_replace_every = 8;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
+};
+
+// A non-finite residual (NaN / Inf) in the PCG loop: CGSolver::solve's
+// "initial residual is NaN/Inf" / "solver diverged" errors
+// (src/CGSolver.cc:182,255), as a type of its own so that a caller
+// can tell it from a device or setup failure.
+struct NonFiniteResidual : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Convergence-rate guard of a solve: at each host check of the stop word
+// after min_iter iterations, the solve ends (result "lagging") when its
+// residual reduction log(rel_k / rel_0) is above log_rate * k / lag, i.e.
+// it needs more than lag times the iterations of a solver whose relres^2
+// falls by exp(log_rate) per iteration.
+struct LagGuard {
+  double log_rate = 0.0;  // < 0: mean log relres^2 reduction per iteration
+  double lag = 1.5;
+  int64_t min_iter = 8;
 };
 
 struct SolveResult {
@@ -85,6 +108,7 @@ struct SolveResult {
   bool converged = false;
   bool zero_rhs = false;
   int breakdown = 0;            // 1: p.Ap <= 0 seen, 2: r.z <= 0 seen
+  bool lagging = false;         // ended by a LagGuard
 };
 
 struct LevelInfo {
@@ -103,7 +127,9 @@ class EngineBase {
  public:
   virtual ~EngineBase() = default;
   virtual void setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm> &perms) = 0;
-  virtual SolveResult solve(const double *b, double *x, double tol, int64_t maxit) = 0;
+  // guard (may be null): ends the solve early when it lags a given rate.
+  virtual SolveResult solve(const double *b, double *x, double tol, int64_t maxit,
+                            const LagGuard *guard = nullptr) = 0;
   virtual void vcycle_level_order(const double *r, double *z) = 0;
   virtual void spmv_level_order(const double *x, double *y) = 0;
   virtual double time_kernel(int which, int reps, double *bytes) = 0;
@@ -112,7 +138,8 @@ class EngineBase {
   virtual const std::vector<double> &history() const = 0;
   virtual void level_info(std::vector<LevelInfo> &out) const = 0;
   // Per-level damping / pre = post sweeps after setup (entries <= 0 keep
-  // the current value); the captured graph is rebuilt on the next solve.
+  // the current value); every entry is validated before any is applied, and
+  // the captured graph is rebuilt on the next solve.
   virtual void set_level_weights(const std::vector<double> &omega,
                                  const std::vector<int> &sweeps) = 0;
 };

@@ -91,13 +91,14 @@ _SIGS = {
     "gs_solver_set_real": (C.c_int, [_vp, C.c_char_p, C.c_double]),
     "gs_solver_set_level_weights": (C.c_int, [_vp, _i32, _vp, _vp]),
     "gs_solver_setup": (C.c_int, [_vp, _vp, _vp]),
-    "gs_solver_solve": (C.c_int, [_vp, _f64p, _f64p, C.c_double, _i64, C.POINTER(_i64),
+    "gs_solver_solve": (C.c_int, [_vp, _i64, _f64p, _i64, _f64p, C.c_double, _i64, C.POINTER(_i64),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]),
     "gs_solver_result": (C.c_int, [_vp, C.c_char_p, C.POINTER(C.c_double)]),
     "gs_solver_history": (C.c_int, [_vp, _f64p, _i64, C.POINTER(_i64)]),
-    "gs_solver_vcycle": (C.c_int, [_vp, _f64p, _f64p]),
-    "gs_solver_spmv": (C.c_int, [_vp, _f64p, _f64p]),
+    "gs_solver_vcycle": (C.c_int, [_vp, _i64, _f64p, _f64p]),
+    "gs_solver_spmv": (C.c_int, [_vp, _i64, _f64p, _f64p]),
+    "gs_solver_size": (C.c_int, [_vp, C.POINTER(_i64)]),
     "gs_solver_time_kernel": (C.c_int, [_vp, _i32, _i32, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]),
     "gs_solver_describe": (C.c_int, [_vp, C.c_char_p, _i64]),
@@ -652,12 +653,12 @@ class Solver:
 
     def solve(self, b: np.ndarray, x0: np.ndarray | None = None, tol: float = 1e-8,
               maxit: int = 1000):
-        b = np.ascontiguousarray(b, np.float64)
-        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True)
+        b = np.ascontiguousarray(b, np.float64).reshape(-1)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, np.float64, copy=True).reshape(-1)
         it = _i64()
         rr, tr, ms = C.c_double(), C.c_double(), C.c_double()
-        check(lib.gs_solver_solve(self.h, b, x, tol, maxit, C.byref(it), C.byref(rr), C.byref(tr),
-                                  C.byref(ms)))
+        check(lib.gs_solver_solve(self.h, b.size, b, x.size, x, tol, maxit, C.byref(it),
+                                  C.byref(rr), C.byref(tr), C.byref(ms)))
         res = {}
         for k in ("converged", "loop_iterations", "breakdown", "wall_ms"):
             v = C.c_double()
@@ -683,16 +684,26 @@ class Solver:
         return out[:n.value]
 
     def vcycle(self, r: np.ndarray) -> np.ndarray:
-        r = np.ascontiguousarray(r, np.float64)
+        r = np.ascontiguousarray(r, np.float64).reshape(-1)
         z = np.empty_like(r)
-        check(lib.gs_solver_vcycle(self.h, r, z))
+        check(lib.gs_solver_vcycle(self.h, r.size, r, z))
         return z
 
     def spmv(self, x: np.ndarray) -> np.ndarray:
-        x = np.ascontiguousarray(x, np.float64)
+        x = np.ascontiguousarray(x, np.float64).reshape(-1)
         y = np.empty_like(x)
-        check(lib.gs_solver_spmv(self.h, x, y))
+        check(lib.gs_solver_spmv(self.h, x.size, x, y))
         return y
+
+    def size(self) -> int:
+        n = _i64()
+        check(lib.gs_solver_size(self.h, C.byref(n)))
+        return n.value
+
+    def result(self, key: str) -> float:
+        v = C.c_double()
+        check(lib.gs_solver_result(self.h, key.encode(), C.byref(v)))
+        return v.value
 
     def time_kernel(self, which: int, reps: int = 20):
         ms, by = C.c_double(), C.c_double()

@@ -242,6 +242,10 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     entries it leaves out multiply exact zeros; same
  *                     results); 0: none (1: level 0, where the sweeps are
  *                     bandwidth-bound)
+ *   "persist_grid"    workgroups of the persistent FP64 3x3-block CG SpMV
+ *                     (a positive multiple of 8; 4096): one share of the
+ *                     slices per XCD, each wavefront walking slices at a
+ *                     stride of the waves in its share
  *   "cg_vcycle_operator" 1 (GS_PREC_MIXED): the CG SpMV reads level 0's
  *                     FP32 V-cycle operator, products and sums in FP64; the
  *                     FP64 operator recomputes r = b - A x every
@@ -269,26 +273,46 @@ int gs_solver_set_level_weights(gs_solver_h s, int32_t nlev, const double *omega
  * level 0's permutation (CGSolver ctor, src/CGSolver.cc:44-70). */
 int gs_solver_setup(gs_solver_h s, gs_hier_h h, const int32_t *const *level_old_to_new);
 /* Learned-weight guard (no counterpart in the reference): when set, a
- * solve of s that breaks down (p.Ap <= 0 or r.z <= 0), fails to converge
- * within lag times the plain solver's last iteration count, or produces a
- * non-finite residual is restarted from the same initial guess with
- * `plain`; "fell_back" in gs_solver_result reports it. */
+ * solve of s that breaks down (p.Ap <= 0 or r.z <= 0), produces a
+ * non-finite residual, or lags the plain rate is restarted from the same
+ * initial guess with `plain`; "fell_back" in gs_solver_result reports it.
+ * The plain rate is the mean relres^2 reduction per iteration of plain's
+ * last converged solve: at every host check of the stop word (each
+ * "graph_iters" iterations) the learned solve must have reduced its
+ * residual as much as plain does in lag times fewer iterations, and it may
+ * take at most lag times plain's last iteration count. A solve with a
+ * fallback fails when plain is not set up, was set up for another size, or
+ * has not yet converged once. */
 int gs_solver_set_fallback(gs_solver_h s, gs_solver_h plain, double lag);
 /* CGSolver::solve (src/CGSolver.cc:87-306): b and x (initial guess in,
- * solution out) in the caller's ordering. iters follows the reference's
+ * solution out) in the caller's ordering; b_len and x_len are their
+ * lengths, checked against the matrix as src/CGSolver.cc:90-98 checks the
+ * Eigen vectors' sizes (same messages). iters follows the reference's
  * convention (getNbOfIterations, src/CGSolver.cc:257-268,308). */
-int gs_solver_solve(gs_solver_h s, const double *b, double *x, double tol, int64_t maxit,
-                    int64_t *iters, double *relres2, double *true_relres, double *solve_ms);
+int gs_solver_solve(gs_solver_h s, int64_t b_len, const double *b, int64_t x_len, double *x,
+                    double tol, int64_t maxit, int64_t *iters, double *relres2,
+                    double *true_relres, double *solve_ms);
+/* Rows of the matrix the solver was set up for (CGSolver's matrix_.rows(),
+ * include/GPUSolver/CGSolver.h:19-86). */
+int gs_solver_size(gs_solver_h s, int64_t *n);
 /* Result fields of the last solve: "converged", "loop_iterations",
- * "breakdown", "wall_ms", "zero_rhs", "fell_back". */
+ * "breakdown", "wall_ms", "zero_rhs", "fell_back", "lagging" (ended by the
+ * fallback's rate check), "fallback_reason" (0 none, 1 lagging the plain
+ * rate, 2 CG breakdown, 3 non-finite residual, 4 iteration budget spent),
+ * "first_iterations" (PCG iterations of the guarded attempt), and
+ * "log_rate" (this solver's rate as a fallback, 0 before its first
+ * converged solve). */
 int gs_solver_result(gs_solver_h s, const char *key, double *value);
 int gs_solver_history(gs_solver_h s, double *relres2, int64_t cap, int64_t *len);
 /* One preconditioner application z = M r (Preconditioner::apply,
- * include/GPUSolver/Preconditioner.h:13-20), vectors in level-0 ordering. */
-int gs_solver_vcycle(gs_solver_h s, const double *r, double *z);
+ * include/GPUSolver/Preconditioner.h:13-20), vectors of length n (the
+ * finest level's size, else an error) in level-0 ordering. */
+int gs_solver_vcycle(gs_solver_h s, int64_t n, const double *r, double *z);
 /* y = A x with the CG operator (Backend::encodeSpmv, include/GPUSolver/
- * Backend.h:68-69), vectors in level-0 ordering. */
-int gs_solver_spmv(gs_solver_h s, const double *x, double *y);
+ * Backend.h:68-69), vectors of length n in level-0 ordering; a length other
+ * than the matrix size fails with encodeSpmv's "size mismatch"
+ * (src/CPU/CPUBackend.cc:220-222). */
+int gs_solver_spmv(gs_solver_h s, int64_t n, const double *x, double *y);
 /* Average ms per launch of a hot kernel timed with HIP events on the
  * solver's stream: 0 CG SpMV, 1 V-cycle, 2 PCG iteration, 3 / 4 level-0 /
  * level-1 SGS or block-GS sweep, 5 level-0 restriction, 6 level-0

@@ -331,7 +331,9 @@ void vcycle(MG<T> &mg, size_t k, const T *rhs, T *x) {
   const int64_t n = lv.A.rows;
   if (k + 1 == mg.L.size()) {
     if (mg.direct_coarse) {
-      // FP64 dense solve: x = A_c^-1 rhs (x is zero on entry).
+      // FP64 dense solve: x = A_c^-1 rhs (x is zero on entry); each row's
+      // sum in column order, rows in parallel.
+#pragma omp parallel for schedule(static) if (n > 512)
       for (int64_t i = 0; i < n; ++i) {
         double s = 0.0;
         for (int64_t j = 0; j < n; ++j) s += mg.coarse_inv[i * n + j] * static_cast<double>(rhs[j]);
@@ -750,7 +752,8 @@ static CsrT<T> csr_cast(const CsrT<double> &A) {
 template <class TM>
 static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const double *omegas,
                      const int32_t *pres, const int32_t *posts, int32_t coarse_steps,
-                     int32_t direct_coarse, int32_t vcycles, int32_t ref_alias) {
+                     int32_t direct_coarse, int32_t vcycles, int32_t ref_alias,
+                     const double *coarse_inv) {
   const int nl = h.nlev;
   mg.L.resize(nl);
   mg.reference_aliasing = ref_alias != 0;
@@ -835,7 +838,11 @@ static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const d
       mg.P.push_back(csr_cast<TM>(Pd));
       mg.R.push_back(csr_cast<TM>(Rd));
     }
-    if (k == nl - 1 && mg.direct_coarse) {
+    if (k == nl - 1 && mg.direct_coarse && coarse_inv) {
+      // A caller-supplied FP64 inverse of the coarsest matrix (computed by
+      // LAPACK for levels too large for the scalar Cholesky below).
+      mg.coarse_inv.assign(coarse_inv, coarse_inv + n * n);
+    } else if (k == nl - 1 && mg.direct_coarse) {
       mg.coarse_inv.assign(n * n, 0.0);
       for (int64_t r = 0; r < n; ++r)
         for (int32_t e = Ad.ptr[r]; e < Ad.ptr[r + 1]; ++e) mg.coarse_inv[r * n + Ad.col[e]] += Ad.val[e];
@@ -848,13 +855,15 @@ static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const d
 template <class TC, class TM>
 static void pcg(const OrcHier &h, int prec, const int32_t *kinds, const double *omegas,
                 const int32_t *pres, const int32_t *posts, int32_t coarse_steps,
-                int32_t direct_coarse, int32_t vcycles, int32_t ref_alias, const double *b_in,
+                int32_t direct_coarse, int32_t vcycles, int32_t ref_alias,
+                const double *coarse_inv, const double *b_in,
                 double *x_io, double tol, int64_t maxit, int64_t *iters, double *relres2,
                 double *history, int64_t *hist_len) {
   const int64_t n = h.n[0];
   MG<TM> mg;
   if (prec == 1)
-    build_mg(mg, h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias);
+    build_mg(mg, h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias,
+             coarse_inv);
   // CG matrix: level-0 input with the order-keeping permutation.
   std::vector<int32_t> ident;
   const int32_t *o2n = nullptr, *n2o = nullptr;
@@ -968,19 +977,22 @@ static void pcg(const OrcHier &h, int prec, const int32_t *kinds, const double *
 extern "C" int orc_pcg(const OrcHier *h, int32_t prec, int32_t cg_double, int32_t mg_double,
             const int32_t *kinds, const double *omegas, const int32_t *pres,
             const int32_t *posts, int32_t coarse_steps, int32_t direct_coarse,
-            int32_t vcycles, int32_t ref_alias, const double *b, double *x, double tol,
-            int64_t maxit, int64_t *iters, double *relres2, double *history,
-            int64_t *hist_len) {
+            int32_t vcycles, int32_t ref_alias, const double *coarse_inv, const double *b,
+            double *x, double tol, int64_t maxit, int64_t *iters, double *relres2,
+            double *history, int64_t *hist_len) {
   ORC_TRY({
     if (cg_double && mg_double)
       pcg<double, double>(*h, prec, kinds, omegas, pres, posts, coarse_steps, direct_coarse,
-                          vcycles, ref_alias, b, x, tol, maxit, iters, relres2, history, hist_len);
+                          vcycles, ref_alias, coarse_inv, b, x, tol, maxit, iters, relres2, history,
+                          hist_len);
     else if (cg_double && !mg_double)
       pcg<double, float>(*h, prec, kinds, omegas, pres, posts, coarse_steps, direct_coarse,
-                         vcycles, ref_alias, b, x, tol, maxit, iters, relres2, history, hist_len);
+                         vcycles, ref_alias, coarse_inv, b, x, tol, maxit, iters, relres2, history,
+                          hist_len);
     else if (!cg_double && !mg_double)
       pcg<float, float>(*h, prec, kinds, omegas, pres, posts, coarse_steps, direct_coarse,
-                        vcycles, ref_alias, b, x, tol, maxit, iters, relres2, history, hist_len);
+                        vcycles, ref_alias, coarse_inv, b, x, tol, maxit, iters, relres2, history,
+                          hist_len);
     else
       throw std::runtime_error("orc_pcg: FP32 CG with an FP64 V-cycle is not a supported mode.");
   });
@@ -991,18 +1003,20 @@ extern "C" int orc_pcg(const OrcHier *h, int32_t prec, int32_t cg_double, int32_
 extern "C" int orc_vcycle(const OrcHier *h, int32_t mg_double, const int32_t *kinds,
                const double *omegas, const int32_t *pres, const int32_t *posts,
                int32_t coarse_steps, int32_t direct_coarse, int32_t vcycles,
-               int32_t ref_alias, const double *r, double *z) {
+               int32_t ref_alias, const double *coarse_inv, const double *r, double *z) {
   ORC_TRY({
     const int64_t n = h->n[0];
     if (mg_double) {
       MG<double> mg;
-      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias);
+      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias,
+               coarse_inv);
       std::vector<double> zz(n, 0.0);
       for (int v = 0; v < vcycles; ++v) vcycle(mg, 0, r, zz.data());
       std::memcpy(z, zz.data(), n * sizeof(double));
     } else {
       MG<float> mg;
-      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias);
+      build_mg(mg, *h, kinds, omegas, pres, posts, coarse_steps, direct_coarse, vcycles, ref_alias,
+               coarse_inv);
       std::vector<float> rr(n), zz(n, 0.0f);
       for (int64_t i = 0; i < n; ++i) rr[i] = static_cast<float>(r[i]);
       for (int v = 0; v < vcycles; ++v) vcycle(mg, 0, rr.data(), zz.data());

@@ -67,11 +67,12 @@ def _lib() -> C.CDLL:
                                          _i32p, _f64p, C.POINTER(_i64)]),
         "orc_block_graph": (C.c_int, [_i64, _i32p, _i32p, C.c_int32, _vp, _vp, C.POINTER(_i64)]),
         "orc_pcg": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, _i32p, _f64p, _i32p, _i32p,
-                              C.c_int32, C.c_int32, C.c_int32, C.c_int32, _f64p, _f64p, C.c_double,
+                              C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp, _f64p, _f64p,
+                              C.c_double,
                               _i64, C.POINTER(_i64), C.POINTER(C.c_double), _f64p,
                               C.POINTER(_i64)]),
         "orc_vcycle": (C.c_int, [_vp, C.c_int32, _i32p, _f64p, _i32p, _i32p, C.c_int32, C.c_int32,
-                                 C.c_int32, C.c_int32, _f64p, _f64p]),
+                                 C.c_int32, C.c_int32, _vp, _f64p, _f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -286,14 +287,29 @@ def _level_params(nlev, kinds, omegas, pres, posts):
     return full(kinds, np.int32), full(omegas, np.float64), full(pres, np.int32), full(posts, np.int32)
 
 
+def _coarse_inv_arg(coarse_inv, hier: HierView):
+    """The caller's dense FP64 inverse of the coarsest matrix (row-major,
+    in the coarsest level's permuted order), or None for the oracle's own
+    Cholesky inverse."""
+    if coarse_inv is None:
+        return None, None
+    a = np.ascontiguousarray(coarse_inv, np.float64)
+    nc = int(hier.h.n[hier.nlev - 1])
+    if a.shape != (nc, nc):
+        raise ValueError(f"coarse inverse must be {nc} x {nc}, got {a.shape}")
+    return a, a.ctypes.data_as(_vp)
+
+
 def pcg(hier: HierView, b, x0=None, tol=1e-8, maxit=1000, prec=1, cg_double=True,
         mg_double=True, kinds=1, omegas=1.0, pres=1, posts=1, coarse_steps=2,
-        direct_coarse=False, vcycles=1, reference_vcycle=False):
+        direct_coarse=False, vcycles=1, reference_vcycle=False, coarse_inv=None):
     """Restated CGSolver::solve with the multigrid (prec=1) or identity (0)
     preconditioner. reference_vcycle reproduces the reference's correction
     aliasing on coarse levels (solution = 2 P e); off, the textbook
-    x += P e. Returns x, iterations, relres2, history."""
+    x += P e. coarse_inv (with direct_coarse): a precomputed inverse of the
+    coarsest matrix. Returns x, iterations, relres2, history."""
     n = len(b)
+    keep, ci = _coarse_inv_arg(coarse_inv, hier)
     k, o, pr, po = _level_params(hier.nlev, kinds, omegas, pres, posts)
     x = np.zeros(n) if x0 is None else np.array(x0, np.float64, copy=True)
     it = _i64()
@@ -301,18 +317,20 @@ def pcg(hier: HierView, b, x0=None, tol=1e-8, maxit=1000, prec=1, cg_double=True
     hist = np.empty(maxit + 1, np.float64)
     hl = _i64()
     _check(lib.orc_pcg(hier.ptr(), prec, int(cg_double), int(mg_double), k, o, pr, po,
-                       coarse_steps, int(direct_coarse), vcycles, int(reference_vcycle),
+                       coarse_steps, int(direct_coarse), vcycles, int(reference_vcycle), ci,
                        np.ascontiguousarray(b, np.float64), x, tol, maxit, C.byref(it),
                        C.byref(rr), hist, C.byref(hl)))
     return x, it.value, rr.value, hist[:hl.value].copy()
 
 
 def vcycle(hier: HierView, r, mg_double=True, kinds=1, omegas=1.0, pres=1, posts=1,
-           coarse_steps=2, direct_coarse=False, vcycles=1, reference_vcycle=False):
+           coarse_steps=2, direct_coarse=False, vcycles=1, reference_vcycle=False,
+           coarse_inv=None):
     k, o, pr, po = _level_params(hier.nlev, kinds, omegas, pres, posts)
+    keep, ci = _coarse_inv_arg(coarse_inv, hier)
     z = np.empty(len(r), np.float64)
     _check(lib.orc_vcycle(hier.ptr(), int(mg_double), k, o, pr, po, coarse_steps,
-                          int(direct_coarse), vcycles, int(reference_vcycle),
+                          int(direct_coarse), vcycles, int(reference_vcycle), ci,
                           np.ascontiguousarray(r, np.float64), z))
     return z
 

@@ -114,3 +114,77 @@ def test_errors_leave_the_solver_usable(small):
     b = gs.uniform_pm1(3, A.rows)
     x, info = s.solve(b, tol=1e-8, maxit=200)
     assert info.converged and info.true_relres <= 1.5e-8
+
+
+def test_vector_lengths_are_checked(small):
+    """src/CGSolver.cc:90-98: a right-hand side or initial guess of another
+    length than the matrix is rejected before anything is read; so are
+    wrong-length vectors for one V-cycle or one SpMV."""
+    A, H = small
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_steps=2)
+    s.setup(H)
+    assert s.size() == A.rows
+    b = gs.uniform_pm1(3, A.rows)
+    with pytest.raises(gs.GpuSolverError, match="RHS size does not match matrix"):
+        s.solve(b[:-1])
+    with pytest.raises(gs.GpuSolverError, match="RHS size does not match matrix"):
+        s.solve(np.concatenate([b, b]))
+    with pytest.raises(gs.GpuSolverError, match="initial guess size does not match matrix"):
+        s.solve(b, x0=np.zeros(A.rows + 5))
+    with pytest.raises(gs.GpuSolverError, match="vector size does not match the finest level"):
+        s.vcycle(b[:10])
+    with pytest.raises(gs.GpuSolverError, match="encodeSpmv: size mismatch"):
+        s.spmv(np.zeros(A.rows - 1))
+    x, info = s.solve(b)  # still usable
+    assert info.converged
+
+
+def test_fallback_preconditions(small):
+    """A guarded solve refuses a plain solver that is not set up, was set up
+    for another size, or has not converged once (no rate to compare with)."""
+    A, H = small
+    b = gs.uniform_pm1(5, A.rows)
+    learned = gs.Solver(gs.PREC_MIXED, smoother=gs.SGS, coarse_steps=2)
+    learned.setup(H)
+    plain = gs.Solver(gs.PREC_MIXED, smoother=gs.SGS, coarse_steps=2)
+    learned.set_fallback(plain, lag=1.5)
+    with pytest.raises(gs.GpuSolverError, match="fallback solver is not set up"):
+        learned.solve(b)
+    plain.setup(H)
+    with pytest.raises(gs.GpuSolverError, match="no converged solve"):
+        learned.solve(b)
+    B = gs.poisson2d(20, 20)
+    HB = gs.Hierarchy.smoothed_aggregation(B, min_dofs=50)
+    other = gs.Solver(gs.PREC_MIXED, smoother=gs.SGS, coarse_steps=2)
+    other.setup(HB)
+    other.solve(gs.uniform_pm1(6, B.rows))
+    learned.set_fallback(other, lag=1.5)
+    with pytest.raises(gs.GpuSolverError, match="another matrix size"):
+        learned.solve(b)
+    plain.solve(b)
+    assert plain.result("log_rate") < 0.0
+    learned.set_fallback(plain, lag=1.5)
+    x, info = learned.solve(b)
+    assert info.converged and not learned.fell_back()
+
+
+def test_rejected_level_weights_change_nothing(small):
+    """An invalid per-level entry (omega >= 2 on the last smoothed level) is
+    rejected before any entry is applied: the next solve reproduces the
+    previous result bit for bit (eager and captured-graph paths)."""
+    A, H = small
+    b = gs.uniform_pm1(7, A.rows)
+    for gi in (0, 8):
+        s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_steps=2, graph_iters=gi)
+        s.setup(H)
+        L = len(s.describe()["levels"])
+        s.set_level_weights(omega=[1.2] * L)
+        x1, i1 = s.solve(b)
+        bad = [0.8] * L
+        bad[L - 2] = 2.5
+        with pytest.raises(gs.GpuSolverError, match="omega must be in"):
+            s.set_level_weights(omega=bad)
+        assert [lv["omega"] for lv in s.describe()["levels"]] == pytest.approx([1.2] * L)
+        x2, i2 = s.solve(b)
+        np.testing.assert_array_equal(x1, x2)
+        assert i1.iterations == i2.iterations

@@ -56,8 +56,10 @@ def test_training_reduces_loss():
 
 
 def test_guarded_solve_falls_back_on_bad_weights():
-    """A deliberately poor learned weight (w = 0.05) lags the plain rate;
-    the plain solver restarts the solve and still reaches 1e-8."""
+    """A deliberately poor learned weight (w = 0.05) lags the plain rate:
+    the rate check ends the guarded attempt before its iteration budget
+    (1.5 x the plain count) is spent, and the plain solver restarts the
+    solve and still reaches 1e-8."""
     A = gs.poisson2d(128, 128)
     H = gs.Hierarchy.smoothed_aggregation(A, min_dofs=500)
     b = gs.uniform_pm1(4, A.rows)
@@ -70,6 +72,9 @@ def test_guarded_solve_falls_back_on_bad_weights():
     bad.set_fallback(plain, lag=1.5)
     x, info = bad.solve(b, tol=1e-8)
     assert bad.fell_back()
+    assert bad.result("fallback_reason") == 1  # lagging the plain rate
+    budget = int(np.ceil(1.5 * ip.loop_iterations))
+    assert bad.result("first_iterations") < budget, (bad.result("first_iterations"), budget)
     assert info.converged and info.true_relres <= 1.5e-8
     np.testing.assert_array_equal(x, xp)
 

@@ -84,6 +84,33 @@ def test_spmv_block_bitexact(elast, prec, lo, pairs):
     np.testing.assert_array_equal(y, yo.astype(np.float64))
 
 
+@pytest.mark.parametrize("grid", [8, 16, 64, 4096])
+def test_persistent_spmv_walks_several_slices(grid):
+    """The persistent FP64 CG SpMV with small grids: with 8 workgroups each
+    wavefront walks about seven SELL slices of this 13 248-block-row
+    operator (its stride loop's later passes); every row sum is bit for bit
+    the oracle's. The PCG with it matches the default grid's iterations and
+    solution to rounding (p.Ap is grouped by workgroup)."""
+    E, rhs, _ = gs.elasticity(24, young=1.0, poisson=0.3, hetero_sigma=2.0, hetero_scale=1.5,
+                              seed=2)
+    H = gs.Hierarchy.smoothed_aggregation(E, block_size=3, eps_strong=0.1, max_levels=10,
+                                          min_dofs=500)
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=0, cg_block3=1,
+                  persist_grid=grid)
+    s.setup(H)
+    assert s.describe()["cg_pair_layout"]
+    slices = (E.rows // 3 + 63) // 64
+    assert grid > 8 or slices // 8 >= 6 * 4  # >= 6 slices per wavefront at grid 8
+    x = gs.uniform_pm1(8, E.rows)
+    np.testing.assert_array_equal(s.spmv(x), orc.spmv(E.ptr, E.col, E.val, x))
+    xs, info = s.solve(rhs, tol=1e-8, maxit=2000)
+    d = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=0, cg_block3=1)
+    d.setup(H)
+    xd, infod = d.solve(rhs, tol=1e-8, maxit=2000)
+    assert info.converged and abs(info.iterations - infod.iterations) <= 1
+    assert np.linalg.norm(xs - xd) <= 1e-10 * np.linalg.norm(xd)
+
+
 # Operator storage: -1 = one thread per row everywhere (SELL-64), 0 = lane
 # groups for every operator, None = the default selection by row length.
 GROUPS = [-1.0, 0.0, None]
@@ -443,3 +470,53 @@ def test_vcycle_bitexact_zero_sweep_level(elast, prec):
     zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=gs.BLOCK_GS, omegas=om,
                     pres=sw, posts=sw, coarse_steps=2)
     np.testing.assert_array_equal(s.vcycle(r), zo)
+
+
+@pytest.fixture(scope="module")
+def hetero_small():
+    """Heterogeneous coefficients at test size: elasticity on 14^3 vertices
+    with E = exp(1.5 g) (g smoothed with sigma 1.5 cells, seed 3), and the
+    variable-coefficient Poisson problem on 24^3 cells with k = exp(2 g)
+    (sigma 2 cells, seed 4)."""
+    E, rhs, _ = gs.elasticity(14, young=1.0, poisson=0.3, hetero_sigma=1.5, hetero_scale=1.5,
+                              seed=3)
+    HE = gs.Hierarchy.smoothed_aggregation(E, block_size=3, eps_strong=0.1, max_levels=10,
+                                           min_dofs=300)
+    V = gs.varcoef3d(24, seed=4, sigma=2.0, scale=2.0)
+    HV = gs.Hierarchy.smoothed_aggregation(V, block_size=1, eps_strong=0.1, max_levels=10,
+                                           min_dofs=200)
+    return E, rhs, HE, V, HV
+
+
+@pytest.mark.parametrize("grp", GROUPS)
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED, gs.PREC_SINGLE])
+def test_vcycle_bitexact_heterogeneous(hetero_small, prec, grp):
+    """Coefficient contrast of several decades (E or k = exp(c g)): block GS
+    on the elasticity hierarchy and SGS on the variable-coefficient one,
+    bit for bit the oracle's V-cycle."""
+    E, rhs, HE, V, HV = hetero_small
+    for mat, hier, kind, om, seed in ((E, HE, gs.BLOCK_GS, 0.7, 31), (V, HV, gs.SGS, 1.0, 32)):
+        s = gs.Solver(prec, smoother=kind, omega=om, coarse_steps=2, **group_opts(grp))
+        s.setup(hier)
+        hv, _ = hier_view(hier)
+        r = gs.uniform_pm1(seed, mat.rows)
+        zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=kind, omegas=om,
+                        coarse_steps=2)
+        np.testing.assert_array_equal(s.vcycle(r), zo)
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+def test_pcg_parity_heterogeneous(hetero_small, prec):
+    E, rhs, HE, V, HV = hetero_small
+    for mat, hier, b, kw in ((E, HE, rhs, dict(smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=0,
+                                                cg_block3=1)),
+                             (V, HV, gs.uniform_pm1(33, V.rows), dict(smoother=gs.SGS,
+                                                                       coarse_steps=2))):
+        s = gs.Solver(prec, **kw)
+        s.setup(hier)
+        hv, _ = hier_view(hier)
+        x, xo, info, true = _pcg_parity(s, hv, mat, b, prec, kinds=kw["smoother"],
+                                        omegas=kw.get("omega", 1.0),
+                                        coarse_steps=kw["coarse_steps"])
+        assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+        assert true <= 1.5e-8

@@ -315,3 +315,121 @@ def test_oracle_dense_spd_inverse_closed_forms():
 def test_oracle_dense_spd_inverse_rejects_indefinite():
     with pytest.raises(RuntimeError):
         orc.dense_spd_inverse(np.array([[1.0, 2.0], [2.0, 1.0]]))
+
+
+# ---------------------------------------------------------------------------
+# V-cycle operator in closed form (src/Smoothers/MultigridSmoother.cc:337-412).
+# ---------------------------------------------------------------------------
+def _sweep_matrix(A, classes, block, omega, sweeps):
+    """Dense approximate inverse B of `sweeps` symmetric multicolour sweeps
+    from x = 0 (x = B b). One colour update is x <- x + K_c (b - A x) with
+    K_c = w E_c D_c^-1 E_c^T, D_c the colour's diagonal (3x3 diagonal
+    blocks for block GS): rows of one colour do not couple, so the
+    reference's (1 - w) x + w D^-1 (b - off-diagonal sum) is exactly this.
+    The error propagates as I - K_c A; a symmetric sweep runs the colours
+    forward, then backward, and B = (I - G) A^-1 for the product G."""
+    n = A.shape[0]
+    I = np.eye(n)
+    Ks = []
+    for rows in classes:
+        idx = np.concatenate([np.arange(block * r, block * r + block) for r in rows]) \
+            if block > 1 else np.asarray(rows)
+        K = np.zeros((n, n))
+        K[np.ix_(idx, idx)] = omega * np.linalg.inv(A[np.ix_(idx, idx)])
+        Ks.append(K)
+    G = I.copy()
+    for _ in range(sweeps):
+        for K in Ks:
+            G = (I - K @ A) @ G
+        for K in reversed(Ks):
+            G = (I - K @ A) @ G
+    return (I - G) @ np.linalg.inv(A)
+
+
+def _classes(Ad, block):
+    m = sp.csr_matrix(Ad)
+    if block == 1:
+        ptr, col = m.indptr.astype(np.int32), m.indices.astype(np.int32)
+    else:
+        ptr, col = orc.block_graph(m.indptr.astype(np.int32), m.indices.astype(np.int32), block)
+    _, verts, offs, nc = orc.coloring(ptr, col)
+    return [verts[offs[c]:offs[c + 1]] for c in range(nc)]
+
+
+def _vcycle_closed_form(As, Ps, Rs, block, omega, coarse_steps, aliasing):
+    """M with z = M r for the V-cycle over dense levels As (finest first):
+    pre-smoothing from zero, residual, restriction, coarse correction,
+    prolongation, post-smoothing. Standard: x += P e. The reference's
+    storage on levels below the finest (aliasing): the prolongated
+    correction overwrites the level's solution and is added to itself,
+    x = 2 P e, dropping the pre-smoothed x."""
+    def level(k):
+        A = As[k]
+        if k == len(As) - 1:
+            return _sweep_matrix(A, _classes(A, block), block, omega, coarse_steps)
+        B = _sweep_matrix(A, _classes(A, block), block, omega, 1)
+        C = Ps[k] @ level(k + 1) @ Rs[k]
+        I = np.eye(A.shape[0])
+        X2 = 2.0 * C @ (I - A @ B) if (aliasing and k > 0) else B + C @ (I - A @ B)
+        return X2 + B @ (I - A @ X2)
+    return level(0)
+
+
+@pytest.mark.parametrize("aliasing", [False, True])
+@pytest.mark.parametrize("levels", [2, 3])
+@pytest.mark.parametrize("problem", ["sgs", "block_gs"])
+def test_vcycle_matches_closed_form_operator(problem, levels, aliasing):
+    """orc.vcycle applied to the unit vectors equals the dense V-cycle
+    operator built from matrix products (two- and three-level hierarchies,
+    both correction variants) to FP64 rounding; a dropped term, a swapped
+    pre / post order or an aliasing slip changes entries by O(1)."""
+    import gpusolver as gs
+    if problem == "sgs":
+        A, block, omega, kind = gs.poisson2d(12, 12), 1, 1.2, 1
+        H = gs.Hierarchy.smoothed_aggregation(A, min_dofs=10, max_levels=levels - 1)
+    else:
+        A, _, _ = gs.elasticity(4, young=1.0, poisson=0.3)
+        block, omega, kind = 3, 0.7, 3
+        H = gs.Hierarchy.smoothed_aggregation(A, block_size=3, min_dofs=6, max_levels=levels - 1)
+    assert H.levels == levels
+    Al = [H.get("A", k) for k in range(levels)]
+    Pl = [H.get("P", k) for k in range(levels - 1)]
+    Rl = [H.get("R", k) for k in range(levels - 1)]
+    hv = orc.HierView([(a.ptr, a.col, a.val) for a in Al], [(p.ptr, p.col, p.val) for p in Pl],
+                      [(r.ptr, r.col, r.val) for r in Rl])
+    M = _vcycle_closed_form([a.to_scipy().toarray() for a in Al],
+                            [p.to_scipy().toarray() for p in Pl],
+                            [r.to_scipy().toarray() for r in Rl], block, omega, 2, aliasing)
+    n = A.rows
+    Mo = np.stack([orc.vcycle(hv, e, kinds=kind, omegas=omega, coarse_steps=2,
+                              reference_vcycle=aliasing) for e in np.eye(n)], axis=1)
+    scale = np.abs(M).max()
+    assert np.abs(Mo - M).max() <= 1e-11 * scale
+    if levels == 3:
+        # The two variants differ on three levels, so the check tells them apart.
+        Mother = _vcycle_closed_form([a.to_scipy().toarray() for a in Al],
+                                     [p.to_scipy().toarray() for p in Pl],
+                                     [r.to_scipy().toarray() for r in Rl], block, omega, 2,
+                                     not aliasing)
+        assert np.abs(Mo - Mother).max() > 1e-6 * scale
+
+
+def test_oracle_caller_coarse_inverse_matches_own():
+    """A caller-supplied coarsest inverse (LAPACK) gives the oracle the same
+    PCG iterations and solution, to rounding, as its own Cholesky inverse."""
+    import gpusolver as gs
+    A = gs.poisson2d(40, 40)
+    H = gs.Hierarchy.smoothed_aggregation(A, min_dofs=100)
+    L = H.levels
+    hv = orc.HierView([(a.ptr, a.col, a.val) for a in (H.get("A", k) for k in range(L))],
+                      [(p.ptr, p.col, p.val) for p in (H.get("P", k) for k in range(L - 1))],
+                      [(r.ptr, r.col, r.val) for r in (H.get("R", k) for k in range(L - 1))])
+    b = np.cos(0.1 * np.arange(A.rows))
+    inv = np.linalg.inv(H.get("A", L - 1).to_scipy().toarray())
+    for mg_double in (True, False):
+        x1, it1, _, _ = orc.pcg(hv, b, mg_double=mg_double, direct_coarse=True)
+        x2, it2, _, _ = orc.pcg(hv, b, mg_double=mg_double, direct_coarse=True, coarse_inv=inv)
+        assert it1 == it2
+        assert np.linalg.norm(x1 - x2) <= 1e-10 * np.linalg.norm(x1)
+    with pytest.raises(ValueError):
+        orc.pcg(hv, b, direct_coarse=True, coarse_inv=np.eye(3))
