@@ -231,23 +231,6 @@ __global__ __launch_bounds__(kBlock) void k_block_inv(const T *inv, const T *res
   }
 }
 
-// Deterministic dot: fixed-tree block sums into partials, then one block
-// (k_dot_final) sums the partials in index order into the device scalar.
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_dot(const T *x, const T *y, int64_t n, T *partials) {
-  T v = T(0);
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * kBlock)
-    v += x[i] * y[i];
-  gsk::block_partial(v, partials);
-}
-
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_dot_final(const T *partials, uint32_t G, T *out) {
-  const T s = gsk::fixed_sum(partials, G);
-  if (threadIdx.x == 0) *out = s;
-}
-
 enum ScalarOp { kDiv, kMul, kSet, kCopy, kNeg, kSqrt };
 template <class T>
 __global__ void k_scalar(int op, const T *a, const T *b, T v, T *out) {
@@ -279,14 +262,9 @@ T *dp(const DevBuf &b) {
   return static_cast<T *>(b.get());
 }
 
-uint32_t dot_grid(int64_t n) {
-  const int64_t g = (n + kBlock - 1) / kBlock;
-  return static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(g, 2048)));
-}
-
 int32_t *zero_diag_flag(gs_encoder_s *e) {
   return reinterpret_cast<int32_t *>(static_cast<char *>(e->scratch.get()) +
-                                     (2048 + 32) * sizeof(double));
+                                     (gsk::kDotMaxGrid + 32) * sizeof(double));
 }
 
 template <class F>
@@ -326,11 +304,11 @@ int gs_backend_create_encoder(gs_backend_h b, gs_encoder_h *out) {
     auto e = std::make_unique<gs_encoder_s>();
     e->be = b;
     GS_HIP(hipStreamCreate(&e->stream));
-    e->scratch.alloc((2048 + 64) * sizeof(double));
+    e->scratch.alloc((gsk::kDotMaxGrid + 64) * sizeof(double));
     // The zero-diagonal flag starts at zero (a fill kernel, so the stream
     // stays capturable).
     k_zero<double><<<1, kBlock, 0, e->stream>>>(
-        reinterpret_cast<double *>(static_cast<char *>(e->scratch.get()) + 2048 * sizeof(double)),
+        reinterpret_cast<double *>(static_cast<char *>(e->scratch.get()) + gsk::kDotMaxGrid * sizeof(double)),
         64);
     *out = e.release();
   });
@@ -363,6 +341,14 @@ int gs_backend_submit_and_wait(gs_backend_h, gs_encoder_h e) {
 int gs_encoder_destroy(gs_encoder_h e) {
   delete e;
   return GS_OK;
+}
+
+int gs_encoder_stream(gs_encoder_h e, void **stream) {
+  return bguard([&] {
+    need(e, "encoder");
+    need(stream, "output");
+    *stream = static_cast<void *>(e->stream);
+  });
 }
 
 int gs_backend_create_vector(gs_backend_h b, int64_t n, const void *values, gs_dvec_h *out) {
@@ -406,6 +392,14 @@ int gs_dvec_download(gs_dvec_h v, void *out) {
 int gs_dvec_destroy(gs_dvec_h v) {
   delete v;
   return GS_OK;
+}
+
+int gs_dvec_data(gs_dvec_h v, void **ptr) {
+  return bguard([&] {
+    need(v, "vector");
+    need(ptr, "output");
+    *ptr = v->d.get();
+  });
 }
 
 int gs_backend_create_index_vector(gs_backend_h b, int64_t n, const int32_t *values,
@@ -465,6 +459,14 @@ int gs_dscalar_download(gs_dscalar_h s, double *value) {
 int gs_dscalar_destroy(gs_dscalar_h s) {
   delete s;
   return GS_OK;
+}
+
+int gs_dscalar_data(gs_dscalar_h s, void **ptr) {
+  return bguard([&] {
+    need(s, "scalar");
+    need(ptr, "output");
+    *ptr = s->d.get();
+  });
 }
 
 int gs_backend_create_csr(gs_backend_h b, gs_csr_h host, gs_dcsr_h *out) {
@@ -767,12 +769,12 @@ int gs_encode_dot(gs_encoder_h e, gs_dvec_h x, gs_dvec_h y, gs_dscalar_h out) {
     need(out, "scalar");
     same_size(x, y, "encodeDot");
     char *base = static_cast<char *>(e->scratch.get());
-    const uint32_t G = dot_grid(x->n);
+    const uint32_t G = gsk::dot_grid(x->n);
     dispatch(e->be, [&](auto t) {
       using T = decltype(t);
-      k_dot<T><<<G, kBlock, 0, e->stream>>>(dp<T>(x->d), dp<T>(y->d), x->n,
+      gsk::k_dot<T><<<G, kBlock, 0, e->stream>>>(dp<T>(x->d), dp<T>(y->d), x->n,
                                             reinterpret_cast<T *>(base));
-      k_dot_final<T><<<1, kBlock, 0, e->stream>>>(reinterpret_cast<const T *>(base), G,
+      gsk::k_dot_final<T><<<1, kBlock, 0, e->stream>>>(reinterpret_cast<const T *>(base), G,
                                                   dp<T>(out->d));
     });
   });

@@ -148,6 +148,32 @@ __device__ __forceinline__ T fixed_sum(const T *partials, uint32_t nparts) {
   return block_sum(s);
 }
 
+// Deterministic dot product (Backend::encodeDot): each workgroup of a grid
+// of at most kDotMaxGrid strides over the vectors and stores one partial
+// (fixed tree), then one workgroup (k_dot_final) adds the partials in
+// index order into the output scalar.
+constexpr uint32_t kDotMaxGrid = 2048;
+
+inline uint32_t dot_grid(int64_t n) {
+  const int64_t g = (n + kBlock - 1) / kBlock;
+  return static_cast<uint32_t>(g < 1 ? 1 : (g > kDotMaxGrid ? kDotMaxGrid : g));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot(const T *x, const T *y, int64_t n, T *partials) {
+  T v = T(0);
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kBlock)
+    v += x[i] * y[i];
+  block_partial(v, partials);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_dot_final(const T *partials, uint32_t G, T *out) {
+  const T s = fixed_sum(partials, G);
+  if (threadIdx.x == 0) *out = s;
+}
+
 template <class T>
 __device__ __forceinline__ bool finite_val(T v) {
   return v == v && v - v == T(0);

@@ -114,6 +114,9 @@ _SIGS.update({
     "gs_backend_submit": (C.c_int, [_vp, _vp]),
     "gs_backend_submit_and_wait": (C.c_int, [_vp, _vp]),
     "gs_encoder_destroy": (C.c_int, [_vp]),
+    "gs_encoder_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "gs_dvec_data": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "gs_dscalar_data": (C.c_int, [_vp, C.POINTER(_vp)]),
     "gs_backend_create_vector": (C.c_int, [_vp, _i64, _vp, _VP]),
     "gs_dvec_size": (C.c_int, [_vp, C.POINTER(_i64)]),
     "gs_dvec_update": (C.c_int, [_vp, _vp, _i64]),
@@ -757,6 +760,12 @@ class DeviceVector(_Obj):
         check(lib.gs_dvec_download(self.h, out.ctypes.data_as(_vp)))
         return out
 
+    def data(self) -> int:
+        """Device address of the values (CudaDeviceVector::data)."""
+        p = _vp()
+        check(lib.gs_dvec_data(self.h, C.byref(p)))
+        return p.value or 0
+
 
 class DeviceIndexVector(_Obj):
     _destroy = "gs_divec_destroy"
@@ -779,6 +788,12 @@ class DeviceScalar(_Obj):
         check(lib.gs_dscalar_download(self.h, C.byref(v)))
         return v.value
 
+    def data(self) -> int:
+        """Device address of the scalar (CudaDeviceScalar::data)."""
+        p = _vp()
+        check(lib.gs_dscalar_data(self.h, C.byref(p)))
+        return p.value or 0
+
 
 class DeviceCSRMatrix(_Obj):
     _destroy = "gs_dcsr_destroy"
@@ -794,6 +809,12 @@ class DeviceELLMatrix(_Obj):
 
 class Encoder(_Obj):
     _destroy = "gs_encoder_destroy"
+
+    def stream(self) -> int:
+        """The encoder's hipStream_t (CudaEncoder::stream)."""
+        p = _vp()
+        check(lib.gs_encoder_stream(self.h, C.byref(p)))
+        return p.value or 0
 
 
 class Backend:

@@ -344,12 +344,19 @@ int gs_backend_create_encoder(gs_backend_h b, gs_encoder_h *out);
 int gs_backend_submit(gs_backend_h b, gs_encoder_h e);
 int gs_backend_submit_and_wait(gs_backend_h b, gs_encoder_h e);
 int gs_encoder_destroy(gs_encoder_h e);
+/* The encoder's HIP stream (a hipStream_t) for work the caller enqueues in
+ * order with it (CudaEncoder::stream, include/GPUSolver/cuda/
+ * CudaEncoder.h:17). Owned by the encoder. */
+int gs_encoder_stream(gs_encoder_h e, void **stream);
 /* createVector(size[, values]) (Backend.h:26-28); values in the backend's dtype. */
 int gs_backend_create_vector(gs_backend_h b, int64_t n, const void *values, gs_dvec_h *out);
 int gs_dvec_size(gs_dvec_h v, int64_t *n);
 int gs_dvec_update(gs_dvec_h v, const void *values, int64_t n);
 int gs_dvec_download(gs_dvec_h v, void *out);
 int gs_dvec_destroy(gs_dvec_h v);
+/* Device address of the vector's values (CudaDeviceVector::data,
+ * include/GPUSolver/cuda/CudaDeviceVector.h:38-40); valid until destroy. */
+int gs_dvec_data(gs_dvec_h v, void **ptr);
 /* createIndexVector(size[, values]) (Backend.h:30-33). */
 int gs_backend_create_index_vector(gs_backend_h b, int64_t n, const int32_t *values,
                                    gs_divec_h *out);
@@ -359,6 +366,9 @@ int gs_divec_destroy(gs_divec_h v);
 int gs_backend_create_scalar(gs_backend_h b, double value, gs_dscalar_h *out);
 int gs_dscalar_download(gs_dscalar_h s, double *value);
 int gs_dscalar_destroy(gs_dscalar_h s);
+/* Device address of the scalar (CudaDeviceScalar::data,
+ * include/GPUSolver/cuda/CudaDeviceScalar.h:15-16). */
+int gs_dscalar_data(gs_dscalar_h s, void **ptr);
 /* createCSRMatrix(const HostCSRMatrix&) (Backend.h:39); updateValues. */
 int gs_backend_create_csr(gs_backend_h b, gs_csr_h host, gs_dcsr_h *out);
 int gs_dcsr_update_values(gs_dcsr_h m, const double *values, int64_t nnz);

@@ -236,3 +236,16 @@ def test_block_gs_invalid_range_raises():
     e = be.create_encoder()
     with pytest.raises(gs.GpuSolverError, match="invalid color range"):
         be.encode_block_gauss_seidel_color3x3(e, dA, dv, di, 3, 2, db, dx, 0.7)
+
+
+def test_device_addresses_and_stream():
+    """CudaDeviceVector::data / CudaDeviceScalar::data /
+    CudaEncoder::stream: stable, distinct device addresses and the
+    encoder's own stream."""
+    be = gs.Backend(np.float64)
+    x, y, s = be.create_vector(np.ones(10)), be.create_vector(10), be.create_scalar(2.0)
+    e1, e2 = be.create_encoder(), be.create_encoder()
+    px, py, ps = x.data(), y.data(), s.data()
+    assert px and py and ps and len({px, py, ps}) == 3
+    assert x.data() == px and s.data() == ps
+    assert e1.stream() and e1.stream() != e2.stream()
