@@ -117,6 +117,7 @@ _SIGS.update({
     "gs_encoder_stream": (C.c_int, [_vp, C.POINTER(_vp)]),
     "gs_dvec_data": (C.c_int, [_vp, C.POINTER(_vp)]),
     "gs_dscalar_data": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "gs_dot_noalloc": (C.c_int, [_i32, _vp, _vp, _i64, _vp, _vp]),
     "gs_backend_create_vector": (C.c_int, [_vp, _i64, _vp, _VP]),
     "gs_dvec_size": (C.c_int, [_vp, C.POINTER(_i64)]),
     "gs_dvec_update": (C.c_int, [_vp, _vp, _i64]),

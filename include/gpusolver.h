@@ -391,6 +391,19 @@ int gs_encode_gauss_seidel_color(gs_encoder_h e, gs_dcsr_h A, gs_divec_h color_v
                                  int64_t color_start, int64_t color_count, gs_dvec_h rhs,
                                  gs_dvec_h x, double omega);
 int gs_encode_dot(gs_encoder_h e, gs_dvec_h x, gs_dvec_h y, gs_dscalar_h out);
+/* No counterpart in the reference: the variant of gs_encode_dot that
+ * allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). out = x . y for caller-owned device arrays of n values of
+ * dtype (GS_DTYPE_F32 / F64) and a caller-owned device scalar, enqueued on
+ * `stream` (a hipStream_t; NULL: the null stream); same kernels and grid as
+ * gs_encode_dot, so the same sum bit for bit. Its partial sums live in a
+ * 16 KiB array of the library in each GPU's memory (2048 partials: the
+ * grid never exceeds 2048 workgroups, so every n fits; negative n is an
+ * error). Calls on one GPU share that array: calls that overlap in time on
+ * one GPU (different streams, no ordering between them) race. */
+int gs_dot_noalloc(int32_t dtype, const void *x, const void *y, int64_t n, void *out,
+                   void *stream);
 int gs_encode_scalar_divide(gs_encoder_h e, gs_dscalar_h num, gs_dscalar_h den, gs_dscalar_h out);
 int gs_encode_scalar_multiply(gs_encoder_h e, gs_dscalar_h a, gs_dscalar_h b, gs_dscalar_h out);
 int gs_encode_scalar_set(gs_encoder_h e, double value, gs_dscalar_h out);
