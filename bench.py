@@ -5,19 +5,34 @@ recursive relative residual, FP64 true residual verified afterwards) of the
 selected workload, with the matrix hierarchy, right-hand side and all
 vectors already resident in HBM. The headline mode is the FP32 V-cycle
 inside FP64 CG with learned per-level smoother weights ("mixed+learned");
-plain FP64 AMG-PCG on the same hierarchy is timed beside it. The dominant
-kernel (the CG operator's SpMV) is timed with HIP events on the solver's
-own stream and reported against the 8 TB/s HBM roofline. The CPU oracle
-(a C++/OpenMP restatement of the reference's CPU path) is timed on a
-bounded sample as a baseline.
+plain FP64 AMG-PCG on the same hierarchy is timed beside it. The hot
+kernels are timed with HIP events on each solver's own stream and reported
+against the 8 TB/s HBM roofline. The CPU oracle (a C++/OpenMP restatement
+of the reference's CPU path) is timed on the same workload as a baseline,
+in FP64 and in FP32 (the reference's own precision).
+
+The other workloads run after the headline one (fewer steps) and go into
+the "configs" object: plain FP64, mixed and mixed + learned device times,
+iterations, FP64 true residuals, the CG SpMV's fraction of the HBM peak,
+kernel launches per PCG iteration and each mode's error against an FP64
+oracle solve. stderr carries one comparison table per workload, as the
+reference's drivers print (examples/multigridPoisson/mainAMGtest.cc:786-834).
+
+Counter collection (rocprofv3) runs the hot kernels without a timed region:
+  --probe kernels        the headline workload's hot kernels, `--reps` times
+                         each, then a 512 MiB device copy (the calibration)
+  --probe headline-solve one headline-mode solve, every kernel its own
+                         dispatch, then the same copies
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+                       [--configs all|none|a,b] [--no-cpu] [--probe MODE]
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -29,6 +44,8 @@ sys.path.insert(0, str(REPO / "gpu-conjugate-gradient-solver_amd"))
 
 METRIC = "AMG-PCG solve-time to 1e-8 relres, mixed+learned vs plain; SpMV % of HBM peak"
 HBM_PEAK_GBS = 8000.0
+HEADLINE = "hetero_elasticity_96"
+OTHER_CONFIGS = ["poisson2d_256", "poisson3d_128", "elasticity_64", "varcoef3d_160"]
 
 
 def build_workload(name: str):
@@ -66,10 +83,10 @@ def build_workload(name: str):
 
 def measured_traffic(config: str, kernel_id: str):
     """HBM bytes per launch of kernel `kernel_id` (cg_spmv, l0_sweep,
-    l1_sweep, l0_restrict, l0_prolong, l0_residual) from the committed rocprofv3 counter
-    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the factor the same
-    counter reads for a copy of known size), or None when none was taken
-    for this workload and kernel."""
+    l1_sweep, l0_restrict, l0_prolong, l0_residual) from the committed
+    rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE, each scaled by the
+    factor the same counter reads for a copy of known size), or None when
+    none was taken for this workload and kernel."""
     path = REPO / "profiles" / f"pmc_{config}.md"
     if not path.exists():
         return None
@@ -92,11 +109,6 @@ def progress(msg: str) -> None:
     """One progress line per finished phase on stderr (stdout carries only
     the result line)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def progress_mode(modes: dict, name: str) -> None:
-    m = modes[name]
-    progress(f"{name}: {m['device_ms']:.2f} ms, {m['iterations']} iterations")
 
 
 def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
@@ -123,6 +135,7 @@ def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
 
 def time_solves(s, b, steps, warmup, tol=1e-8, maxit=2000):
     infos = []
+    x = None
     for _ in range(warmup):
         s.solve(b, tol=tol, maxit=maxit)
     t0 = time.perf_counter()
@@ -133,33 +146,237 @@ def time_solves(s, b, steps, warmup, tol=1e-8, maxit=2000):
     return x, infos, (t1 - t0) * 1e3 / steps
 
 
-def cpu_baseline(H, b, kinds, omega, coarse_steps):
-    """The oracle (a C++/OpenMP restatement of the reference's CPU solve)
-    runs the plain FP64 method on the same hierarchy to 1e-8: its wall time
-    is the CPU baseline and its solution the FP64 reference every GPU mode's
-    error is reported against."""
+def hier_view(H, levels=None):
     sys.path.insert(0, str(REPO / "oracle"))
     import oracle_api as orc
-    L = H.levels
+    L = H.levels if levels is None else levels
     Al = [H.get("A", k) for k in range(L)]
-    hv = orc.HierView([(a.ptr, a.col, a.val) for a in Al],
-                      [(p.ptr, p.col, p.val) for p in (H.get("P", k) for k in range(L - 1))],
-                      [(r.ptr, r.col, r.val) for r in (H.get("R", k) for k in range(L - 1))])
+    return orc.HierView([(a.ptr, a.col, a.val) for a in Al],
+                        [(p.ptr, p.col, p.val) for p in (H.get("P", k) for k in range(L - 1))],
+                        [(r.ptr, r.col, r.val) for r in (H.get("R", k) for k in range(L - 1))])
+
+
+def host_cpu() -> dict:
+    """What the CPU baseline ran on: logical CPUs of the host, the CPUs this
+    process may use, the CPU model and the OpenMP thread setting."""
+    model = platform.processor() or ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": usable, "host_logical_cpus": os.cpu_count(), "model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def oracle_solve(H, b, kinds, omega, coarse_steps, fp32=False, maxit=2000):
+    """One plain-method solve by the oracle (a C++/OpenMP restatement of
+    the reference's CPU solve) on the same hierarchy to 1e-8: FP64 (the
+    parity reference) or FP32 throughout (the reference's own precision).
+    Returns x, iterations, relres^2 and the wall time in ms (smoother setup
+    included, as the reference's solver builds it per solve)."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle_api as orc
+    hv = hier_view(H)
     orc.lib.orc_set_parallel_dot(1)
-    threads = orc.lib.orc_num_threads()
     t0 = time.perf_counter()
-    x, its, rr, _ = orc.pcg(hv, b, tol=1e-8, maxit=2000, kinds=kinds, omegas=omega,
-                            coarse_steps=coarse_steps)
-    t = time.perf_counter() - t0
-    return x, {
-        "value": round(t * 1e3, 1),
+    x, its, rr, _ = orc.pcg(hv, b, tol=1e-8, maxit=maxit, kinds=kinds, omegas=omega,
+                            coarse_steps=coarse_steps, cg_double=not fp32, mg_double=not fp32)
+    return x, its, rr, (time.perf_counter() - t0) * 1e3
+
+
+def cpu_baseline(H, b, kinds, omega, coarse_steps):
+    """FP64 and FP32 oracle solves of the headline workload on the host's
+    cores; the FP64 solution is the reference every GPU mode's error is
+    reported against."""
+    sys.path.insert(0, str(REPO / "oracle"))
+    import oracle_api as orc
+    threads = orc.lib.orc_num_threads()
+    xo, its, rr, ms = oracle_solve(H, b, kinds, omega, coarse_steps)
+    progress(f"CPU oracle FP64: {ms:.0f} ms, {its} iterations")
+    # FP32 throughout may stall above the FP64 target: capped at 1.5 x the
+    # FP64 iteration count (reported as not converged then).
+    x32, its32, rr32, ms32 = oracle_solve(H, b, kinds, omega, coarse_steps, fp32=True,
+                                          maxit=int(1.5 * its) + 10)
+    progress(f"CPU oracle FP32: {ms32:.0f} ms, {its32} iterations")
+    cpu = {
+        "value": round(ms, 1),
         "unit": "ms per solve",
         "cores": threads,
         "kind": "port",
         "sample": (f"one full FP64 solve of the same workload (same hierarchy, smoother and "
                    f"tolerance) by the oracle on {threads} OpenMP threads: {its} iterations, "
                    f"relres^2 {rr:.3g}; includes its smoother setup (block inverses, colouring)"),
+        "host": host_cpu(),
+        "fp32_as_reference": {"ms": round(ms32, 1), "iterations": its32,
+                              "relres2": float(f"{rr32:.4g}"),
+                              "converged": bool(rr32 < 1e-16)},
     }
+    return xo, cpu
+
+
+def make_modes(gs, sk, graph_iters):
+    """(name, precision, extra options, learned weights?) of each mode."""
+    mixed_kw = dict(sk)
+    # FP64 dense direct solve at the first level within max_direct unknowns
+    # (the inverse is built on the device); deeper levels are not used.
+    mixed_kw["coarse_direct"] = 1
+    mixed_kw["direct_truncate"] = 1
+    return [("plain_fp64", gs.PREC_DOUBLE, dict(sk), False),
+            ("fp64_direct", gs.PREC_DOUBLE, mixed_kw, False),
+            ("mixed", gs.PREC_MIXED, mixed_kw, False),
+            ("mixed_learned", gs.PREC_MIXED, mixed_kw, True)]
+
+
+def run_modes(config, H, b, sk, steps, warmup, graph_iters, names):
+    """Sets up and times each named mode; returns (modes, solvers,
+    solutions). The plain FP64 solver serves as the learned mode's
+    fallback."""
+    import gpusolver as gs
+    modes, solvers, sols = {}, {}, {}
+    for name, prec, kw, use_learned in make_modes(gs, sk, graph_iters):
+        if name not in names:
+            continue
+        lw = None
+        if use_learned:
+            lw = learned_weights(config, H)
+            if lw is None:
+                continue
+        t0 = time.perf_counter()
+        s = gs.Solver(prec, graph_iters=graph_iters, **kw)
+        s.setup(H)
+        if lw is not None:
+            s.set_level_weights(omega=lw[0], sweeps=lw[1])
+            s.set_fallback(solvers["plain_fp64"], lag=1.5)
+        setup_ms = (time.perf_counter() - t0) * 1e3
+        x, infos, host_ms = time_solves(s, b, steps, warmup)
+        d = s.describe()
+        m = dict(ms=host_ms, device_ms=float(np.mean([i.solve_ms for i in infos])),
+                 iterations=infos[-1].iterations, true_relres=infos[-1].true_relres,
+                 setup_ms=setup_ms, launches_per_iteration=d["launches_per_iteration"],
+                 levels_used=len(d["levels"]))
+        if lw is not None:
+            m.update(fell_back=s.fell_back(), omega=[round(float(w), 3) for w in lw[0]],
+                     sweeps=[int(v) for v in lw[1]])
+        modes[name] = m
+        solvers[name] = s
+        sols[name] = x
+        progress(f"{config} {name}: {m['device_ms']:.2f} ms, {m['iterations']} iterations, "
+                 f"{m['launches_per_iteration']:.0f} launches per iteration")
+    return modes, solvers, sols
+
+
+def comparison_table(config, modes, t_hier_ms):
+    """The reference drivers' comparison table (examples/multigridPoisson/
+    mainAMGtest.cc:786-834), on stderr: device solve time of the resident
+    solve, host setup time of each solver."""
+    w = 96
+    lines = ["=" * w, f"PRECONDITIONER COMPARISON: {config}".center(w), "=" * w,
+             f"{'Mode':<28}{'Iterations':>12}{'Setup (ms)':>14}{'Solve (ms)':>14}"
+             f"{'Total (ms)':>14}{'True residual':>14}", "-" * w]
+    for name, m in modes.items():
+        lines.append(f"{name:<28}{m['iterations']:>12}{m['setup_ms']:>14.3f}"
+                     f"{m['device_ms']:>14.3f}{m['setup_ms'] + m['device_ms']:>14.3f}"
+                     f"{m['true_relres']:>14.3e}")
+    lines += ["=" * w, f"Shared AMG preprocessing: hierarchy construction = {t_hier_ms:.1f} ms"]
+    print("\n".join(lines), file=sys.stderr, flush=True)
+
+
+def spmv_entry(s, reps=50):
+    ms, by = s.time_kernel(0, reps)
+    gbs = by / ms / 1e6
+    return {"avg_launch_ms": round(ms, 5), "algorithmic_bytes": by, "achieved_GBs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def other_config(name, steps, warmup, graph_iters, with_oracle):
+    """One non-headline workload: plain FP64, mixed and mixed + learned,
+    the plain solver's CG SpMV against the roofline, and each mode's error
+    against an FP64 oracle solve."""
+    import gpusolver as gs
+    t0 = time.perf_counter()
+    A, b, hk, sk, desc = build_workload(name)
+    H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
+    t_hier = (time.perf_counter() - t0) * 1e3
+    modes, solvers, sols = run_modes(name, H, b, sk, steps, warmup, graph_iters,
+                                     ("plain_fp64", "mixed", "mixed_learned"))
+    out = {"description": desc, "n": int(A.rows), "nnz": int(A.nnz), "levels": H.levels,
+           "hierarchy_ms": round(t_hier, 1),
+           "cg_spmv": spmv_entry(solvers["plain_fp64"]),
+           "modes": modes}
+    head = "mixed_learned" if "mixed_learned" in modes else "mixed"
+    out["speedup_vs_plain"] = round(modes["plain_fp64"]["device_ms"] / modes[head]["device_ms"], 3)
+    if with_oracle:
+        xo, its, rr, ms = oracle_solve(H, b, sk["smoother"], sk["omega"], sk["coarse_steps"])
+        no = float(np.linalg.norm(xo))
+        for k, x in sols.items():
+            modes[k]["rel_err_vs_cpu_oracle"] = float(np.linalg.norm(x - xo) / no)
+        out["oracle_fp64"] = {"ms": round(ms, 1), "iterations": its}
+    for m in modes.values():
+        for k, v in list(m.items()):
+            if isinstance(v, float):
+                m[k] = float(f"{v:.6g}")
+    comparison_table(name, modes, t_hier)
+    return out
+
+
+def probe(a) -> None:
+    """Counter-collection runs (no timed region): see the module docstring."""
+    import gpusolver as gs
+    A, b, hk, sk, desc = build_workload(a.config)
+    H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
+
+    def copies() -> int:
+        n = 512 * (1 << 20) // 8
+        be = gs.Backend(np.float64)
+        x = be.create_vector(np.ones(n))
+        y = be.create_vector(n)
+        for _ in range(a.reps):
+            e = be.create_encoder()
+            be.encode_copy(e, x, y)
+            be.submit_and_wait(e)
+        return n
+
+    mk = dict(sk)
+    mk.update(coarse_direct=1, direct_truncate=1)
+    if a.probe == "headline-solve":
+        m = gs.Solver(gs.PREC_MIXED, graph_iters=0, **mk)
+        m.setup(H)
+        w = learned_weights(a.config, H)
+        if w is not None:
+            m.set_level_weights(omega=w[0], sweeps=w[1])
+        m.solve(b, tol=1e-8, maxit=2000)
+        x, info = m.solve(b, tol=1e-8, maxit=2000)
+        n = copies()
+        print(json.dumps({"config": a.config, "mode": "mixed_learned" if w is not None else "mixed",
+                          "iterations": info.iterations, "solve_ms": info.solve_ms,
+                          "copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                          "reps": a.reps}))
+        return
+    s = gs.Solver(gs.PREC_DOUBLE, graph_iters=0, **sk)
+    s.setup(H)
+    s.solve(b, tol=1e-8, maxit=3)  # leaves valid vectors in p
+    res = {"config": a.config}
+    res["spmv_ms"], res["spmv_algorithmic_bytes"] = s.time_kernel(0, a.reps)
+    # Hot kernels of the mixed-precision solver (FP32 V-cycle).
+    m = gs.Solver(gs.PREC_MIXED, graph_iters=0, **mk)
+    m.setup(H)
+    m.solve(b, tol=1e-8, maxit=3)
+    for key, which in (("sweep", 3), ("sweep1", 4), ("restrict", 5), ("prolong", 6),
+                       ("residual", 7)):
+        try:
+            res[f"{key}_ms"], res[f"{key}_algorithmic_bytes"] = m.time_kernel(which, a.reps)
+        except gs.GpuSolverError:
+            pass  # not present in this hierarchy
+    n = copies()
+    res.update(copy_bytes_read=n * 8, copy_bytes_written=n * 8, reps=a.reps)
+    print(json.dumps(res))
 
 
 def main() -> None:
@@ -167,16 +384,24 @@ def main() -> None:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="hetero_elasticity_96")
+    ap.add_argument("--config", default=HEADLINE)
+    ap.add_argument("--configs", default="all",
+                    help="other workloads for the configs object: all, none, or a comma list")
+    ap.add_argument("--config-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph-iters", type=int, default=8,
                     help="PCG iterations per captured HIP graph (0: eager launches of the same kernels)")
+    ap.add_argument("--probe", choices=["kernels", "headline-solve"])
+    ap.add_argument("--reps", type=int, default=5, help="launches per kernel in --probe runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "the solve runs on one GPU; --gpus above 1 is not supported "
                                    "(a multi-GPU run would shard rows by contiguous colour "
                                    "blocks and exchange halos, which is not built)"}))
         sys.exit(2)
+    if a.probe:
+        probe(a)
+        return
     import gpusolver as gs
 
     t0 = time.perf_counter()
@@ -185,61 +410,12 @@ def main() -> None:
     t_hier = time.perf_counter() - t0
     progress(f"hierarchy built: {H.levels} levels, {t_hier:.1f} s")
 
-    modes = {}
-    # Plain FP64 AMG-PCG, the reference's method in FP64.
-    t0 = time.perf_counter()
-    plain = gs.Solver(gs.PREC_DOUBLE, graph_iters=a.graph_iters, **sk)
-    plain.setup(H)
-    t_setup_plain = time.perf_counter() - t0
-    xp, ip, ms_plain = time_solves(plain, b, a.steps, a.warmup)
-    modes["plain_fp64"] = dict(ms=ms_plain, iterations=ip[-1].iterations,
-                               true_relres=ip[-1].true_relres,
-                               device_ms=float(np.mean([i.solve_ms for i in ip])))
-    progress_mode(modes, "plain_fp64")
-    mixed_kw = dict(sk)
-    # FP64 dense direct solve at the first level within max_direct unknowns
-    # (the inverse is built on the device); deeper levels are not used.
-    mixed_kw["coarse_direct"] = 1
-    mixed_kw["direct_truncate"] = 1
-    # FP64 throughout with the FP64 direct coarsest solve: separates the gain of
-    # the direct coarsest solve from that of the FP32 V-cycle.
-    fd = gs.Solver(gs.PREC_DOUBLE, graph_iters=a.graph_iters, **mixed_kw)
-    fd.setup(H)
-    xd, idd, ms_fd = time_solves(fd, b, a.steps, a.warmup)
-    modes["fp64_direct"] = dict(ms=ms_fd, iterations=idd[-1].iterations,
-                                true_relres=idd[-1].true_relres,
-                                device_ms=float(np.mean([i.solve_ms for i in idd])))
-    progress_mode(modes, "fp64_direct")
-    del fd
-    # Mixed: FP32 V-cycle in FP64 CG, FP64 dense direct coarse solve.
-    mixed = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
-    mixed.setup(H)
-    xm, im, ms_mixed = time_solves(mixed, b, a.steps, a.warmup)
-    modes["mixed"] = dict(ms=ms_mixed, iterations=im[-1].iterations,
-                          true_relres=im[-1].true_relres,
-                          device_ms=float(np.mean([i.solve_ms for i in im])))
-    progress_mode(modes, "mixed")
-    head = "mixed"
-    ml = None
-    lw = learned_weights(a.config, H)
-    if lw is not None:
-        ml = gs.Solver(gs.PREC_MIXED, graph_iters=a.graph_iters, **mixed_kw)
-        ml.setup(H)
-        ml.set_level_weights(omega=lw[0], sweeps=lw[1])
-        ml.set_fallback(plain, lag=1.5)
-        xl, il, ms_l = time_solves(ml, b, a.steps, a.warmup)
-        modes["mixed_learned"] = dict(ms=ms_l, iterations=il[-1].iterations,
-                                      true_relres=il[-1].true_relres,
-                                      device_ms=float(np.mean([i.solve_ms for i in il])),
-                                      fell_back=ml.fell_back(),
-                                      omega=[round(float(w), 3) for w in lw[0]],
-                                      sweeps=[int(v) for v in lw[1]])
-        progress_mode(modes, "mixed_learned")
-        head = "mixed_learned"
+    modes, solvers, sols = run_modes(a.config, H, b, sk, a.steps, a.warmup, a.graph_iters,
+                                     ("plain_fp64", "fp64_direct", "mixed", "mixed_learned"))
+    plain = solvers["plain_fp64"]
+    head = "mixed_learned" if "mixed_learned" in modes else "mixed"
     hm = modes[head]
-    sols = {"plain_fp64": xp, "fp64_direct": xd, "mixed": xm}
-    if head == "mixed_learned":
-        sols["mixed_learned"] = xl
+    xp = sols["plain_fp64"]
     # Error of the headline solution against the plain FP64 GPU solve (both
     # verified by their FP64 true residual).
     err_vs_plain = float(np.linalg.norm(sols[head] - xp) / np.linalg.norm(xp))
@@ -247,9 +423,9 @@ def main() -> None:
     # Hot kernels, HIP events on each solver's own stream: the plain mode's
     # CG SpMV (FP64), and the headline solver's level-0 and level-1
     # symmetric Gauss-Seidel sweeps (one colour launch per colour and
-    # direction). The roofline entry is the headline kernel with the largest
-    # time per PCG iteration (sweep time x sweeps per V-cycle).
-    hs = ml if head == "mixed_learned" else mixed
+    # direction), transfers and residual. The roofline entry is the
+    # headline kernel with the largest time per PCG iteration.
+    hs = solvers[head]
     hdesc = hs.describe()
     spmv_ms, spmv_bytes = plain.time_kernel(0, 50)
     cands = [{"id": "cg_spmv", "kernel": "CG operator SpMV (FP64, plain mode)", "ms": spmv_ms,
@@ -292,14 +468,27 @@ def main() -> None:
     copy_gbs = copy_bandwidth()
     achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
     desc_plain = plain.describe()
+    progress("headline kernels timed")
+    comparison_table(a.config, modes, t_hier * 1e3)
 
     cpu = None
     if not a.no_cpu:
-        progress("kernels timed; CPU baseline running")
         xo, cpu = cpu_baseline(H, b, sk["smoother"], sk["omega"], sk["coarse_steps"])
         no = float(np.linalg.norm(xo))
         for name, xs in sols.items():
             modes[name]["rel_err_vs_cpu_oracle"] = float(np.linalg.norm(xs - xo) / no)
+    setup_s = {"hierarchy": round(t_hier, 2),
+               "plain_upload": round(modes["plain_fp64"]["setup_ms"] / 1e3, 2),
+               "headline_upload": round(hm["setup_ms"] / 1e3, 2)}
+    del solvers, sols, hs, plain
+
+    configs = {}
+    names = OTHER_CONFIGS if a.configs == "all" else \
+        ([] if a.configs == "none" else [c for c in a.configs.split(",") if c])
+    for name in names:
+        if name == a.config:
+            continue
+        configs[name] = other_config(name, a.config_steps, 1, a.graph_iters, not a.no_cpu)
 
     # Inputs resident in HBM: the device (HIP-event) time of the solve loop;
     # the host time per step additionally holds the b / x transfers, the
@@ -345,8 +534,9 @@ def main() -> None:
                  "traffic": measured_traffic(a.config, "cg_spmv")},
         "hbm_copy": {"GBs": round(copy_gbs, 1), "frac": round(copy_gbs / HBM_PEAK_GBS, 4),
                      "kernel": "k_copy<double>, 10 x 512 MiB, host-timed"},
-        "setup_s": {"hierarchy": round(t_hier, 2), "plain_upload": round(t_setup_plain, 2)},
+        "setup_s": setup_s,
         "levels": desc_plain["levels"],
+        "configs": configs,
     }
     print(json.dumps(out))
 

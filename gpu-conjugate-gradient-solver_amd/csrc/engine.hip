@@ -218,6 +218,8 @@ class Engine final : public EngineBase {
   bool cg_vop_ = false;
   // The CG operator is stored as planes of value pairs (FP64, block SELL).
   bool cg_pairs_ = false;
+  // Kernel nodes per PCG iteration in the captured graph (0: none yet).
+  double graph_nodes_per_iter_ = 0.0;
   int64_t replace_every() const { return cg_vop_ ? opt_.cg_replace_every : 0; }
   // Host wall time of the setup phases (ms), in order, for describe().
   std::vector<std::pair<std::string, double>> setup_ms_;
@@ -886,6 +888,9 @@ void Engine<TC, TM>::capture_graph() {
   GS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < opt_.graph_iters; ++i) encode_iteration(stop);
   GS_HIP(hipStreamEndCapture(stream_, &graph_));
+  size_t nodes = 0;
+  GS_HIP(hipGraphGetNodes(graph_, nullptr, &nodes));
+  graph_nodes_per_iter_ = static_cast<double>(nodes) / opt_.graph_iters;
   GS_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
 }
 
@@ -1192,7 +1197,7 @@ std::string Engine<TC, TM>::describe() const {
      << ", \"cg_pair_layout\": " << (cg_pairs_ ? "true" : "false")
      << ", \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
      << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters
-     << "}";
+     << ", \"launches_per_iteration\": " << graph_nodes_per_iter_ << "}";
   return os.str();
 }
 
