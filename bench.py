@@ -479,7 +479,9 @@ def main() -> None:
             modes[name]["rel_err_vs_cpu_oracle"] = float(np.linalg.norm(xs - xo) / no)
     setup_s = {"hierarchy": round(t_hier, 2),
                "plain_upload": round(modes["plain_fp64"]["setup_ms"] / 1e3, 2),
-               "headline_upload": round(hm["setup_ms"] / 1e3, 2)}
+               "headline_upload": round(hm["setup_ms"] / 1e3, 2),
+               "plain_phases_ms": {k: round(v, 1) for k, v in desc_plain["setup_ms"].items()},
+               "headline_phases_ms": {k: round(v, 1) for k, v in hdesc["setup_ms"].items()}}
     del solvers, sols, hs, plain
 
     configs = {}

@@ -164,8 +164,11 @@ class Engine final : public EngineBase {
       if (k < static_cast<int>(omega.size()) && omega[k] > 0.0 &&
           (lv.kind == kSGS || lv.kind == kBlockGS) && !(omega[k] < 2.0))
         throw std::runtime_error("Gauss-Seidel smoother: omega must be in (0, 2).");
+      // A per-level weight may leave a coarse level unsmoothed (the
+      // V-cycle stays symmetric); the finest level keeps the reference's
+      // rule for symmetric Gauss-Seidel.
       if (k < static_cast<int>(sweeps.size()) && sweeps[k] == 0 && lv.kind == kSGS &&
-          !lv.direct && k + 1 < nlev_)
+          !lv.direct && k == 0 && k + 1 < nlev_)
         throw std::runtime_error("Preconditioner requires at least one smoothing step.");
     }
     if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
@@ -241,8 +244,15 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   int pre = opt_.pre, post = opt_.post;
   if (k < static_cast<int>(opt_.level_omega.size()) && opt_.level_omega[k] > 0.0)
     om = opt_.level_omega[k];
-  if (k < static_cast<int>(opt_.level_sweeps.size()) && opt_.level_sweeps[k] >= 0)
+  // Per-level sweeps (learned weights): 0 leaves a level below the finest
+  // unsmoothed (no symmetric Gauss-Seidel sweep; the V-cycle stays
+  // symmetric). Without them, the reference's rule holds (src/Smoothers/
+  // SymmetricGaussSeidelSmoother.cc:89-92).
+  bool level_off = false;
+  if (k < static_cast<int>(opt_.level_sweeps.size()) && opt_.level_sweeps[k] >= 0) {
     pre = post = opt_.level_sweeps[k];
+    level_off = pre == 0 && k > 0 && !coarsest;
+  }
   lv.omega = static_cast<TM>(om);
   lv.pre = pre;
   lv.post = post;
@@ -252,7 +262,7 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
       if (!std::isfinite(om) || om <= 0.0 || om >= 2.0)
         throw std::runtime_error("SymmetricGaussSeidelSmoother: invalid omega.");
       const int steps_needed = coarsest ? opt_.coarse_steps : std::min(pre, post);
-      if (steps_needed == 0)
+      if (steps_needed == 0 && !level_off)
         throw std::runtime_error("Preconditioner requires at least one smoothing step.");
     }
     if (lv.kind == kBlockGS && (om <= 0.0 || om >= 2.0))

@@ -14,9 +14,9 @@ smoother, w in (0, 2)), so CG remains valid; the plain weights stay the
 fallback (gs_solver_set_fallback).
 
 Usage on a GPU box:
-  python learned.py label --kind sgs --first 0 --count 100 --out labels_sgs.jsonl
+  python learned.py label --kind sgs --first 0 --count 100 --size 64 --out labels_sgs.jsonl
   python learned.py fit --kind sgs --labels labels_sgs.jsonl --out learned_weights.json
-  python learned.py evaluate --kind sgs --weights learned_weights.json
+  python learned.py evaluate --kind sgs --size 160 --weights learned_weights.json
 (label appends one JSON line per seed, so a long label run can be split
 over several invocations; `train` runs label and fit in one go.)
 """
@@ -44,11 +44,12 @@ HIDDEN = 64
 # (MG-GS w = 1, examples/multigridPoisson/mainAMGtest.cc:697-698; block GS
 # w = 0.7, examples/elasticity/cantileverBeam.cc:572-573).
 # coarse_sweeps: the sweep counts tried on levels below the finest (0 turns a
-# level's smoothing off: a no-op block smoother, as the reference's
-# 0-sweep block GS; symmetric GS requires a sweep).
+# level's smoothing off: its restricted residual goes straight down and the
+# prolongated correction straight back, which keeps the V-cycle symmetric;
+# for block GS this is the reference's own 0-sweep smoother).
 KINDS = {
     "sgs": dict(smoother=gs.SGS, plain_omega=1.0, omegas=[0.7, 0.85, 1.0, 1.15, 1.3, 1.45, 1.6],
-                sweeps=[1, 2], coarse_sweeps=[1, 2], block=1, coarse_steps=2),
+                sweeps=[1, 2], coarse_sweeps=[0, 1, 2], block=1, coarse_steps=2),
     "bgs": dict(smoother=gs.BLOCK_GS, plain_omega=0.7,
                 omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2],
                 coarse_sweeps=[0, 1, 2], block=3, coarse_steps=0),
@@ -155,6 +156,8 @@ def sweep_labels(H, b, spec, maxit=400, passes=3):
                 for sw in (spec["sweeps"] if k == 0 else spec["coarse_sweeps"]):
                     if om == omega[k] and sw == sweeps[k]:
                         continue
+                    if sw == 0 and om != spec["plain_omega"]:
+                        continue  # an unsmoothed level has no damping
                     o2, s2 = list(omega), list(sweeps)
                     o2[k], s2[k] = om, sw
                     c, _ = cost(o2, s2)
@@ -228,6 +231,22 @@ def read_labels(kind: str, paths):
     return [seen[k] for k in sorted(seen)]
 
 
+def constant_policy(rows, spec) -> dict:
+    """The best fixed setting the labels support, for comparison with the
+    network: per level index, the most frequent sweep count and the mean
+    damping of the labels that smooth that level."""
+    depth = max(len(r["omega"]) for r in rows)
+    omega, sweeps = [], []
+    for k in range(depth):
+        sw = [r["sweeps"][k] for r in rows if k < len(r["sweeps"])]
+        vals, counts = np.unique(sw, return_counts=True)
+        s_k = int(vals[np.argmax(counts)])
+        om = [r["omega"][k] for r in rows if k < len(r["omega"]) and r["sweeps"][k] > 0]
+        omega.append(float(np.mean(om)) if (om and s_k > 0) else spec["plain_omega"])
+        sweeps.append(s_k)
+    return {"omega": omega, "sweeps": sweeps}
+
+
 def fit(kind: str, label_paths, epochs: int, out: Path) -> dict:
     """Trains the MLP on the GPU (HIP forward / backward / Adam) on the
     labelled levels and stores it under `kind` in `out`."""
@@ -246,9 +265,11 @@ def fit(kind: str, label_paths, epochs: int, out: Path) -> dict:
     print(f"[fit {kind}] seeds {len(rows)} samples {X.shape[0]} loss {l0:.4f} -> {loss:.4f}",
           flush=True)
     res = {"hidden": HIDDEN, "params": net.get_params().tolist(), "samples": int(X.shape[0]),
-           "seeds": len(rows), "size": rows[0]["size"], "epochs": epochs, "loss_initial": l0,
-           "loss_final": loss,
-           "label_time_gain_mean": float(np.mean([r["plain_ms"] / r["best_ms"] for r in rows]))}
+           "seeds": len(rows), "size": rows[0]["size"],
+           "sizes": sorted({int(r["size"]) for r in rows}), "epochs": epochs,
+           "loss_initial": l0, "loss_final": loss,
+           "label_time_gain_mean": float(np.mean([r["plain_ms"] / r["best_ms"] for r in rows])),
+           "constant_policy": constant_policy(rows, spec)}
     doc = json.loads(out.read_text()) if out.exists() else {}
     doc[kind] = res
     out.write_text(json.dumps(doc))
@@ -263,31 +284,65 @@ def train(kind: str, seeds: int, size: int, epochs: int, out: Path) -> dict:
     return fit(kind, [lab], epochs, out)
 
 
+def constant_weights(kind: str, L: int, path: Path = MODEL_PATH):
+    """The stored constant policy extended to L levels (plain weights past
+    the labelled depth)."""
+    spec = KINDS[kind]
+    c = json.loads(Path(path).read_text())[kind].get("constant_policy")
+    if c is None:
+        return None
+    om = (list(c["omega"]) + [spec["plain_omega"]] * L)[:L]
+    sw = (list(c["sweeps"]) + [1] * L)[:L]
+    sw[0] = max(sw[0], 1)
+    return om, sw
+
+
 def evaluate(kind: str, seeds, size: int, path: Path) -> dict:
-    """Held-out comparison: plain weights vs learned weights, both mixed."""
+    """Held-out comparison, all in the mixed headline configuration: plain
+    weights, the network's weights, and the best constant policy of the
+    labels. Each entry is the better of two timed solves."""
     spec = KINDS[kind]
     rows = []
     for seed in seeds:
         A, b, H = training_problem(kind, seed, size)
         L = H.levels
-        plain = make_solver(H, spec, [spec["plain_omega"]] * L, [1] * L)
         om, sw = predict(H, kind, path)
-        learned = make_solver(H, spec, om.tolist(), sw.tolist())
+        policies = [("plain", [spec["plain_omega"]] * L, [1] * L),
+                    ("learned", om.tolist(), sw.tolist())]
+        cw = constant_weights(kind, L, path)
+        if cw is not None:
+            policies.append(("constant", cw[0], cw[1]))
         nnz = level_nnz(H)
         res = {}
-        for name, s, swk in (("plain", plain, [1] * L), ("learned", learned, sw.tolist())):
+        for name, omk, swk in policies:
+            s = make_solver(H, spec, omk, swk)
             s.solve(b, tol=1e-8, maxit=1000)
-            x, info = s.solve(b, tol=1e-8, maxit=1000)
-            res[name] = dict(iterations=info.iterations, ms=info.solve_ms,
-                             work=(info.iterations + 1) * work_per_iteration(nnz, swk),
-                             true_relres=info.true_relres)
+            best = None
+            for _ in range(2):
+                x, info = s.solve(b, tol=1e-8, maxit=1000)
+                if best is None or info.solve_ms < best.solve_ms:
+                    best = info
+            used = len(s.describe()["levels"])
+            res[name] = dict(iterations=best.iterations, ms=best.solve_ms,
+                             work=(best.iterations + 1) * work_per_iteration(nnz, swk),
+                             true_relres=best.true_relres, converged=best.converged,
+                             omega=[round(float(v), 3) for v in omk[:used - 1]],
+                             sweeps=[int(v) for v in swk[:used - 1]])
+            del s
         rows.append(res)
-        print(f"[eval {kind}] seed {seed}: plain it {res['plain']['iterations']} "
-              f"{res['plain']['ms']:.2f} ms | learned it {res['learned']['iterations']} "
-              f"{res['learned']['ms']:.2f} ms", flush=True)
+        print(f"[eval {kind} {size}] seed {seed}: " + " | ".join(
+            f"{k} it {v['iterations']} {v['ms']:.2f} ms" for k, v in res.items()), flush=True)
     agg = {k: float(np.mean([r["plain"][k] / r["learned"][k] for r in rows]))
            for k in ("iterations", "ms", "work")}
-    return {"per_seed": rows, "mean_ratio_plain_over_learned": agg}
+    out = {"per_seed": rows, "mean_ratio_plain_over_learned": agg}
+    if all("constant" in r for r in rows):
+        out["mean_ratio_constant_over_learned"] = {
+            k: float(np.mean([r["constant"][k] / r["learned"][k] for r in rows]))
+            for k in ("iterations", "ms")}
+        out["mean_ratio_plain_over_constant"] = {
+            k: float(np.mean([r["plain"][k] / r["constant"][k] for r in rows]))
+            for k in ("iterations", "ms")}
+    return out
 
 
 def main():
@@ -295,15 +350,21 @@ def main():
     ap.add_argument("cmd", choices=["label", "fit", "train", "evaluate"])
     ap.add_argument("--kind", choices=sorted(KINDS), default="sgs")
     ap.add_argument("--seeds", type=int, default=200)
-    ap.add_argument("--first", type=int, default=0)
-    ap.add_argument("--count", type=int, default=10)
+    ap.add_argument("--first", type=int, default=-1,
+                    help="first seed (label: 0, evaluate: 200, the held-out seeds)")
+    ap.add_argument("--count", type=int, default=-1, help="seeds (label: 10, evaluate: 20)")
     ap.add_argument("--labels", nargs="*", default=[])
     ap.add_argument("--size", type=int, default=0)
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--out", default=str(MODEL_PATH))
     ap.add_argument("--weights", default=str(MODEL_PATH))
+    ap.add_argument("--json-out", default="", help="evaluate: write the full result here")
     a = ap.parse_args()
     size = a.size or DEFAULT_SIZE[a.kind]
+    if a.first < 0:
+        a.first = 200 if a.cmd == "evaluate" else 0
+    if a.count < 0:
+        a.count = 20 if a.cmd == "evaluate" else 10
     if a.cmd == "label":
         label(a.kind, range(a.first, a.first + a.count), size, Path(a.out))
     elif a.cmd == "fit":
@@ -311,8 +372,10 @@ def main():
     elif a.cmd == "train":
         train(a.kind, a.seeds, size, a.epochs, Path(a.out))
     else:
-        r = evaluate(a.kind, range(200, 220), size, Path(a.weights))
-        print(json.dumps(r["mean_ratio_plain_over_learned"]))
+        r = evaluate(a.kind, range(a.first, a.first + a.count), size, Path(a.weights))
+        if a.json_out:
+            Path(a.json_out).write_text(json.dumps(r))
+        print(json.dumps({k: v for k, v in r.items() if k != "per_seed"}))
 
 
 if __name__ == "__main__":

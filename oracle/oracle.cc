@@ -255,6 +255,7 @@ struct Level {
   int kind = kSGS;
   T omega = T(1);
   int pre = 1, post = 1;
+  bool unsmoothed = false;  // a level below the finest given 0 sweeps
   Coloring col;            // SGS: scalar colouring; block GS: block colouring
   std::vector<T> dinv;     // Jacobi: 1/a_ii; block: 9 per block
   std::vector<T> rhs, res, corr, az, tmp;
@@ -282,6 +283,10 @@ template <class T>
 void smooth(Level<T> &lv, const T *rhs, T *x, int iters) {
   const int64_t n = lv.A.rows;
   if (lv.kind == kSGS) {
+    // src/Smoothers/SymmetricGaussSeidelSmoother.cc:89-92; a level below
+    // the finest whose per-level sweep count is 0 (learned weights, no
+    // counterpart in the reference) is left unsmoothed instead.
+    if (iters == 0 && lv.unsmoothed) return;
     if (iters == 0)
       throw std::runtime_error("Preconditioner requires at least one smoothing step.");
     for (int it = 0; it < iters; ++it) {
@@ -295,6 +300,14 @@ void smooth(Level<T> &lv, const T *rhs, T *x, int iters) {
           throw std::runtime_error("CPUBackend::encodeGaussSeidelColor: zero diagonal entry.");
     }
   } else if (lv.kind == kBlockGS) {
+    // Each colour's full block range [offset_c, offset_c+1), as the Metal
+    // backend sweeps it (src/metal/MetalBackend.cc:719-760, which reads the
+    // pair the smoother passes as (start, count)). The smoother passes
+    // (begin, count) (src/Smoothers/BlockGaussSeidelSmoother.cc:155-166,
+    // 196-207) into the CPU backend's (begin, end) (src/CPU/CPUBackend.cc:
+    // 364), whose literal loop would sweep colour 0 and a prefix of colour 1
+    // and skip the rest; this oracle, like the GPU engine, follows the
+    // intended full-colour semantics instead.
     for (int it = 0; it < iters; ++it) {
       for (int c = 0; c < lv.col.ncolors; ++c)
         if (lv.col.offsets[c + 1] > lv.col.offsets[c])
@@ -778,6 +791,7 @@ static void build_mg(MG<TM> &mg, const OrcHier &h, const int32_t *kinds, const d
     lv.omega = static_cast<TM>(omegas[k]);
     lv.pre = pres[k];
     lv.post = posts[k];
+    lv.unsmoothed = k > 0 && k + 1 < nl && pres[k] == 0 && posts[k] == 0;
     if (lv.kind == kSGS) {
       if (!(lv.omega > TM(0) && lv.omega < TM(2)))
         throw std::runtime_error("SymmetricGaussSeidelSmoother: invalid omega.");

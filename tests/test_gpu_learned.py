@@ -79,17 +79,26 @@ def test_guarded_solve_falls_back_on_bad_weights():
     np.testing.assert_array_equal(x, xp)
 
 
+# Held-out device-time gain each model must show over the plain weights
+# (mean over seeds 200-202 at the model's training size, both in the mixed
+# headline configuration): measured 2.46x (block GS, 48^3) and 1.77x (SGS,
+# 128^3) over seeds 200-219.
+GAIN_BAR = {"bgs": 1.5, "sgs": 1.3}
+
+
 def test_learned_weights_gain_over_plain():
-    """Held-out seeds (never labelled): with the trained network's weights
-    the mixed solve is, on average, no slower than with the plain weights
-    (device time, 3% allowance for timing noise), and every learned solve
-    still reaches 1e-8 in FP64."""
+    """Held-out seeds (never labelled): the trained network's weights make
+    the mixed solve faster than the plain weights by at least the bar above
+    (device time), every learned solve still reaches 1e-8 in FP64, and the
+    network is compared with the best constant policy of its labels (it
+    must not be more than 3% slower)."""
     import learned
     if not learned.MODEL_PATH.exists():
         pytest.skip("no trained weights committed yet")
     doc = json.loads(learned.MODEL_PATH.read_text())
     for kind in doc:
         r = learned.evaluate(kind, range(200, 203), doc[kind]["size"], learned.MODEL_PATH)
-        assert r["mean_ratio_plain_over_learned"]["ms"] >= 0.97, r
+        assert r["mean_ratio_plain_over_learned"]["ms"] >= GAIN_BAR[kind], (kind, r)
+        assert r["mean_ratio_constant_over_learned"]["ms"] >= 0.97, (kind, r)
         for row in r["per_seed"]:
-            assert row["learned"]["true_relres"] <= 1.5e-8, row
+            assert row["learned"]["converged"] and row["learned"]["true_relres"] <= 1.5e-8, row

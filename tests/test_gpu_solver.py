@@ -520,3 +520,26 @@ def test_pcg_parity_heterogeneous(hetero_small, prec):
                                         coarse_steps=kw["coarse_steps"])
         assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
         assert true <= 1.5e-8
+
+
+@pytest.mark.parametrize("sweeps", [[1, 0, 1], [1, 0, 0], [2, 1, 0]])
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+def test_vcycle_bitexact_unsmoothed_sgs_levels(cfg1, prec, sweeps):
+    """Per-level weights that leave coarse symmetric Gauss-Seidel levels
+    unsmoothed (0 sweeps: the restricted residual goes straight down and the
+    prolongated correction straight back): bit for bit the oracle's V-cycle
+    with the same per-level sweeps; the finest level still refuses 0."""
+    A, H = cfg1
+    L = H.levels
+    sw = sweeps + [1] * (L - len(sweeps))
+    om = [1.1, 0.9, 1.2] + [1.0] * (L - 3)
+    s = gs.Solver(prec, smoother=gs.SGS, coarse_steps=2)
+    s.setup(H)
+    s.set_level_weights(omega=om, sweeps=sw)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(19, A.rows)
+    zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=gs.SGS, omegas=om, pres=sw,
+                    posts=sw, coarse_steps=2)
+    np.testing.assert_array_equal(s.vcycle(r), zo)
+    with pytest.raises(gs.GpuSolverError, match="at least one smoothing step"):
+        s.set_level_weights(sweeps=[0] + sw[1:])
