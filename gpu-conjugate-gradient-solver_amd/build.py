@@ -35,7 +35,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contr
 
 HOST_SOURCES = ["host/sparse.cc", "host/ordering.cc", "host/amg.cc", "host/problems.cc",
                 "host/io.cc"]
-HIP_SOURCES = ["engine.hip", "backend.hip", "capi.cc", "mlp.hip", "dense.hip", "noalloc.hip"]
+HIP_SOURCES = ["engine.hip", "backend.hip", "capi.cc", "mlp.hip", "dense.hip", "noalloc.hip",
+               "setup_device.hip"]
 
 
 def _run(cmd: list[str]) -> None:

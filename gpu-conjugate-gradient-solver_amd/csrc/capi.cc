@@ -13,6 +13,7 @@
 #include "dense.hpp"
 #include "engine.hpp"
 #include "host/setup.h"
+#include "setup_device.hpp"
 
 struct gs_csr_s {
   gsx::Csr A;
@@ -351,22 +352,63 @@ int gs_gaussian_field(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double 
   });
 }
 
+namespace {
+gsx::Hierarchy build_sa(gs_csr_h A, int32_t bs, double eps, int32_t max_levels, int64_t min_dofs,
+                        const double *ns, int32_t ncols, const gsx::GalerkinOps *ops) {
+  need(A, "matrix");
+  gsx::SAParams prm;
+  prm.block_size = bs;
+  prm.eps_strong = eps;
+  prm.max_levels = max_levels;
+  prm.min_dofs = min_dofs;
+  std::vector<double> B;
+  if (ncols > 0) {
+    need(ns, "nullspace");
+    B.assign(ns, ns + A->A.rows * ncols);
+  }
+  return gsx::build_sa_hierarchy(A->A, prm, B, ncols, ops);
+}
+}  // namespace
+
 int gs_hierarchy_build_sa(gs_csr_h A, int32_t bs, double eps, int32_t max_levels, int64_t min_dofs,
                           const double *ns, int32_t ncols, gs_hier_h *out) {
   return guard([&] {
-    need(A, "matrix");
-    gsx::SAParams prm;
-    prm.block_size = bs;
-    prm.eps_strong = eps;
-    prm.max_levels = max_levels;
-    prm.min_dofs = min_dofs;
-    std::vector<double> B;
-    if (ncols > 0) {
-      need(ns, "nullspace");
-      B.assign(ns, ns + A->A.rows * ncols);
-    }
+    need(out, "out");
     auto h = std::make_unique<gs_hier_s>();
-    h->H = gsx::build_sa_hierarchy(A->A, prm, B, ncols);
+    h->H = build_sa(A, bs, eps, max_levels, min_dofs, ns, ncols, nullptr);
+    *out = h.release();
+  });
+}
+
+int gs_hierarchy_build_sa_device(gs_csr_h A, int32_t bs, double eps, int32_t max_levels,
+                                 int64_t min_dofs, const double *ns, int32_t ncols,
+                                 gs_hier_h *out) {
+  return guard([&] {
+    need(out, "out");
+    const gsdev::DeviceGalerkinOps ops;
+    auto h = std::make_unique<gs_hier_s>();
+    h->H = build_sa(A, bs, eps, max_levels, min_dofs, ns, ncols, &ops);
+    *out = h.release();
+  });
+}
+
+int gs_csr_spgemm(gs_csr_h A, gs_csr_h B, int32_t on_device, gs_csr_h *out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(B, "matrix");
+    need(out, "out");
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = on_device ? gsdev::device_spgemm(A->A, B->A) : gsx::spgemm(A->A, B->A);
+    *out = h.release();
+  });
+}
+
+int gs_csr_transpose(gs_csr_h A, int32_t on_device, gs_csr_h *out) {
+  return guard([&] {
+    need(A, "matrix");
+    need(out, "out");
+    auto h = std::make_unique<gs_csr_s>();
+    h->A = on_device ? gsdev::device_transpose(A->A) : gsx::transpose(A->A);
     *out = h.release();
   });
 }

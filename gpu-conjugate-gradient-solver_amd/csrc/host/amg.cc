@@ -232,8 +232,44 @@ struct CoarsenResult {
   std::vector<double> nullspace;  // coarse nullspace (nc x ncols row-major)
 };
 
+// The prolongation smoother M = I - w D_f^-1 A_f as an explicit matrix: row
+// i keeps, in A's entry order, its diagonal entry (value 1 - w) and its
+// strong off-diagonal entries (value -w a_ij / d_i), d_i being the filtered
+// diagonal (diagonal plus the weak entries lumped into it). The products
+// of M P_tent then run in the same order as a fused loop over A's entries.
+Csr smoothing_operator(const Csr &A, const std::vector<char> &strong, double omega) {
+  const int64_t n = A.rows;
+  Csr M;
+  M.rows = n;
+  M.cols = A.cols;
+  M.ptr.assign(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    int32_t cnt = 0;
+    for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) cnt += (A.col[k] == i || strong[k]) ? 1 : 0;
+    M.ptr[i + 1] = M.ptr[i] + cnt;
+  }
+  M.col.resize(M.ptr[n]);
+  M.val.resize(M.ptr[n]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t i = 0; i < n; ++i) {
+    double dia = 0.0;
+    for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+      if (A.col[k] == i || !strong[k]) dia += A.val[k];
+    const double scale = -omega / dia;
+    int32_t d = M.ptr[i];
+    for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+      const int32_t ca = A.col[k];
+      if (ca != i && !strong[k]) continue;
+      M.col[d] = ca;
+      M.val[d] = (ca == i) ? (1.0 - omega) : scale * A.val[k];
+      ++d;
+    }
+  }
+  return M;
+}
+
 CoarsenResult coarsen(const Csr &A, const SAParams &prm,
-                      const std::vector<double> &B, int32_t ncols) {
+                      const std::vector<double> &B, int32_t ncols, const GalerkinOps &ops) {
   const int32_t bs = prm.block_size;
   if (A.rows % bs != 0)
     throw std::runtime_error(
@@ -340,61 +376,29 @@ CoarsenResult coarsen(const Csr &A, const SAParams &prm,
       }
     }
   }
-  // ---- prolongation smoothing -------------------------------------------------
+  // ---- prolongation smoothing and Galerkin product ----------------------------
+  // P = M P_tent with M = I - w D_f^-1 A_f, then R = P^T and A_c = R (A P).
   const double omega = prm.relax * (2.0 / 3.0);
-  Csr P;
-  P.rows = n;
-  P.cols = Pt.cols;
-  P.ptr.assign(n + 1, 0);
-  std::vector<std::vector<std::pair<int32_t, double>>> prow(n);
-#pragma omp parallel
-  {
-    std::vector<int32_t> mark(Pt.cols, -1);
-#pragma omp for schedule(dynamic, 1024)
-    for (int64_t i = 0; i < n; ++i) {
-      double dia = 0.0;
-      for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
-        if (A.col[k] == i || !strong[k]) dia += A.val[k];
-      const double scale = -omega / dia;
-      auto &row = prow[i];
-      for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
-        const int32_t ca = A.col[k];
-        if (ca != i && !strong[k]) continue;
-        const double va = (ca == i) ? (1.0 - omega) : scale * A.val[k];
-        for (int32_t kp = Pt.ptr[ca]; kp < Pt.ptr[ca + 1]; ++kp) {
-          const int32_t cp = Pt.col[kp];
-          if (mark[cp] < 0) {
-            mark[cp] = static_cast<int32_t>(row.size());
-            row.emplace_back(cp, va * Pt.val[kp]);
-          } else {
-            row[mark[cp]].second += va * Pt.val[kp];
-          }
-        }
-      }
-      for (auto &e : row) mark[e.first] = -1;
-      std::sort(row.begin(), row.end());
-    }
-  }
-  for (int64_t i = 0; i < n; ++i) P.ptr[i + 1] = P.ptr[i] + static_cast<int32_t>(prow[i].size());
-  P.col.resize(P.ptr[n]);
-  P.val.resize(P.ptr[n]);
-  for (int64_t i = 0; i < n; ++i) {
-    int64_t d = P.ptr[i];
-    for (auto &e : prow[i]) { P.col[d] = e.first; P.val[d] = e.second; ++d; }
-    std::vector<std::pair<int32_t, double>>().swap(prow[i]);
-  }
-  res.R = transpose(P);
-  Csr AP = spgemm(A, P);
-  res.Ac = spgemm(res.R, AP);
-  res.P = std::move(P);
+  const Csr M = smoothing_operator(A, strong, omega);
+  ops.run(A, M, Pt, res.P, res.R, res.Ac);
   return res;
 }
 
 }  // namespace
 
+void GalerkinOps::run(const Csr &A, const Csr &M, const Csr &Pt, Csr &P, Csr &R,
+                      Csr &Ac) const {
+  P = spgemm(M, Pt);
+  R = transpose(P);
+  const Csr AP = spgemm(A, P);
+  Ac = spgemm(R, AP);
+}
+
 Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
                              const std::vector<double> &nullspace,
-                             int32_t ncols) {
+                             int32_t ncols, const GalerkinOps *ops) {
+  const GalerkinOps host_ops;
+  const GalerkinOps &gops = ops ? *ops : host_ops;
   if (A.rows != A.cols) throw std::runtime_error("AMGHierarchyBuilder: matrix must be square.");
   if (prm.max_levels <= 0) throw std::runtime_error("AMGHierarchyBuilder: maxLevels must be positive.");
   if (prm.min_dofs <= 0) throw std::runtime_error("AMGHierarchyBuilder: minDofs must be positive.");
@@ -409,7 +413,7 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
     if (cur.rows <= prm.min_dofs) break;
     CoarsenResult r;
     try {
-      r = coarsen(cur, lp, B, ncols);
+      r = coarsen(cur, lp, B, ncols, gops);
     } catch (const EmptyLevel &) {
       break;
     }

@@ -237,9 +237,20 @@ struct Hierarchy {
   std::vector<std::vector<int32_t>> aggregates;  // per transfer, fine DOF -> aggregate (or -1)
 };
 // nullspace: n x ncols row-major near-nullspace (may be empty).
+// The sparse products of one coarsening step: P = M P_tent (M the
+// prolongation smoother as a matrix), R = P^T and A_c = R (A P). The base
+// class runs them on the host (spgemm, transpose); a GPU implementation
+// (gsdev::DeviceGalerkinOps) overrides run() with the same results bit for
+// bit.
+struct GalerkinOps {
+  virtual ~GalerkinOps() = default;
+  virtual void run(const Csr &A, const Csr &M, const Csr &Pt, Csr &P, Csr &R, Csr &Ac) const;
+};
+
+// ops: where the products of each coarsening step run (null: the host).
 Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
                              const std::vector<double> &nullspace,
-                             int32_t ncols);
+                             int32_t ncols, const GalerkinOps *ops = nullptr);
 
 // Binary files (layout in io.cc): one CSR matrix, or a whole hierarchy
 // (A, P, R and the aggregate maps). Loading validates every record.

@@ -70,6 +70,10 @@ _SIGS = {
     "gs_gaussian_field": (C.c_int, [_i32, _i32, _i32, C.c_uint64, C.c_double, _f64p]),
     "gs_hierarchy_build_sa": (C.c_int, [_vp, _i32, C.c_double, _i32, _i64, _vp, _i32,
                                         C.POINTER(_vp)]),
+    "gs_hierarchy_build_sa_device": (C.c_int, [_vp, _i32, C.c_double, _i32, _i64, _vp, _i32,
+                                               C.POINTER(_vp)]),
+    "gs_csr_spgemm": (C.c_int, [_vp, _vp, _i32, C.POINTER(_vp)]),
+    "gs_csr_transpose": (C.c_int, [_vp, _i32, C.POINTER(_vp)]),
     "gs_hierarchy_create": (C.c_int, [C.POINTER(_vp)]),
     "gs_problem_elasticity_mesh": (C.c_int, [_i64, _f64p, _i64, _i32p, C.c_double, C.c_double, _vp,
                                              C.c_double, C.c_double, C.c_double, C.POINTER(_vp),
@@ -385,6 +389,18 @@ def gaussian_field(nx: int, ny: int, nz: int, seed: int, sigma: float) -> np.nda
 # ---------------------------------------------------------------------------
 # Setup artifacts
 # ---------------------------------------------------------------------------
+def spgemm(A: Csr, B: Csr, device: bool = False) -> Csr:
+    """C = A B, rows sorted (gs_csr_spgemm); host or GPU, same bits."""
+    ha, hb = A.handle(), B.handle()  # alive for the call
+    return _from_handle_fn(lib.gs_csr_spgemm, ha.h, hb.h, int(device))
+
+
+def transpose(A: Csr, device: bool = False) -> Csr:
+    """A^T, rows sorted (gs_csr_transpose); host or GPU, same bits."""
+    ha = A.handle()  # alive for the call
+    return _from_handle_fn(lib.gs_csr_transpose, ha.h, int(device))
+
+
 def sell_layout(A: Csr):
     """(slots, min column, max column) of A packed as SELL-64; raises when
     the packed layout would break a kernel's assumptions."""
@@ -506,7 +522,11 @@ class Hierarchy:
     @classmethod
     def smoothed_aggregation(cls, A: Csr, block_size: int = 1, eps_strong: float = 0.1,
                              max_levels: int = 10, min_dofs: int = 1000,
-                             nullspace: np.ndarray | None = None) -> "Hierarchy":
+                             nullspace: np.ndarray | None = None,
+                             device: bool = False) -> "Hierarchy":
+        """SA hierarchy (gs_hierarchy_build_sa); device=True forms the
+        products of each coarsening step on the GPU (the same hierarchy bit
+        for bit, gs_hierarchy_build_sa_device)."""
         ah = A.handle()
         h = _vp()
         ncols = 0
@@ -516,8 +536,8 @@ class Hierarchy:
             ncols = ns.shape[1]
             nsp = ns.ctypes.data_as(_vp)
             keep = ns  # noqa: F841  (keeps the buffer alive during the call)
-        check(lib.gs_hierarchy_build_sa(ah.h, block_size, eps_strong, max_levels, min_dofs, nsp,
-                                        ncols, C.byref(h)))
+        build = lib.gs_hierarchy_build_sa_device if device else lib.gs_hierarchy_build_sa
+        check(build(ah.h, block_size, eps_strong, max_levels, min_dofs, nsp, ncols, C.byref(h)))
         return cls(h)
 
     @classmethod

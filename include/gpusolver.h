@@ -147,6 +147,23 @@ typedef struct gs_hier_s *gs_hier_h;
 int gs_hierarchy_build_sa(gs_csr_h A, int32_t block_size, double eps_strong, int32_t max_levels,
                           int64_t min_dofs, const double *nullspace, int32_t nullspace_cols,
                           gs_hier_h *out);
+/* The same hierarchy, bit for bit, with the sparse products of each
+ * coarsening step on the GPU: the prolongation smoothing P = (I - w
+ * D_f^-1 A_f) P_tent, R = P^T and the Galerkin product A_c = R A P
+ * (amgcl's transfer_operators / coarse_operator, examples/amgSolver/src/
+ * SmoothedAggregationCoarsener.cc:93-97). Aggregation stays on the host:
+ * it is sequential by definition. */
+int gs_hierarchy_build_sa_device(gs_csr_h A, int32_t block_size, double eps_strong,
+                                 int32_t max_levels, int64_t min_dofs, const double *nullspace,
+                                 int32_t nullspace_cols, gs_hier_h *out);
+/* C = A B with rows sorted by column, each entry summed in the order
+ * A's row entries and then B's visit it (the products of the
+ * smoothed-aggregation setup, above); on_device 0: host, 1: GPU, the
+ * same result bit for bit. */
+int gs_csr_spgemm(gs_csr_h A, gs_csr_h B, int32_t on_device, gs_csr_h *out);
+/* A^T with rows sorted by column (R = P^T of the setup); on_device as for
+ * gs_csr_spgemm. */
+int gs_csr_transpose(gs_csr_h A, int32_t on_device, gs_csr_h *out);
 int gs_hierarchy_create(gs_hier_h *out);
 int gs_hierarchy_push_level(gs_hier_h h, gs_csr_h A);
 int gs_hierarchy_push_transfer(gs_hier_h h, gs_csr_h P, gs_csr_h R);

@@ -203,3 +203,22 @@ def test_hetero96_mixed_learned(hetero96):
                             kinds=gs.BLOCK_GS, omegas=omega, pres=sweeps, posts=sweeps,
                             coarse_steps=0, direct_coarse=True, coarse_inv=inv)
     check(x, info, xo, ito, A, b)
+
+
+def test_hetero96_device_hierarchy(hetero96):
+    """The benchmark's hierarchy built with the GPU-side products equals the
+    host-built one bit for bit (every A, P, R and aggregate map)."""
+    import time
+    A, H, b = hetero96
+    say("hetero_elasticity_96 device-built hierarchy")
+    t0 = time.perf_counter()
+    Hd = gs.Hierarchy.smoothed_aggregation(A, block_size=3, eps_strong=0.1, max_levels=10,
+                                           min_dofs=5000, device=True)
+    say(f"device-built hierarchy in {time.perf_counter() - t0:.2f} s")
+    assert Hd.levels == H.levels
+    for w, L in (("A", H.levels), ("P", H.levels - 1), ("R", H.levels - 1)):
+        for k in range(L):
+            x, y = Hd.get(w, k), H.get(w, k)
+            np.testing.assert_array_equal(x.ptr, y.ptr)
+            np.testing.assert_array_equal(x.col, y.col)
+            np.testing.assert_array_equal(x.val.view(np.uint64), y.val.view(np.uint64))

@@ -1,0 +1,90 @@
+"""GPU side of the smoothed-aggregation setup against the host, bit for bit:
+the sparse products (gs_csr_spgemm, gs_csr_transpose with on_device = 1)
+and whole hierarchies (gs_hierarchy_build_sa_device). The host products are
+checked against scipy on the CPU (test_setup_artifacts.py). GPU."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import gpusolver as gs
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a: gs.Csr, b: gs.Csr):
+    assert a.shape == b.shape
+    np.testing.assert_array_equal(a.ptr, b.ptr)
+    np.testing.assert_array_equal(a.col, b.col)
+    np.testing.assert_array_equal(a.val.view(np.uint64), b.val.view(np.uint64))
+
+
+def rand_csr(m, n, density, seed, empty_rows=()):
+    rng = np.random.default_rng(seed)
+    M = sp.random(m, n, density=density, format="lil", random_state=seed,
+                  data_rvs=lambda k: rng.uniform(-1, 1, k))
+    for r in empty_rows:
+        M[r, :] = 0
+    M = M.tocsr()
+    M.eliminate_zeros()
+    M.sort_indices()
+    return gs.Csr.from_scipy(M)
+
+
+CASES = [
+    ("square", (300, 300, 0.02), (300, 300, 0.02)),
+    ("rectangular", (500, 120, 0.03), (120, 80, 0.05)),
+    ("empty rows", (200, 150, 0.02), (150, 200, 0.03)),
+    ("long rows", (40, 2000, 0.5), (2000, 3000, 0.02)),  # ~20 000 products per row
+]
+
+
+@pytest.mark.parametrize("name,sa,sb", CASES, ids=[c[0] for c in CASES])
+def test_device_spgemm_and_transpose_match_host(name, sa, sb):
+    A = rand_csr(*sa, seed=1, empty_rows=(0, 5, 17) if name == "empty rows" else ())
+    B = rand_csr(*sb, seed=2, empty_rows=(3, 4) if name == "empty rows" else ())
+    C_host = gs.spgemm(A, B)
+    same(gs.spgemm(A, B, device=True), C_host)
+    same(gs.transpose(A, device=True), gs.transpose(A))
+    same(gs.transpose(C_host, device=True), gs.transpose(C_host))
+
+
+def test_device_spgemm_signed_zero_first_product():
+    """A column whose first product is -0.0 keeps it (the sum starts from
+    the first product, not from +0), as on the host."""
+    A = gs.Csr.from_scipy(sp.csr_matrix(np.array([[-1.0, 2.0]])))
+    B = gs.Csr(np.array([0, 1, 2], np.int32), np.array([0, 0], np.int32),
+               np.array([0.0, -0.0]), (2, 1))
+    C = gs.spgemm(A, B, device=True)
+    same(C, gs.spgemm(A, B))
+    assert np.signbit(C.val[0])  # (-1 * 0) + (2 * -0) = -0 + -0 = -0
+
+
+def _hier_same(Hd, Hh):
+    assert Hd.levels == Hh.levels
+    for k in range(Hh.levels):
+        same(Hd.get("A", k), Hh.get("A", k))
+    for k in range(Hh.levels - 1):
+        same(Hd.get("P", k), Hh.get("P", k))
+        same(Hd.get("R", k), Hh.get("R", k))
+        np.testing.assert_array_equal(Hd.aggregates(k), Hh.aggregates(k))
+
+
+@pytest.mark.parametrize("problem", ["elasticity", "elasticity_nullspace", "varcoef", "poisson2d"])
+def test_device_hierarchy_matches_host(problem):
+    kw = {}
+    if problem.startswith("elasticity"):
+        A, _, ns = gs.elasticity(16, young=1.0, poisson=0.3, hetero_sigma=2.0, hetero_scale=1.5,
+                                 seed=3)
+        kw = dict(block_size=3, min_dofs=300)
+        if problem.endswith("nullspace"):
+            kw["nullspace"] = ns
+    elif problem == "varcoef":
+        A = gs.varcoef3d(40, seed=4, sigma=3.0, scale=2.0)
+        kw = dict(block_size=1, min_dofs=200)
+    else:
+        A = gs.poisson2d(100, 100)
+        kw = dict(block_size=1, min_dofs=100)
+    Hh = gs.Hierarchy.smoothed_aggregation(A, **kw)
+    Hd = gs.Hierarchy.smoothed_aggregation(A, device=True, **kw)
+    assert Hh.levels >= 3
+    _hier_same(Hd, Hh)

@@ -291,3 +291,25 @@ def test_sell_layout_diagonal_only_rows():
             M[i, i + 1] = -1.0
     slots, lo, hi = gs.sell_layout(gs.Csr.from_scipy(M.tocsr()))
     assert slots > M.nnz and hi < 70
+
+
+def test_host_spgemm_and_transpose_match_scipy():
+    """gs_csr_spgemm / gs_csr_transpose on the host (the products of the
+    smoothed-aggregation setup): structure equal to scipy's, values equal to
+    rounding, rows sorted; rectangular shapes and empty rows included."""
+    import scipy.sparse as sp
+    A = sp.random(60, 40, density=0.1, format="lil", random_state=1)
+    A[7, :] = 0
+    A = A.tocsr()
+    B = sp.random(40, 30, density=0.15, format="csr", random_state=2)
+    for M in (A, B):
+        M.sort_indices()
+    a, b = gs.Csr.from_scipy(A), gs.Csr.from_scipy(B)
+    C = gs.spgemm(a, b).to_scipy()
+    ref = (A @ B).tocsr()
+    ref.sort_indices()
+    assert (abs(C - ref) > 1e-14).nnz == 0
+    assert all(np.all(np.diff(C.indices[C.indptr[i]:C.indptr[i + 1]]) > 0) for i in range(60))
+    T = gs.transpose(a).to_scipy()
+    np.testing.assert_array_equal(T.toarray(), A.T.toarray())
+    assert T.shape == (40, 60)
