@@ -302,7 +302,7 @@ def other_config(name, steps, warmup, graph_iters, with_oracle):
     import gpusolver as gs
     t0 = time.perf_counter()
     A, b, hk, sk, desc = build_workload(name)
-    H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
+    H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, device=True, **hk)
     t_hier = (time.perf_counter() - t0) * 1e3
     modes, solvers, sols = run_modes(name, H, b, sk, steps, warmup, graph_iters,
                                      ("plain_fp64", "mixed", "mixed_learned"))
@@ -404,11 +404,19 @@ def main() -> None:
         return
     import gpusolver as gs
 
-    t0 = time.perf_counter()
     A, b, hk, sk, desc = build_workload(a.config)
-    H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
+    # The hierarchy with the products of each coarsening step on the GPU
+    # (the one the solvers use), and the host-only build timed beside it
+    # (the same hierarchy bit for bit).
+    t0 = time.perf_counter()
+    H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, device=True, **hk)
     t_hier = time.perf_counter() - t0
-    progress(f"hierarchy built: {H.levels} levels, {t_hier:.1f} s")
+    t0 = time.perf_counter()
+    Hh = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, **hk)
+    t_hier_host = time.perf_counter() - t0
+    del Hh
+    progress(f"hierarchy built: {H.levels} levels, {t_hier:.1f} s with GPU products, "
+             f"{t_hier_host:.1f} s on the host")
 
     modes, solvers, sols = run_modes(a.config, H, b, sk, a.steps, a.warmup, a.graph_iters,
                                      ("plain_fp64", "fp64_direct", "mixed", "mixed_learned"))
@@ -477,7 +485,7 @@ def main() -> None:
         no = float(np.linalg.norm(xo))
         for name, xs in sols.items():
             modes[name]["rel_err_vs_cpu_oracle"] = float(np.linalg.norm(xs - xo) / no)
-    setup_s = {"hierarchy": round(t_hier, 2),
+    setup_s = {"hierarchy": round(t_hier, 2), "hierarchy_host_only": round(t_hier_host, 2),
                "plain_upload": round(modes["plain_fp64"]["setup_ms"] / 1e3, 2),
                "headline_upload": round(hm["setup_ms"] / 1e3, 2),
                "plain_phases_ms": {k: round(v, 1) for k, v in desc_plain["setup_ms"].items()},

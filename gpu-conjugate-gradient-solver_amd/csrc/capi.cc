@@ -579,6 +579,7 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "cg_level_order") o.cg_level_order = v != 0;
     else if (k == "cg_vcycle_operator") o.cg_vcycle_operator = v != 0;
     else if (k == "cg_pair_layout") o.cg_pair_layout = v != 0;
+    else if (k == "device_pack") o.device_pack = v != 0;
     else if (k == "persist_grid") {
       if (v < 8 || v % 8 != 0 || v > (int64_t(1) << 20))
         throw std::runtime_error("persist_grid must be a positive multiple of 8.");

@@ -19,6 +19,7 @@
 // float/float as-reference).
 #include "dense.hpp"
 #include "engine.hpp"
+#include "setup_device.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -94,6 +95,28 @@ void with_group(int G, F &&f) {
     case 64: f(std::integral_constant<int, 64>{}); break;
     default: throw std::runtime_error("lane-group size must be 8, 16, 32 or 64.");
   }
+}
+
+// 3x3-block operator from a block CSR packed on the device: the fields and
+// storage DOp::build gives the same blocks on the host (lane groups from
+// group_min blocks per block row, else BSR3-SELL-64 with rows in `order`).
+template <class T>
+void build_b3_device(DOp<T> &op, const gsdev::DBsr3Csr &B, double group_min,
+                     const int32_t *order) {
+  op.n = 3 * B.nb;
+  op.b3 = true;
+  op.nblocks = B.nblocks;
+  op.nnz = 9 * B.nblocks;
+  op.bptr = B.host_ptr;
+  op.G = gsd::pick_block_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min);
+  if (op.G) gsdev::bsr3_rows<T>(B, op.bw);
+  else gsdev::bsr3_sell_pack<T>(B, order, op.b);
+}
+
+bool is_identity(const std::vector<int32_t> &o2n) {
+  for (size_t i = 0; i < o2n.size(); ++i)
+    if (o2n[i] != static_cast<int32_t>(i)) return false;
+  return true;
 }
 
 }  // namespace
@@ -224,6 +247,9 @@ class Engine final : public EngineBase {
   // Kernel nodes per PCG iteration in the captured graph (0: none yet).
   double graph_nodes_per_iter_ = 0.0;
   int64_t replace_every() const { return cg_vop_ ? opt_.cg_replace_every : 0; }
+  // Level 0's block CSR packed on the device, kept until the CG operator
+  // (the same blocks when level 0 is not permuted) has been built.
+  std::unique_ptr<gsdev::DBsr3Csr> dev_l0_;
   // Host wall time of the setup phases (ms), in order, for describe().
   std::vector<std::pair<std::string, double>> setup_ms_;
   std::chrono::steady_clock::time_point tmark_;
@@ -292,7 +318,12 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
   }
   mark("level " + std::to_string(k) + " colouring");
   const std::vector<int32_t> qinv = inverse_of(lv.q);
-  gsx::Csr Ai = relabel(Ak, lv.q, qinv);
+  // Block-smoother levels: relabelled and packed on the device from one
+  // upload of Ak (the host copy Ai is then not formed).
+  const bool dev_pack = opt_.device_pack && blocky && !lv.direct && Ak.rows % 3 == 0 &&
+                        keeps_triples(lv.q);
+  gsx::Csr Ai;
+  if (!dev_pack) Ai = relabel(Ak, lv.q, qinv);
   mark("level " + std::to_string(k) + " relabel");
   if (lv.direct) {
     if (lv.n > opt_.max_direct)
@@ -331,11 +362,31 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     key.resize(lv.n / 3);
     for (int64_t j = 0; j < lv.n / 3; ++j) key[j] = lv.q[3 * j] / 3;
   }
-  lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
-             opt_.group_min_block);
+  std::unique_ptr<gsdev::DBsr3Csr> DB;
+  if (dev_pack) {
+    DB = std::make_unique<gsdev::DBsr3Csr>();
+    {
+      const gsdev::DCsr dAk = gsdev::upload(Ak);
+      const gsdev::DCsr dAi = gsdev::relabel(dAk, lv.q, qinv);
+      gsdev::bsr3_pack(dAi, key.data(), *DB);
+    }
+    build_b3_device(lv.A, *DB, opt_.group_min_block, nullptr);
+  } else {
+    lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
+               opt_.group_min_block);
+  }
   lv.has_low = false;
-  if (!lv.direct && (lv.kind == kSGS || lv.kind == kBlockGS) && k < opt_.zero_start_lower &&
-      lv.ncolors > 1) {
+  if (dev_pack && lv.kind == kBlockGS && k < opt_.zero_start_lower && lv.ncolors > 1) {
+    // The earlier-colour blocks: block column below the row's colour range.
+    std::vector<int32_t> start(lv.n / 3);
+    for (int c = 0; c < lv.ncolors; ++c)
+      for (int64_t u = lv.coff[c]; u < lv.coff[c + 1]; ++u)
+        start[u] = static_cast<int32_t>(lv.coff[c]);
+    const gsdev::DBsr3Csr L = gsdev::bsr3_below(*DB, start);
+    build_b3_device(lv.Alow, L, lv.A.G ? 0.0 : -1.0, nullptr);
+    lv.has_low = true;
+  } else if (!dev_pack && !lv.direct && (lv.kind == kSGS || lv.kind == kBlockGS) &&
+             k < opt_.zero_start_lower && lv.ncolors > 1) {
     // Colours are contiguous internal ranges: an entry belongs to an earlier
     // colour exactly when its column (block column) lies below the row's
     // colour range. Row order and entry order are kept.
@@ -408,11 +459,21 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
       }
       if (lv.kind == kSGS) lv.diag.upload(d);
       else lv.dinv.upload(di);
+    } else if (dev_pack) {
+      // The inverse diagonal blocks of Ak moved to the internal block
+      // order: internal block row b is Ak's block row q[3b] / 3, with the
+      // same diagonal block.
+      const std::vector<double> dk = gsx::inverse_diag_blocks3(Ak);
+      std::vector<double> dinv(dk.size());
+      for (int64_t b = 0; b < lv.n / 3; ++b)
+        std::copy(&dk[9 * (lv.q[3 * b] / 3)], &dk[9 * (lv.q[3 * b] / 3)] + 9, &dinv[9 * b]);
+      gsd::upload_values<TM>(lv.dinv, dinv.data(), dinv.size());
     } else {
       const std::vector<double> dinv = gsx::inverse_diag_blocks3(Ai);
       gsd::upload_values<TM>(lv.dinv, dinv.data(), dinv.size());
     }
   }
+  if (dev_pack && k == 0) dev_l0_ = std::move(DB);
   if (k > 0) {
     lv.rhs_buf.alloc(lv.n * sizeof(TM));
     lv.x_buf.alloc(lv.n * sizeof(TM));
@@ -528,9 +589,11 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       rq[i] = perms[0].new_to_old[L_[0].q[i]];
       cmap[i] = q0inv[perms[0].old_to_new[i]];
     }
-    gsx::Csr Ai = relabel(H.A[0], rq, cmap);
     const bool blocky = opt_.fine_kind == kBlockGS || opt_.fine_kind == kBlockJacobi ||
                         opt_.cg_block3;
+    const bool dev_cg = opt_.device_pack && blocky && n_ % 3 == 0;
+    gsx::Csr Ai;
+    if (!dev_cg) Ai = relabel(H.A[0], rq, cmap);
     int2lvl0_ = L_[0].q;
     int2orig_.resize(n_);
     for (int64_t i = 0; i < n_; ++i) int2orig_[i] = perms[0].new_to_old[L_[0].q[i]];
@@ -550,8 +613,24 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
       for (int64_t p = 0; p < n_ / 3; ++p) ord[p] = q0inv[3 * p] / 3;
     }
     Acg_.b.pairs = opt_.cg_pair_layout && std::is_same<TC, double>::value;
-    Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data(), -1.0, -1.0,
-               ord.empty() ? nullptr : ord.data());
+    if (dev_cg) {
+      // With level 0 unpermuted and its input already sorted, this is the
+      // block CSR level 0 was packed from (same relabelling, same block
+      // ranks); otherwise the input is uploaded and packed here.
+      const bool reuse = dev_l0_ && is_identity(perms[0].old_to_new) &&
+                         gsx::rows_sorted_unique(H.A[0]);
+      gsdev::DBsr3Csr own;
+      if (!reuse) {
+        const gsdev::DCsr dA = gsdev::upload(H.A[0]);
+        const gsdev::DCsr dAi = gsdev::relabel(dA, rq, cmap);
+        gsdev::bsr3_pack(dAi, key.data(), own);
+      }
+      build_b3_device(Acg_, reuse ? *dev_l0_ : own, -1.0, ord.empty() ? nullptr : ord.data());
+    } else {
+      Acg_.build(Ai, blocky && n_ % 3 == 0, key.empty() ? nullptr : key.data(), -1.0, -1.0,
+                 ord.empty() ? nullptr : ord.data());
+    }
+    dev_l0_.reset();
     cg_pairs_ = Acg_.b3 && Acg_.b.pairs;
     mark("CG operator");
   }

@@ -75,6 +75,10 @@ struct Options {
   // XCD); each wavefront walks one SELL slice per pass.
   int persist_grid = static_cast<int>(gsk::kPersistGrid);
   int cg_replace_every = 8;
+  // Block-smoother levels (and a 3x3-block CG operator) are relabelled and
+  // packed on the GPU from one upload of the level matrix, with the same
+  // layouts as the host packers; off: packed on the host.
+  bool device_pack = true;
   std::vector<double> level_omega;  // per-level overrides (> 0 applies)
   std::vector<int> level_sweeps;    // per-level pre = post sweeps (>= 0 applies)
 };

@@ -88,6 +88,9 @@ void check_columns(const Csr &A, const char *what);
 Csr transpose(const Csr &A);                // sorted output
 Csr spgemm(const Csr &A, const Csr &B);     // sorted output, OpenMP
 void sort_rows(Csr &A);                     // sort + sum duplicates
+// True when every row's columns are strictly increasing (sort_rows would
+// leave A unchanged).
+bool rows_sorted_unique(const Csr &A);
 
 // ---------------------------------------------------------------------------
 // ELL packing (src/HostELLMatrix.cc:10-206): slot-major

@@ -150,6 +150,14 @@ void sort_rows(Csr &A) {
   A.ptr.swap(newptr);
 }
 
+bool rows_sorted_unique(const Csr &A) {
+  bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int64_t r = 0; r < A.rows; ++r)
+    for (int32_t k = A.ptr[r] + 1; k < A.ptr[r + 1]; ++k) ok = ok && A.col[k - 1] < A.col[k];
+  return ok;
+}
+
 Csr permute_sorted(const Csr &A, const int32_t *row_o2n,
                    const int32_t *col_o2n) {
   Csr B;

@@ -44,13 +44,7 @@ uint32_t blocks_for(int64_t n) {
   return static_cast<uint32_t>(std::max<int64_t>(1, (n + kThreads - 1) / kThreads));
 }
 
-struct DCsr {
-  int64_t rows = 0, cols = 0, nnz = 0;
-  DevBuf ptr, col, val;
-  const int32_t *p() const { return ptr.as<int32_t>(); }
-  const int32_t *c() const { return col.as<int32_t>(); }
-  const double *v() const { return val.as<double>(); }
-};
+}  // namespace
 
 DCsr upload(const gsx::Csr &A) {
   DCsr d;
@@ -62,6 +56,8 @@ DCsr upload(const gsx::Csr &A) {
   d.val.upload(A.val.data(), A.val.size());
   return d;
 }
+
+namespace {
 
 gsx::Csr download(const DCsr &d) {
   gsx::Csr A;
@@ -357,5 +353,317 @@ gsx::Csr device_spgemm(const gsx::Csr &A, const gsx::Csr &B) {
 }
 
 gsx::Csr device_transpose(const gsx::Csr &A) { return download(transpose(upload(A))); }
+
+
+// ---------------------------------------------------------------------------
+// Operator packing on the device: the relabelling and the 3x3-block layouts
+// of a level operator, equal to gsx::relabel, gsx::bsr3_pack and
+// gsx::bsr3_sell64_pack (+ gsx::bsr3_pair_planes) array for array.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_relabel_len(const int32_t *ap, const int32_t *rq, int64_t n, int32_t *len) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) len[i] = ap[rq[i] + 1] - ap[rq[i]];
+}
+
+__global__ void k_relabel_copy(const int32_t *ap, const int32_t *ac, const double *av,
+                               const int32_t *rq, const int32_t *cmap, int64_t n,
+                               const int32_t *bp, int32_t *bc, double *bv) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  int64_t d = bp[i];
+  for (int32_t k = ap[rq[i]]; k < ap[rq[i] + 1]; ++k, ++d) {
+    bc[d] = cmap[ac[k]];
+    bv[d] = av[k];
+  }
+}
+
+__device__ void heap_sort64(int64_t *keys, int64_t n) {
+  auto sift = [&](int64_t root, int64_t end) {
+    for (;;) {
+      int64_t child = 2 * root + 1;
+      if (child >= end) return;
+      if (child + 1 < end && keys[child + 1] > keys[child]) ++child;
+      if (keys[root] >= keys[child]) return;
+      const int64_t t = keys[root];
+      keys[root] = keys[child];
+      keys[child] = t;
+      root = child;
+    }
+  };
+  for (int64_t s = n / 2 - 1; s >= 0; --s) sift(s, n);
+  for (int64_t e = n - 1; e > 0; --e) {
+    const int64_t t = keys[0];
+    keys[0] = keys[e];
+    keys[e] = t;
+    sift(0, e);
+  }
+}
+
+// Block columns of block row b (the union over its three scalar rows),
+// ordered by (rank, column) and made unique, left at the start of the
+// rows' own CSR range of `scratch` (keys rank << 32 | column); cnt[b] their
+// number.
+__global__ void k_b3_columns(const int32_t *ap, const int32_t *ac, int64_t nb,
+                             const int32_t *key, int64_t *scratch, int64_t *cnt) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (b >= nb) return;
+  const int64_t k0 = ap[3 * b], k1 = ap[3 * b + 3];
+  int64_t *s = scratch + k0;
+  for (int64_t k = k0; k < k1; ++k) {
+    const int64_t cb = ac[k] / 3;
+    const int64_t rank = key ? key[cb] : cb;
+    s[k - k0] = (rank << 32) | cb;
+  }
+  const int64_t m = k1 - k0;
+  heap_sort64(s, m);
+  int64_t u = 0;
+  for (int64_t i = 0; i < m; ++i)
+    if (i == 0 || s[i] != s[u - 1]) s[u++] = s[i];
+  cnt[b] = u;
+}
+
+// Block columns and values of every block row: values start at 0 and the
+// scalar entries are added in CSR order (rows 3b, 3b + 1, 3b + 2).
+__global__ void k_b3_fill(const int32_t *ap, const int32_t *ac, const double *av, int64_t nb,
+                          const int64_t *scratch, const int64_t *bptr, int32_t *bcol,
+                          double *bval) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (b >= nb) return;
+  const int64_t *s = scratch + ap[3 * b];
+  const int64_t p0 = bptr[b], m = bptr[b + 1] - p0;
+  for (int64_t i = 0; i < m; ++i) bcol[p0 + i] = static_cast<int32_t>(s[i] & 0xffffffff);
+  double *v = bval + 9 * p0;
+  for (int64_t i = 0; i < 9 * m; ++i) v[i] = 0.0;
+  for (int l = 0; l < 3; ++l)
+    for (int32_t k = ap[3 * b + l]; k < ap[3 * b + l + 1]; ++k) {
+      const int32_t cb = ac[k] / 3;
+      int64_t slot = 0;
+      while (bcol[p0 + slot] != cb) ++slot;
+      v[9 * slot + 3 * l + ac[k] % 3] += av[k];
+    }
+}
+
+__global__ void k_b3_below_count(const int64_t *bptr, const int32_t *bcol, int64_t nb,
+                                 const int32_t *start, int64_t *cnt) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (b >= nb) return;
+  int64_t c = 0;
+  for (int64_t k = bptr[b]; k < bptr[b + 1]; ++k) c += bcol[k] < start[b] ? 1 : 0;
+  cnt[b] = c;
+}
+
+__global__ void k_b3_below_copy(const int64_t *bptr, const int32_t *bcol, const double *bval,
+                                int64_t nb, const int32_t *start, const int64_t *optr,
+                                int32_t *ocol, double *oval) {
+  const int64_t b = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (b >= nb) return;
+  int64_t d = optr[b];
+  for (int64_t k = bptr[b]; k < bptr[b + 1]; ++k)
+    if (bcol[k] < start[b]) {
+      ocol[d] = bcol[k];
+      for (int e = 0; e < 9; ++e) oval[9 * d + e] = bval[9 * k + e];
+      ++d;
+    }
+}
+
+// Width of each 64-position slice (position p holds block row order[p], or
+// p without an order).
+__global__ void k_b3_slice_width(const int64_t *bptr, const int32_t *order, int64_t nb,
+                                 int64_t ns, int32_t *w) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (s >= ns) return;
+  int64_t m = 0;
+  for (int64_t p = 64 * s; p < 64 * s + 64 && p < nb; ++p) {
+    const int64_t r = order ? order[p] : p;
+    m = max(m, bptr[r + 1] - bptr[r]);
+  }
+  w[s] = static_cast<int32_t>(m);
+}
+
+// One (slice, lane) per thread: the lane's block row across the slice's
+// slots, with the host packer's padding (a zero block on the position's own
+// column for square operators, else the row's last block column or 0).
+template <class T>
+__global__ void k_b3_sell_fill(const int64_t *bptr, const int32_t *bcol, const double *bval,
+                               const int32_t *order, int64_t nb, bool square, int64_t ns,
+                               const int64_t *slice_ptr, const int32_t *slice_w, bool pairs,
+                               int32_t *obcol, T *oval) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const int64_t s = t >> 6, l = t & 63;
+  if (s >= ns) return;
+  const int64_t p = 64 * s + l;
+  const bool valid = p < nb;
+  const int64_t r = valid ? (order ? order[p] : p) : 0;
+  const int64_t k0 = valid ? bptr[r] : 0, len = valid ? bptr[r + 1] - k0 : 0;
+  int32_t pc = 0;
+  if (square) pc = valid ? static_cast<int32_t>(p) : 0;
+  else if (len > 0) pc = bcol[k0 + len - 1];
+  for (int32_t j = 0; j < slice_w[s]; ++j) {
+    const int64_t slot = slice_ptr[s] + static_cast<int64_t>(j) * 64;
+    const bool have = j < len;
+    obcol[slot + l] = have ? bcol[k0 + j] : pc;
+    T *v = oval + slot * 9;
+    for (int e = 0; e < 9; ++e) {
+      const int64_t off = pairs ? (e < 8 ? 128 * (e >> 1) + 2 * l + (e & 1) : 512 + l) : 64 * e + l;
+      v[off] = have ? static_cast<T>(bval[9 * (k0 + j) + e]) : T(0);
+    }
+  }
+}
+
+template <class T>
+__global__ void k_cast(const double *src, int64_t n, T *dst) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) dst[i] = static_cast<T>(src[i]);
+}
+
+template <class T>
+std::vector<T> fetch(const DevBuf &b, int64_t n) {
+  std::vector<T> h(n);
+  if (n) GS_HIP(hipMemcpy(h.data(), b.get(), n * sizeof(T), hipMemcpyDeviceToHost));
+  return h;
+}
+
+// Exclusive prefix of per-row counts (downloaded), as int64 offsets.
+std::vector<int64_t> offsets(const DevBuf &cnt, int64_t n) {
+  const std::vector<int64_t> c = fetch<int64_t>(cnt, n);
+  std::vector<int64_t> ptr(n + 1, 0);
+  for (int64_t i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + c[i];
+  return ptr;
+}
+
+}  // namespace
+
+DCsr relabel(const DCsr &A, const std::vector<int32_t> &rq, const std::vector<int32_t> &cmap) {
+  const int64_t n = A.rows;
+  DCsr B;
+  B.rows = n;
+  B.cols = A.cols;
+  B.nnz = A.nnz;
+  DevBuf rq_d, cmap_d, len_d(std::max<int64_t>(n, 1) * sizeof(int32_t));
+  rq_d.upload(rq);
+  cmap_d.upload(cmap);
+  k_relabel_len<<<blocks_for(n), kThreads>>>(A.p(), rq_d.as<int32_t>(), n, len_d.as<int32_t>());
+  const std::vector<int32_t> len = fetch<int32_t>(len_d, n);
+  B.ptr.upload(row_ptr(len, "relabel: nnz exceeds int32."));
+  B.col.alloc(A.nnz * sizeof(int32_t));
+  B.val.alloc(A.nnz * sizeof(double));
+  k_relabel_copy<<<blocks_for(n), kThreads>>>(A.p(), A.c(), A.v(), rq_d.as<int32_t>(),
+                                             cmap_d.as<int32_t>(), n, B.ptr.as<int32_t>(),
+                                             B.col.as<int32_t>(), B.val.as<double>());
+  GS_HIP(hipGetLastError());
+  GS_HIP(hipDeviceSynchronize());
+  return B;
+}
+
+bool bsr3_pack(const DCsr &A, const int32_t *key, DBsr3Csr &out) {
+  if (A.rows % 3 != 0 || A.cols % 3 != 0) return false;
+  const int64_t nb = A.rows / 3;
+  DevBuf key_d, scratch(std::max<int64_t>(A.nnz, 1) * sizeof(int64_t)),
+      cnt(std::max<int64_t>(nb, 1) * sizeof(int64_t));
+  if (key) key_d.upload(key, static_cast<size_t>(A.cols / 3));
+  k_b3_columns<<<blocks_for(nb), kThreads>>>(A.p(), A.c(), nb, key ? key_d.as<int32_t>() : nullptr,
+                                            scratch.as<int64_t>(), cnt.as<int64_t>());
+  GS_HIP(hipGetLastError());
+  out.nb = nb;
+  out.ncb = A.cols / 3;
+  out.host_ptr = offsets(cnt, nb);
+  out.nblocks = out.host_ptr[nb];
+  int64_t width = 0;
+  for (int64_t b = 0; b < nb; ++b) width = std::max(width, out.host_ptr[b + 1] - out.host_ptr[b]);
+  out.width = static_cast<int32_t>(width);
+  out.ptr.upload(out.host_ptr);
+  out.bcol.alloc(out.nblocks * sizeof(int32_t));
+  out.val.alloc(9 * out.nblocks * sizeof(double));
+  k_b3_fill<<<blocks_for(nb), kThreads>>>(A.p(), A.c(), A.v(), nb, scratch.as<int64_t>(),
+                                         out.ptr.as<int64_t>(), out.bcol.as<int32_t>(),
+                                         out.val.as<double>());
+  GS_HIP(hipGetLastError());
+  GS_HIP(hipDeviceSynchronize());
+  return true;
+}
+
+DBsr3Csr bsr3_below(const DBsr3Csr &B, const std::vector<int32_t> &start) {
+  DBsr3Csr out;
+  out.nb = B.nb;
+  out.ncb = B.ncb;
+  DevBuf start_d, cnt(std::max<int64_t>(B.nb, 1) * sizeof(int64_t));
+  start_d.upload(start);
+  k_b3_below_count<<<blocks_for(B.nb), kThreads>>>(B.ptr.as<int64_t>(), B.bcol.as<int32_t>(), B.nb,
+                                                  start_d.as<int32_t>(), cnt.as<int64_t>());
+  out.host_ptr = offsets(cnt, B.nb);
+  out.nblocks = out.host_ptr[B.nb];
+  int64_t width = 0;
+  for (int64_t b = 0; b < B.nb; ++b) width = std::max(width, out.host_ptr[b + 1] - out.host_ptr[b]);
+  out.width = static_cast<int32_t>(width);
+  out.ptr.upload(out.host_ptr);
+  out.bcol.alloc(out.nblocks * sizeof(int32_t));
+  out.val.alloc(9 * out.nblocks * sizeof(double));
+  k_b3_below_copy<<<blocks_for(B.nb), kThreads>>>(B.ptr.as<int64_t>(), B.bcol.as<int32_t>(),
+                                                 B.val.as<double>(), B.nb, start_d.as<int32_t>(),
+                                                 out.ptr.as<int64_t>(), out.bcol.as<int32_t>(),
+                                                 out.val.as<double>());
+  GS_HIP(hipGetLastError());
+  GS_HIP(hipDeviceSynchronize());
+  return out;
+}
+
+template <class T>
+void bsr3_sell_pack(const DBsr3Csr &B, const int32_t *order, gsd::DBsr3<T> &out) {
+  const int64_t nb = B.nb, ns = (nb + 63) / 64;
+  DevBuf order_d, w_d(std::max<int64_t>(ns, 1) * sizeof(int32_t));
+  if (order) order_d.upload(order, static_cast<size_t>(nb));
+  const int32_t *ord = order ? order_d.as<int32_t>() : nullptr;
+  k_b3_slice_width<<<blocks_for(ns), kThreads>>>(B.ptr.as<int64_t>(), ord, nb, ns,
+                                                w_d.as<int32_t>());
+  const std::vector<int32_t> w = fetch<int32_t>(w_d, ns);
+  std::vector<int64_t> sp(ns + 1, 0);
+  for (int64_t s = 0; s < ns; ++s) sp[s + 1] = sp[s] + int64_t(w[s]) * 64;
+  out.nb = nb;
+  out.nblocks = B.nblocks;
+  out.slots = sp[ns];
+  out.slice_ptr.upload(sp);
+  out.slice_w.upload(w);
+  out.bcol.alloc(out.slots * sizeof(int32_t));
+  out.val.alloc(9 * out.slots * sizeof(T));
+  k_b3_sell_fill<T><<<blocks_for(64 * ns), kThreads>>>(
+      B.ptr.as<int64_t>(), B.bcol.as<int32_t>(), B.val.as<double>(), ord, nb, B.nb == B.ncb, ns,
+      out.slice_ptr.template as<int64_t>(), out.slice_w.template as<int32_t>(), out.pairs,
+      out.bcol.template as<int32_t>(), out.val.template as<T>());
+  GS_HIP(hipGetLastError());
+  if (order) {
+    out.row_map = std::move(order_d);
+    out.mapped = true;
+  } else {
+    out.row_map.release();
+    out.mapped = false;
+  }
+  GS_HIP(hipDeviceSynchronize());
+}
+
+template <class T>
+void bsr3_rows(const DBsr3Csr &B, gsd::DBRows<T> &out) {
+  out.nb = B.nb;
+  out.nblocks = B.nblocks;
+  out.ptr.upload(B.host_ptr);
+  out.bcol.alloc(std::max<int64_t>(B.nblocks, 1) * sizeof(int32_t));
+  if (B.nblocks)
+    GS_HIP(hipMemcpy(out.bcol.get(), B.bcol.get(), B.nblocks * sizeof(int32_t),
+                     hipMemcpyDeviceToDevice));
+  out.val.alloc(9 * B.nblocks * sizeof(T));
+  if (B.nblocks)
+    k_cast<T><<<blocks_for(9 * B.nblocks), kThreads>>>(B.val.as<double>(), 9 * B.nblocks,
+                                                      out.val.template as<T>());
+  GS_HIP(hipGetLastError());
+  out.row_map.release();
+  out.mapped = false;
+  GS_HIP(hipDeviceSynchronize());
+}
+
+template void bsr3_sell_pack<float>(const DBsr3Csr &, const int32_t *, gsd::DBsr3<float> &);
+template void bsr3_sell_pack<double>(const DBsr3Csr &, const int32_t *, gsd::DBsr3<double> &);
+template void bsr3_rows<float>(const DBsr3Csr &, gsd::DBRows<float> &);
+template void bsr3_rows<double>(const DBsr3Csr &, gsd::DBRows<double> &);
 
 }  // namespace gsdev

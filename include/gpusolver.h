@@ -259,6 +259,10 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     entries it leaves out multiply exact zeros; same
  *                     results); 0: none (1: level 0, where the sweeps are
  *                     bandwidth-bound)
+ *   "device_pack"     1: 3x3-block operators (block-smoother levels, a
+ *                     3x3-block CG operator) are relabelled and packed on
+ *                     the GPU from one upload of the level matrix, into the
+ *                     layouts the host packers produce; 0: on the host (1)
  *   "persist_grid"    workgroups of the persistent FP64 3x3-block CG SpMV
  *                     (a positive multiple of 8; 4096): one share of the
  *                     slices per XCD, each wavefront walking slices at a
