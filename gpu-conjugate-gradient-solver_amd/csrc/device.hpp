@@ -265,8 +265,8 @@ inline int pick_group(double avg, double min_avg, int max_g = 64) {
   else if (avg <= 32.0) g = 32;
   return g < max_g ? g : max_g;
 }
-// Block rows: two blocks per lane and round, so a round of G lanes covers
-// 2 G blocks.
+// Block rows: lanes per block row from the average blocks per row (the
+// smoothers load two blocks per lane and round; the transfers pick 2-4).
 inline int pick_block_group(double avg, double min_avg) {
   if (min_avg < 0.0 || avg < min_avg) return 0;
   if (avg <= 16.0) return 8;
@@ -280,7 +280,8 @@ inline int pick_block_group(double avg, double min_avg) {
 template <class T>
 struct DMat {
   int G = 0;  // lanes per row; 0: SELL-64, one thread per row
-  int K = 2;  // lane groups: entries per lane and load round (2, 4 or 8)
+  int K = 2;  // lane groups: entries (scalar: 2, 4, 8; 3x3 blocks: 2, 3, 4)
+             // per lane and load round
   bool b3 = false;
   int64_t rows = 0, cols = 0, nnz = 0;
   int64_t nblocks = 0;  // 3x3 blocks (block storage)
@@ -302,8 +303,16 @@ struct DMat {
     cols = A.cols;
     nblocks = B.nblocks;
     nnz = 9 * B.nblocks;
-    K = 2;
-    G = pick_block_group(B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0, group_min);
+    const double mean = B.nb ? static_cast<double>(B.nblocks) / B.nb : 0.0;
+    G = pick_block_group(mean, group_min);
+    // Blocks per lane and round: one round for rows up to 1.25x the mean
+    // block count where possible (2, 3 or 4).
+    K = 4;
+    for (int k : {2, 3})
+      if (G && G * k >= 1.25 * mean) {
+        K = k;
+        break;
+      }
     if (G) bw.build(B, order);
     else sb.build(B, sigma, order);
   }

@@ -85,6 +85,18 @@ void with_k(int K, F &&f) {
   }
 }
 
+// Calls f with the 3x3-block transfer rows' blocks per lane and round (2, 3,
+// 4) as a compile-time constant.
+template <class F>
+void with_bk(int K, F &&f) {
+  switch (K) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, 3>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    default: throw std::runtime_error("blocks per lane must be 2, 3 or 4.");
+  }
+}
+
 // Calls f with the lane-group size G as a compile-time constant.
 template <class F>
 void with_group(int G, F &&f) {
@@ -855,9 +867,11 @@ void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int
   if (lv.R.b3) {
     if (lv.R.G)
       with_group(lv.R.G, [&](auto g) {
-        constexpr int G = decltype(g)::value;
-        gsk::k_restrict_bw<TM, G><<<gsk::grid_rows<G>(cl.n / 3), kBlock, 0, stream_>>>(
-            lv.R.bw.view(), res, cl.rhs, corr_c, stop);
+        with_bk(lv.R.K, [&](auto kk) {
+          constexpr int G = decltype(g)::value, K = decltype(kk)::value;
+          gsk::k_restrict_bw<TM, G, K><<<gsk::grid_rows<G>(cl.n / 3), kBlock, 0, stream_>>>(
+              lv.R.bw.view(), res, cl.rhs, corr_c, stop);
+        });
       });
     else
       gsk::k_restrict_b3<TM><<<grid_for(cl.n / 3), kBlock, 0, stream_>>>(lv.R.sb.view(), res,
@@ -885,9 +899,11 @@ void Engine<TC, TM>::encode_prolong(int k, bool alias, const int32_t *stop) {
   if (lv.P.b3) {
     if (lv.P.G)
       with_group(lv.P.G, [&](auto g) {
-        constexpr int G = decltype(g)::value;
-        gsk::k_prolong_bw<TM, G><<<gsk::grid_rows<G>(lv.n / 3), kBlock, 0, stream_>>>(
-            lv.P.bw.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+        with_bk(lv.P.K, [&](auto kk) {
+          constexpr int G = decltype(g)::value, K = decltype(kk)::value;
+          gsk::k_prolong_bw<TM, G, K><<<gsk::grid_rows<G>(lv.n / 3), kBlock, 0, stream_>>>(
+              lv.P.bw.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+        });
       });
     else
       gsk::k_prolong_b3<TM><<<grid_for(lv.n / 3), kBlock, 0, stream_>>>(

@@ -156,31 +156,45 @@ __device__ __forceinline__ T grp_row_dot_k(const Rows<T> &A, int64_t rb, int64_t
 // C = G * grp_bunroll<G>() blocks; products are staged line-major (line l
 // of round slot j at lp[3 C l + 3 j + c]) so each summing lane adds one
 // contiguous run. lp: 9 C values.
-template <class T, int G>
+// U: blocks per lane and round (transfer rows may take more than the
+// smoothers' grp_bunroll, so a long row needs one round). A round issues
+// every block-column index and value load first, then the x gathers, then
+// the products: slots past the row end reload the row's last block (an
+// in-bounds index, so no load is behind a branch) and are staged but not
+// summed; the excluded block becomes -0.0 by a select.
+template <class T, int G, int U = grp_bunroll<G>()>
 __device__ __forceinline__ T grp_brow_dot(const BRows<T> &A, int64_t rb, int64_t re, const T *x,
                                           int64_t excl, T *lp, int lane) {
-  constexpr int U = grp_bunroll<G>();
   constexpr int C = G * U;
   T s = T(0);
   for (int64_t base = rb; base < re; base += C) {
+    int32_t bc[U];
+    T v[U][9];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t k = base + lane + u * G;
+      k = k < re ? k : re - 1;
+      bc[u] = A.bcol[k];
+      const T *vp = A.val + 9 * k;
+#pragma unroll
+      for (int e = 0; e < 9; ++e) v[u][e] = vp[e];
+    }
+    T xv[U][3];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T *xp = x + 3 * static_cast<int64_t>(bc[u]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) xv[u][c] = xp[c];
+    }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int j = lane + u * G;
-      const int64_t k = base + j;
-      if (k < re) {
-        const int32_t bc = A.bcol[k];
-        const T *v = A.val + 9 * k;
-        const T x0 = x[3 * static_cast<int64_t>(bc)];
-        const T x1 = x[3 * static_cast<int64_t>(bc) + 1];
-        const T x2 = x[3 * static_cast<int64_t>(bc) + 2];
-        const bool d = bc == excl;
+      const bool d = bc[u] == excl;
 #pragma unroll
-        for (int l = 0; l < 3; ++l) {
-          lp[3 * C * l + 3 * j + 0] = d ? neg_zero<T>() : v[3 * l + 0] * x0;
-          lp[3 * C * l + 3 * j + 1] = d ? neg_zero<T>() : v[3 * l + 1] * x1;
-          lp[3 * C * l + 3 * j + 2] = d ? neg_zero<T>() : v[3 * l + 2] * x2;
-        }
-      }
+      for (int l = 0; l < 3; ++l)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          lp[3 * C * l + 3 * j + c] = d ? neg_zero<T>() : v[u][3 * l + c] * xv[u][c];
     }
     wave_sync();
     if (lane < 3) {
@@ -372,14 +386,18 @@ __global__ __launch_bounds__(kBlock) void k_residual_bw(BRows<T> A, const T *__r
 }
 
 // 3x3-block transfer rows, one G-lane group per block row (as
-// k_restrict_b3 / k_prolong_b3).
-template <class T, int G>
+// k_restrict_b3 / k_prolong_b3), K blocks per lane and round.
+#define GSK_TRANSFER_BLOCK_LDS(G, K)        \
+  __shared__ T lp[9 * kBlock * K];          \
+  const int lane = threadIdx.x % G;         \
+  const int g0 = (threadIdx.x - lane) * K
+
+template <class T, int G, int K>
 __global__ __launch_bounds__(kBlock) void k_restrict_bw(BRows<T> R, const T *__restrict__ res,
                                                         T *__restrict__ rhs_c,
                                                         T *__restrict__ corr_c,
                                                         const int32_t *stop) {
-  GSK_GROUP_BLOCK_LDS(G);
-  (void)glr;
+  GSK_TRANSFER_BLOCK_LDS(G, K);
   const int64_t pos = grp_row<G>(0);
   const bool active = pos < R.nb;
   int64_t rb = 0, re = 0, b = pos;
@@ -389,19 +407,18 @@ __global__ __launch_bounds__(kBlock) void k_restrict_bw(BRows<T> R, const T *__r
     if (R.row_map) b = R.row_map[pos];
   }
   if (stop_word(stop)) return;
-  const T s = grp_brow_dot<T, G>(R, rb, re, res, -1, lp + 9 * g0, lane);
+  const T s = grp_brow_dot<T, G, K>(R, rb, re, res, -1, lp + 9 * g0, lane);
   if (active && lane < 3) {
     rhs_c[3 * b + lane] = s;
     if (corr_c) corr_c[3 * b + lane] = T(0);
   }
 }
 
-template <class T, int G>
+template <class T, int G, int K>
 __global__ __launch_bounds__(kBlock) void k_prolong_bw(BRows<T> P, const T *__restrict__ e,
                                                        T *__restrict__ x, int alias,
                                                        const int32_t *stop) {
-  GSK_GROUP_BLOCK_LDS(G);
-  (void)glr;
+  GSK_TRANSFER_BLOCK_LDS(G, K);
   const int64_t pos = grp_row<G>(0);
   const bool active = pos < P.nb;
   int64_t rb = 0, re = 0, b = pos;
@@ -413,7 +430,7 @@ __global__ __launch_bounds__(kBlock) void k_prolong_bw(BRows<T> P, const T *__re
     if (!alias && lane < 3) xl = x[3 * b + lane];
   }
   if (stop_word(stop)) return;
-  const T c = grp_brow_dot<T, G>(P, rb, re, e, -1, lp + 9 * g0, lane);
+  const T c = grp_brow_dot<T, G, K>(P, rb, re, e, -1, lp + 9 * g0, lane);
   if (active && lane < 3) x[3 * b + lane] = alias ? c + T(1) * c : xl + T(1) * c;
 }
 
