@@ -589,6 +589,11 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
       if (v < 1) throw std::runtime_error("cg_replace_every must be at least 1.");
       o.cg_replace_every = static_cast<int>(v);
     }
+    else if (k == "cg_true_residual_every") {
+      if (v < 0 || v > (int64_t(1) << 30))
+        throw std::runtime_error("cg_true_residual_every must be non-negative.");
+      o.cg_true_residual_every = static_cast<int>(v);
+    }
     else if (k == "zero_start_lower") {
       if (v < 0) throw std::runtime_error("zero_start_lower must be non-negative.");
       o.zero_start_lower = static_cast<int>(v);

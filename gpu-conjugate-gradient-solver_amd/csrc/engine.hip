@@ -223,7 +223,7 @@ class Engine final : public EngineBase {
   void encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed = false);
   void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
                      bool x_zeroed = false);
-  void encode_iteration(const int32_t *stop);
+  void encode_iteration(const int32_t *stop, bool with_replace = true);
   void encode_residual(Level<TM> &lv, const int32_t *stop);
   void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop);
   void encode_prolong(int k, bool alias, const int32_t *stop);
@@ -258,7 +258,11 @@ class Engine final : public EngineBase {
   bool cg_pairs_ = false;
   // Kernel nodes per PCG iteration in the captured graph (0: none yet).
   double graph_nodes_per_iter_ = 0.0;
-  int64_t replace_every() const { return cg_vop_ ? opt_.cg_replace_every : 0; }
+  // Residual replacement period: mixed precision only (the FP64 operator
+  // recomputes r = b - A x and confirms every stop on convergence).
+  int64_t replace_every() const {
+    return std::is_same<TC, TM>::value ? 0 : (cg_vop_ ? opt_.cg_replace_every : opt_.cg_true_residual_every);
+  }
   // Level 0's block CSR packed on the device, kept until the CG operator
   // (the same blocks when level 0 is not permuted) has been built.
   std::unique_ptr<gsdev::DBsr3Csr> dev_l0_;
@@ -935,7 +939,7 @@ void Engine<TC, TM>::apply_preconditioner(const int32_t *stop) {
 }
 
 template <class TC, class TM>
-void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
+void Engine<TC, TM>::encode_iteration(const int32_t *stop, bool with_replace) {
   auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
   TC *parts = partials_.template as<TC>();
   const DOp<TM> &Av = L_[0].A;
@@ -963,8 +967,9 @@ void Engine<TC, TM>::encode_iteration(const int32_t *stop) {
   gsk::k_cg_update_xr<TC, TM><<<gn, kBlock, 0, stream_>>>(
       x_.template as<TC>(), r_.template as<TC>(), rm, p_.template as<TC>(), q_.template as<TC>(), n_, ctl, parts);
   gsk::k_cg_rr_final<TC><<<1, kBlock, 0, stream_>>>(ctl, parts, gn);
-  if (cg_vop_) {
-    // Replacement (skipped on the device unless due): r = b - A x in FP64.
+  if (with_replace && replace_every() > 0) {
+    // Replacement (skipped on the device unless due or a convergence check
+    // is pending): r = b - A x in FP64.
     const uint32_t gr = grid_for(Acg_.b3 ? n_ / 3 : n_);
     if (cg_pairs_)
       gsk::k_cg_replace_b3<TC, TM, true><<<gr, kBlock, 0, stream_>>>(
@@ -991,7 +996,14 @@ void Engine<TC, TM>::capture_graph() {
   auto *ctl = ctl_.template as<gsk::Ctl<TC>>();
   const int32_t *stop = &ctl->stop;
   GS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  for (int i = 0; i < opt_.graph_iters; ++i) encode_iteration(stop);
+  // The replacement pair sits in the last iteration of the graph (in every
+  // one when the period is shorter): a due replacement waits there, and a
+  // convergence check pending from an earlier iteration (whose remaining
+  // kernels skipped) is resolved there, where a failed check lets that
+  // iteration's preconditioner and direction update run.
+  const int64_t every = replace_every();
+  for (int i = 0; i < opt_.graph_iters; ++i)
+    encode_iteration(stop, every > 0 && (every < opt_.graph_iters || i == opt_.graph_iters - 1));
   GS_HIP(hipStreamEndCapture(stream_, &graph_));
   size_t nodes = 0;
   GS_HIP(hipGraphGetNodes(graph_, nullptr, &nodes));
@@ -1049,22 +1061,25 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
       b_.template as<TC>(), r_.template as<TC>(), z_.template as<TM>(), p_.template as<TC>(), n_, ctl, partials_.template as<TC>());
   gsk::k_cg_init_final<TC><<<1, kBlock, 0, stream_>>>(ctl, partials_.template as<TC>(), grid_for(n_));
   const int32_t *stop = &ctl->stop;
-  int64_t launched = 0;
+  int64_t launches = 0;  // each runs at least one iteration while stop == 0
   double log_rel0 = 0.0;
   if (maxit > 0) {
     if (opt_.graph_iters > 0) capture_graph();
-    const int per = opt_.graph_iters > 0 ? opt_.graph_iters : 1;
     for (;;) {
       GS_HIP(hipMemcpyAsync(host_ctl_, ctl_.get(), sizeof(gsk::Ctl<TC>), hipMemcpyDeviceToHost,
                             stream_));
       GS_HIP(hipStreamSynchronize(stream_));
-      if (host_ctl_->stop != 0 || launched >= maxit) break;
+      if (host_ctl_->stop == 5)
+        throw std::logic_error("CGSolver: convergence check left pending.");
+      // The device's iteration count, not the launches: a graph launch in
+      // which a convergence check failed runs fewer than graph_iters.
+      if (host_ctl_->stop != 0 || host_ctl_->iter >= maxit || launches > maxit) break;
       if (guard) {
         // The first check sees the initial residual; later ones compare the
         // reduction so far with the guard's rate stretched by its lag.
         const double rel = static_cast<double>(host_ctl_->rel);
         const int64_t it = host_ctl_->iter;
-        if (launched == 0) {
+        if (launches == 0) {
           log_rel0 = std::log(rel);
         } else if (it >= guard->min_iter && rel > 0.0 &&
                    std::log(rel) - log_rel0 > guard->log_rate * static_cast<double>(it) / guard->lag) {
@@ -1074,7 +1089,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
       }
       if (graph_exec_) GS_HIP(hipGraphLaunch(graph_exec_, stream_));
       else encode_iteration(stop);
-      launched += per;
+      ++launches;
     }
   } else {
     GS_HIP(hipMemcpyAsync(host_ctl_, ctl_.get(), sizeof(gsk::Ctl<TC>), hipMemcpyDeviceToHost,
@@ -1299,6 +1314,7 @@ std::string Engine<TC, TM>::describe() const {
   for (size_t i = 0; i < setup_ms_.size(); ++i)
     os << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
   os << "}, \"cg_vcycle_operator\": " << (cg_vop_ ? "true" : "false")
+     << ", \"replace_every\": " << replace_every()
      << ", \"cg_pair_layout\": " << (cg_pairs_ ? "true" : "false")
      << ", \"cg_block3\": " << (Acg_.b3 ? "true" : "false")
      << ", \"cg_spmv_bytes\": " << Acg_.spmv_bytes() << ", \"graph_iters\": " << opt_.graph_iters

@@ -75,6 +75,11 @@ struct Options {
   // XCD); each wavefront walks one SELL slice per pass.
   int persist_grid = static_cast<int>(gsk::kPersistGrid);
   int cg_replace_every = 8;
+  // Mixed precision with the FP64 CG operator: the true residual b - A x is
+  // recomputed in FP64 every cg_true_residual_every iterations and before
+  // every stop on convergence, and replaces the recursive residual
+  // (0: only the final true residual after the solve).
+  int cg_true_residual_every = 32;
   // Block-smoother levels (and a 3x3-block CG operator) are relabelled and
   // packed on the GPU from one upload of the level matrix, with the same
   // layouts as the host packers; off: packed on the host.

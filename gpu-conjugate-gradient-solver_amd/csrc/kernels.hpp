@@ -79,16 +79,28 @@ struct Ctl {
   int64_t iter;       // loop index of the next iteration
   int64_t maxit;
   int64_t iters_out;  // reported iteration count (reference convention)
-  int32_t stop;       // 0 run, 1 converged, 2 maxit, 3 non-finite, 4 breakdown
+  // 0 run, 1 converged, 2 maxit, 3 non-finite, 4 breakdown, 5 the
+  // recursive residual passed the tolerance and waits for the replaced one
+  // (every kernel but the replacement pair skips; the pair sets 1, 2 or
+  // back to 0, and with 0 the kernels after it finish the iteration)
+  int32_t stop;
   int32_t breakdown;  // 1: p.Ap <= 0, 2: r.z <= 0 (recorded, not fatal)
   T *hist;            // relres^2 per iteration (length maxit + 1) or null
-  // Residual replacement (CG SpMV on the V-cycle-type operator): every
-  // replace_every iterations, and whenever the recursive residual passes
-  // the tolerance, r is recomputed as b - A x with the full-precision
-  // operator and the stop test uses it. 0: off.
+  // Residual replacement (mixed precision): every replace_every
+  // iterations, and whenever the recursive residual passes the tolerance,
+  // r is recomputed as b - A x with the full-precision operator and the
+  // stop test uses it. 0: off.
   int64_t replace_every;
-  int32_t replace;    // set for the current iteration
+  int32_t replace;    // a periodic replacement is due (kept until done)
 };
+
+// The replacement pair runs for a due periodic replacement or a pending
+// convergence check.
+template <class T>
+__device__ __forceinline__ bool replace_runs(const Ctl<T> *ctl) {
+  const int32_t s = ctl->stop;
+  return s == 5 || (s == 0 && ctl->replace);
+}
 
 // Blocks are dispatched round-robin over the eight XCDs; remap so each XCD
 // walks one contiguous range of rows, which keeps stencil neighbours of x
@@ -1123,7 +1135,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_rr_final(Ctl<TC> *ctl, const TC *
       ctl->stop = 1;
       ctl->iters_out = it;
     } else if (rel < ctl->tol2) {
-      ctl->replace = 1;  // confirm with the replaced residual
+      ctl->stop = 5;  // confirm with the replaced residual
     } else if (it + 1 >= ctl->maxit) {
       ctl->stop = 2;
       ctl->iters_out = ctl->maxit;
@@ -1139,7 +1151,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace(Sell<TC> A, const TC *__r
                                                        const TC *__restrict__ x,
                                                        TC *__restrict__ r, TM *__restrict__ rm,
                                                        const Ctl<TC> *ctl, TC *partials) {
-  if (ctl->stop || !ctl->replace) return;
+  if (!replace_runs(ctl)) return;
   const int64_t row = xcd_block() * kBlock + threadIdx.x;
   TC v = TC(0);
   if (row < A.rows) {
@@ -1156,7 +1168,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace_b3(Bsr3<TC> A, const TC *
                                                           const TC *__restrict__ x,
                                                           TC *__restrict__ r, TM *__restrict__ rm,
                                                           const Ctl<TC> *ctl, TC *partials) {
-  if (ctl->stop || !ctl->replace) return;
+  if (!replace_runs(ctl)) return;
   const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   TC v = TC(0);
   if (pos < A.nb) {
@@ -1175,11 +1187,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace_b3(Bsr3<TC> A, const TC *
 }
 
 // rr, rel and the stop test from the replaced residual (the iteration was
-// counted by k_cg_rr_final; its history entry is overwritten).
+// counted by k_cg_rr_final; its history entry is overwritten). A pending
+// convergence check that fails resumes the iteration (stop back to 0).
 template <class TC>
 __global__ __launch_bounds__(kBlock) void k_cg_replace_final(Ctl<TC> *ctl, const TC *partials,
                                                              uint32_t G) {
-  if (ctl->stop || !ctl->replace) return;
+  if (!replace_runs(ctl)) return;
   const TC rr = fixed_sum(partials, G);
   if (threadIdx.x == 0) {
     const TC rel = rr / ctl->bb;
@@ -1196,6 +1209,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_replace_final(Ctl<TC> *ctl, const
     } else if (it >= ctl->maxit) {
       ctl->stop = 2;
       ctl->iters_out = ctl->maxit;
+    } else {
+      ctl->stop = 0;
     }
   }
 }

@@ -273,6 +273,11 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     "cg_replace_every" iterations (8) and confirms every
  *                     stop on convergence; needs level 0 smoothed and
  *                     stored one row / block row per thread, else ignored (0)
+ *   "cg_true_residual_every" (GS_PREC_MIXED, FP64 CG operator): the true
+ *                     residual b - A x is recomputed in FP64 every that many
+ *                     iterations and before every stop on convergence, and
+ *                     replaces the recursive residual; 0: only the final
+ *                     true residual after the solve (32)
  * Real options: "omega" (default 1.0); "group_min_scalar" (24) and
  * "group_min_block" (20): operators whose average row length reaches the
  * value (scalar entries per row, 3x3 blocks per block row) are processed
