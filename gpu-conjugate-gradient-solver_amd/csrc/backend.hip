@@ -428,6 +428,24 @@ int gs_divec_destroy(gs_divec_h v) {
   return GS_OK;
 }
 
+int gs_divec_size(gs_divec_h v, int64_t *n) {
+  return bguard([&] {
+    need(v, "index vector");
+    need(n, "output");
+    *n = v->n;
+  });
+}
+
+// src/CPU/CPUDeviceIndexVector.cc:24-31: same size, or an error.
+int gs_divec_update(gs_divec_h v, const int32_t *values, int64_t n) {
+  return bguard([&] {
+    need(v, "index vector");
+    need(values, "values");
+    if (n != v->n) throw std::runtime_error("DeviceIndexVector::updateValues: size mismatch.");
+    GS_HIP(hipMemcpy(v->d.get(), values, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  });
+}
+
 int gs_backend_create_scalar(gs_backend_h b, double value, gs_dscalar_h *out) {
   return bguard([&] {
     need(b, "backend");
@@ -511,6 +529,15 @@ int gs_dcsr_destroy(gs_dcsr_h m) {
   return GS_OK;
 }
 
+int gs_dcsr_info(gs_dcsr_h m, int64_t *rows, int64_t *cols, int64_t *nnz) {
+  return bguard([&] {
+    need(m, "matrix");
+    if (rows) *rows = m->rows;
+    if (cols) *cols = m->cols;
+    if (nnz) *nnz = m->nnz;
+  });
+}
+
 int gs_backend_create_ell(gs_backend_h b, gs_csr_h host, int32_t width, gs_dell_h *out) {
   return bguard([&] {
     need(b, "backend");
@@ -547,6 +574,66 @@ int gs_backend_create_ell(gs_backend_h b, gs_csr_h host, int32_t width, gs_dell_
 int gs_dell_destroy(gs_dell_h m) {
   delete m;
   return GS_OK;
+}
+
+// An ELL matrix already packed on the host (the arrays HostELLMatrix holds,
+// include/GPUSolver/HostELLMatrix.h:19-31: slot-major ELL, overflow CSR).
+// Every column index and the overflow row pointer are checked here, since
+// the SpMV kernel reads x through them unchecked.
+int gs_backend_create_ell_packed(gs_backend_h b, int64_t rows, int64_t cols, int64_t width,
+                                 const int32_t *ell_col, const void *ell_val, int64_t ovf_nnz,
+                                 const int32_t *ovf_ptr, const int32_t *ovf_col,
+                                 const void *ovf_val, gs_dell_h *out) {
+  return bguard([&] {
+    need(b, "backend");
+    need(out, "output");
+    if (rows <= 0 || cols <= 0)
+      throw std::runtime_error("DeviceELLMatrix: matrix dimensions must be nonzero.");
+    if (width <= 0) throw std::runtime_error("DeviceELLMatrix: ELL width must be nonzero.");
+    if (ovf_nnz < 0) throw std::runtime_error("DeviceELLMatrix: negative overflow count.");
+    if (!ell_col || !ell_val || !ovf_ptr || (ovf_nnz > 0 && (!ovf_col || !ovf_val)))
+      throw std::runtime_error("DeviceELLMatrix: storage pointers must not be null.");
+    const int64_t slots = rows * width;
+    for (int64_t i = 0; i < slots; ++i)
+      if (ell_col[i] < 0 || ell_col[i] >= cols)
+        throw std::runtime_error("DeviceELLMatrix: column index out of bounds.");
+    if (ovf_ptr[0] != 0 || ovf_ptr[rows] != ovf_nnz)
+      throw std::runtime_error("DeviceELLMatrix: overflow row pointer does not match its count.");
+    for (int64_t r = 0; r < rows; ++r)
+      if (ovf_ptr[r + 1] < ovf_ptr[r])
+        throw std::runtime_error("DeviceELLMatrix: invalid overflow row pointer.");
+    for (int64_t k = 0; k < ovf_nnz; ++k)
+      if (ovf_col[k] < 0 || ovf_col[k] >= cols)
+        throw std::runtime_error("DeviceELLMatrix: column index out of bounds.");
+    const size_t es = b->esize();
+    auto m = std::make_unique<gs_dell_s>();
+    m->be = b;
+    m->rows = rows;
+    m->cols = cols;
+    m->width = width;
+    m->ovf_nnz = ovf_nnz;
+    m->col.upload(ell_col, static_cast<size_t>(slots));
+    m->val.alloc(slots * es);
+    GS_HIP(hipMemcpy(m->val.get(), ell_val, slots * es, hipMemcpyHostToDevice));
+    m->optr.upload(ovf_ptr, static_cast<size_t>(rows + 1));
+    m->ocol.alloc(std::max<int64_t>(ovf_nnz, 1) * sizeof(int32_t));
+    m->oval.alloc(std::max<int64_t>(ovf_nnz, 1) * es);
+    if (ovf_nnz > 0) {
+      GS_HIP(hipMemcpy(m->ocol.get(), ovf_col, ovf_nnz * sizeof(int32_t), hipMemcpyHostToDevice));
+      GS_HIP(hipMemcpy(m->oval.get(), ovf_val, ovf_nnz * es, hipMemcpyHostToDevice));
+    }
+    *out = m.release();
+  });
+}
+
+int gs_dell_info(gs_dell_h m, int64_t *rows, int64_t *cols, int64_t *width, int64_t *ovf_nnz) {
+  return bguard([&] {
+    need(m, "matrix");
+    if (rows) *rows = m->rows;
+    if (cols) *cols = m->cols;
+    if (width) *width = m->width;
+    if (ovf_nnz) *ovf_nnz = m->ovf_nnz;
+  });
 }
 
 // ---- encode* (include/GPUSolver/Backend.h:47-114) --------------------------

@@ -138,6 +138,13 @@ _SIGS.update({
     "gs_dcsr_destroy": (C.c_int, [_vp]),
     "gs_backend_create_ell": (C.c_int, [_vp, _vp, _i32, _VP]),
     "gs_dell_destroy": (C.c_int, [_vp]),
+    "gs_divec_size": (C.c_int, [_vp, C.POINTER(_i64)]),
+    "gs_divec_update": (C.c_int, [_vp, _i32p, _i64]),
+    "gs_dcsr_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "gs_backend_create_ell_packed": (C.c_int, [_vp, _i64, _i64, _i64, _i32p, _vp, _i64, _i32p,
+                                               _vp, _vp, _VP]),
+    "gs_dell_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                               C.POINTER(_i64)]),
     "gs_encode_scale": (C.c_int, [_vp, _vp, C.c_double]),
     "gs_encode_scale_dev": (C.c_int, [_vp, _vp, _vp]),
     "gs_encode_copy": (C.c_int, [_vp, _vp, _vp]),
@@ -800,6 +807,15 @@ class DeviceIndexVector(_Obj):
         check(lib.gs_divec_download(self.h, out))
         return out
 
+    def size(self) -> int:
+        n = _i64()
+        check(lib.gs_divec_size(self.h, C.byref(n)))
+        return n.value
+
+    def update_values(self, values) -> None:
+        a = np.ascontiguousarray(values, np.int32)
+        check(lib.gs_divec_update(self.h, a, a.size))
+
 
 class DeviceScalar(_Obj):
     _destroy = "gs_dscalar_destroy"
@@ -823,9 +839,21 @@ class DeviceCSRMatrix(_Obj):
         a = np.ascontiguousarray(values, np.float64)
         check(lib.gs_dcsr_update_values(self.h, a, a.size))
 
+    def info(self) -> tuple:
+        """(rows, cols, nnz)."""
+        v = [_i64() for _ in range(3)]
+        check(lib.gs_dcsr_info(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
 
 class DeviceELLMatrix(_Obj):
     _destroy = "gs_dell_destroy"
+
+    def info(self) -> tuple:
+        """(rows, cols, ellWidth, overflowNnz)."""
+        v = [_i64() for _ in range(4)]
+        check(lib.gs_dell_info(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
 
 class Encoder(_Obj):
@@ -895,6 +923,22 @@ class Backend:
         ah = A.handle()
         h = _vp()
         check(lib.gs_backend_create_ell(self.h, ah.h, width, C.byref(h)))
+        return DeviceELLMatrix(h, self)
+
+    def create_ell_packed(self, rows: int, cols: int, width: int, ell_col, ell_val, ovf_ptr,
+                          ovf_col, ovf_val) -> DeviceELLMatrix:
+        """An ELL matrix from arrays packed on the host (slot-major ELL, then
+        the overflow CSR), values in the backend's dtype."""
+        ec = np.ascontiguousarray(ell_col, np.int32)
+        ev = np.ascontiguousarray(ell_val, self.np_dtype)
+        op = np.ascontiguousarray(ovf_ptr, np.int32)
+        oc = np.ascontiguousarray(ovf_col, np.int32)
+        ov = np.ascontiguousarray(ovf_val, self.np_dtype)
+        h = _vp()
+        check(lib.gs_backend_create_ell_packed(
+            self.h, rows, cols, width, ec, ev.ctypes.data, oc.size, op,
+            oc.ctypes.data if oc.size else None, ov.ctypes.data if ov.size else None,
+            C.byref(h)))
         return DeviceELLMatrix(h, self)
 
     # encode* (Backend.h:47-114)

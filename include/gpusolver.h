@@ -207,7 +207,7 @@ int gs_hierarchy_load(const char *path, gs_hier_h *out);
 
 /* ------------------------------------------------------------------ */
 /* Solver: CGSolver (src/CGSolver.cc:18-308) with one of the reference's */
-/* preconditioners (src/Preconditioners/*.cc) on the GPU.              */
+/* preconditioners (src/Preconditioners) on the GPU.                  */
 /* ------------------------------------------------------------------ */
 typedef struct gs_solver_s *gs_solver_h;
 int gs_solver_create(int32_t precision, gs_solver_h *out);
@@ -388,6 +388,10 @@ int gs_backend_create_index_vector(gs_backend_h b, int64_t n, const int32_t *val
                                    gs_divec_h *out);
 int gs_divec_download(gs_divec_h v, int32_t *out);
 int gs_divec_destroy(gs_divec_h v);
+/* size() / updateValues(values, n) (DeviceIndexVector.h:14-20); n must be
+ * the vector's size (src/CPU/CPUDeviceIndexVector.cc:24-31). */
+int gs_divec_size(gs_divec_h v, int64_t *n);
+int gs_divec_update(gs_divec_h v, const int32_t *values, int64_t n);
 /* createScalar([value]) (Backend.h:35-37). */
 int gs_backend_create_scalar(gs_backend_h b, double value, gs_dscalar_h *out);
 int gs_dscalar_download(gs_dscalar_h s, double *value);
@@ -399,9 +403,24 @@ int gs_dscalar_data(gs_dscalar_h s, void **ptr);
 int gs_backend_create_csr(gs_backend_h b, gs_csr_h host, gs_dcsr_h *out);
 int gs_dcsr_update_values(gs_dcsr_h m, const double *values, int64_t nnz);
 int gs_dcsr_destroy(gs_dcsr_h m);
+/* rows() / cols() / nnz() (DeviceSparseMatrix.h:8-10); null outputs skipped. */
+int gs_dcsr_info(gs_dcsr_h m, int64_t *rows, int64_t *cols, int64_t *nnz);
 /* createELLMatrix(HostELLMatrix(csr, width)) (Backend.h:41). */
 int gs_backend_create_ell(gs_backend_h b, gs_csr_h host, int32_t width, gs_dell_h *out);
 int gs_dell_destroy(gs_dell_h m);
+/* createELLMatrix(const HostELLMatrix&) (Backend.h:41) from the arrays the
+ * host matrix holds (HostELLMatrix.h:19-31, src/HostELLMatrix.cc:10-206):
+ * rows x width slot-major ELL (index slot * rows + row; padding column 0,
+ * value 0), then the overflow CSR (ovf_ptr: rows + 1 entries; ovf_col,
+ * ovf_val: ovf_nnz entries, may be null when ovf_nnz is 0). Values in the
+ * backend's dtype. Column indices and the overflow row pointer are
+ * checked. */
+int gs_backend_create_ell_packed(gs_backend_h b, int64_t rows, int64_t cols, int64_t width,
+                                 const int32_t *ell_col, const void *ell_val, int64_t ovf_nnz,
+                                 const int32_t *ovf_ptr, const int32_t *ovf_col,
+                                 const void *ovf_val, gs_dell_h *out);
+/* rows() / cols() / ellWidth() / overflowNnz() (DeviceELLMatrix.h:17-21). */
+int gs_dell_info(gs_dell_h m, int64_t *rows, int64_t *cols, int64_t *width, int64_t *ovf_nnz);
 /* encode* (Backend.h:47-114). */
 int gs_encode_scale(gs_encoder_h e, gs_dvec_h x, double alpha);
 int gs_encode_scale_dev(gs_encoder_h e, gs_dvec_h x, gs_dscalar_h alpha);

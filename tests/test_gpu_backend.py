@@ -187,6 +187,50 @@ def test_ell_spmv_matches_csr(width):
 
 
 @pytest.mark.parametrize("dt", DTYPES)
+@pytest.mark.parametrize("width", [3, 7])
+def test_ell_packed_matches_library_packing(dt, width):
+    """createELLMatrix from arrays packed on the host (what HostELLMatrix
+    holds) gives the matrix the library packs itself, bit for bit."""
+    A = gs.poisson3d(12)
+    x = np.random.default_rng(2).uniform(-1, 1, A.rows)
+    be = gs.Backend(dt)
+    ec, ev, op, oc, ov = gs.ell_pack(A, width)
+    P = be.create_ell_packed(A.rows, A.rows, width, ec, ev.astype(dt), op, oc, ov.astype(dt))
+    assert P.info() == (A.rows, A.rows, width, oc.size)
+    y1, y2 = be.create_vector(A.rows), be.create_vector(A.rows)
+    xd = be.create_vector(x)
+    be.spmv(P, xd, y1)
+    be.spmv(be.create_ell_matrix(A, width), xd, y2)
+    np.testing.assert_array_equal(y1.download(), y2.download())
+    np.testing.assert_array_equal(y1.download(),
+                                  orc.spmv(A.ptr, A.col, A.val.astype(dt), x.astype(dt)))
+
+
+def test_ell_packed_checks_indices():
+    be = gs.Backend(np.float32)
+    one = np.ones(2, np.float32)
+    with pytest.raises(gs.GpuSolverError, match="column index out of bounds"):
+        be.create_ell_packed(2, 2, 1, [0, 2], one, [0, 0, 0], [], [])
+    with pytest.raises(gs.GpuSolverError, match="overflow row pointer does not match"):
+        be.create_ell_packed(2, 2, 1, [0, 1], one, [0, 0, 1], [], [])
+    with pytest.raises(gs.GpuSolverError, match="ELL width must be nonzero"):
+        be.create_ell_packed(2, 2, 0, [], [], [0, 0, 0], [], [])
+
+
+def test_device_object_sizes_and_updates():
+    A = gs.poisson2d(9, 7)
+    be = gs.Backend(np.float32)
+    m = be.create_csr_matrix(A)
+    assert m.info() == (A.rows, A.rows, A.nnz)
+    iv = be.create_index_vector(np.arange(5, dtype=np.int32))
+    assert iv.size() == 5
+    iv.update_values([4, 3, 2, 1, 0])
+    np.testing.assert_array_equal(iv.download(), [4, 3, 2, 1, 0])
+    with pytest.raises(gs.GpuSolverError, match="DeviceIndexVector::updateValues: size mismatch"):
+        iv.update_values([1, 2, 3])
+
+
+@pytest.mark.parametrize("dt", DTYPES)
 def test_gs_color_sweep_bitexact_vs_oracle(dt):
     A = gs.poisson2d(40, 30)
     colors, verts, offs, nc = gs.graph_coloring(A)
