@@ -594,6 +594,10 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
         throw std::runtime_error("cg_true_residual_every must be non-negative.");
       o.cg_true_residual_every = static_cast<int>(v);
     }
+    else if (k == "tail_rows") {
+      if (v < 0) throw std::runtime_error("tail_rows must be non-negative.");
+      o.tail_rows = v;
+    }
     else if (k == "zero_start_lower") {
       if (v < 0) throw std::runtime_error("zero_start_lower must be non-negative.");
       o.zero_start_lower = static_cast<int>(v);

@@ -19,6 +19,7 @@
 // float/float as-reference).
 #include "dense.hpp"
 #include "engine.hpp"
+#include "kernels_tail.hpp"
 #include "setup_device.hpp"
 
 #include <algorithm>
@@ -162,6 +163,12 @@ struct Level {
   TM *rhs = nullptr, *x = nullptr;  // bound to own buffers or the CG vectors
   DMat<TM> R, P;                     // transfer to level k + 1
   std::vector<int32_t> q;            // internal index -> level index
+  // Single-workgroup tail (small scalar Gauss-Seidel levels): the level
+  // operator, the transfers to level k + 1 and the colour offsets as
+  // row-contiguous CSR in the internal order.
+  bool tail_ok = false;
+  gsd::DRows<TM> tA, tR, tP;
+  DevBuf tcoff;
 };
 
 template <class TC, class TM>
@@ -227,6 +234,7 @@ class Engine final : public EngineBase {
   void encode_residual(Level<TM> &lv, const int32_t *stop);
   void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop);
   void encode_prolong(int k, bool alias, const int32_t *stop);
+  void encode_tail(const int32_t *stop);
   void capture_graph();
   void apply_preconditioner(const int32_t *stop);
 
@@ -258,6 +266,9 @@ class Engine final : public EngineBase {
   bool cg_pairs_ = false;
   // Kernel nodes per PCG iteration in the captured graph (0: none yet).
   double graph_nodes_per_iter_ = 0.0;
+  // First level of the single-workgroup tail (-1: none) and its LDS bytes.
+  int tail_k0_ = -1;
+  int64_t tail_lds_ = 0;
   // Residual replacement period: mixed precision only (the FP64 operator
   // recomputes r = b - A x and confirms every stop on convergence).
   int64_t replace_every() const {
@@ -439,6 +450,18 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     lv.Alow.build(Low, blocky, key.empty() ? nullptr : key.data(), gmin, gmin);
     lv.has_low = lv.Alow.b3 == lv.A.b3;
   }
+  // Every colour of a tail level fits one round of the workgroup's row
+  // groups (at most kTailNG rows): a colour step in the tail costs about
+  // what one launch of it does, so a level with a larger colour, which
+  // would take several rounds in one workgroup, stays on launches.
+  int64_t widest = 0;
+  for (int c = 0; c < lv.ncolors; ++c) widest = std::max<int64_t>(widest, lv.coff[c + 1] - lv.coff[c]);
+  lv.tail_ok = k > 0 && !lv.direct && lv.kind == kSGS && opt_.tail_rows > 0 &&
+               lv.n <= opt_.tail_rows && lv.ncolors > 0 && widest <= gsk::kTailNG;
+  if (lv.tail_ok) {
+    lv.tA.build(Ai);
+    lv.tcoff.upload(lv.coff);
+  }
   mark("level " + std::to_string(k) + " pack + upload");
   // Stored entries (3x3 blocks for block GS) per colour, for the bytes of a
   // colour launch.
@@ -591,7 +614,34 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
         L_[k].R.build(Ri, opt_.group_min_scalar, opt_.group_max_transfer, opt_.transfer_sigma,
                       lo ? qc_inv.data() : nullptr);
       }
+      if (L_[k].tail_ok && L_[k + 1].tail_ok) {
+        L_[k].tR.build(Ri);
+        L_[k].tP.build(Pi);
+      }
       mark("transfers " + std::to_string(k));
+    }
+  }
+  // The tail: the longest run of tail levels ending at the coarsest whose
+  // transfers were kept, within the kernel's level limit and one
+  // workgroup's LDS (every tail level's vectors and row offsets, and the
+  // staging slots).
+  tail_k0_ = -1;
+  tail_lds_ = 0;
+  if (opt_.precond == kPrecMultigrid) {
+    int k0 = nlev_;
+    int64_t bytes = int64_t(gsk::kTailBlock) * gsk::kTailKS * sizeof(TM);
+    while (k0 - 1 >= 1 && L_[k0 - 1].tail_ok && (k0 == nlev_ || L_[k0 - 1].tR.rows > 0) &&
+           nlev_ - (k0 - 1) <= gsk::kTailMaxLevels &&
+           bytes + gsk::tail_level_bytes<TM>(L_[k0 - 1].n) <= gsk::kTailLdsMax) {
+      bytes += gsk::tail_level_bytes<TM>(L_[k0 - 1].n);
+      --k0;
+    }
+    if (k0 < nlev_) {
+      tail_k0_ = k0;
+      tail_lds_ = bytes;
+      GS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&gsk::k_vcycle_tail<TM>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 static_cast<int>(bytes)));
     }
   }
   // CG operator: level-0 input with the order-keeping permutation, then the
@@ -830,6 +880,11 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
 
 template <class TC, class TM>
 void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed) {
+  if (k == tail_k0_) {
+    if (!zero_x) throw std::logic_error("V-cycle tail entered with a nonzero start.");
+    encode_tail(stop);
+    return;
+  }
   Level<TM> &lv = L_[k];
   if (k == nlev_ - 1) {
     if (lv.direct) {
@@ -861,6 +916,42 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool
   encode_vcycle(k + 1, stop, true, corr_c != nullptr);
   encode_prolong(k, opt_.reference_vcycle && k > 0, stop);
   encode_smooth(lv, lv.post, false, stop);
+}
+
+// Levels tail_k0_ .. nlev_ - 1 in one single-workgroup launch, from x = 0
+// (the state encode_vcycle(tail_k0_, ...) starts from below the finest).
+template <class TC, class TM>
+void Engine<TC, TM>::encode_tail(const int32_t *stop) {
+  gsk::TailArgs<TM> a{};
+  a.nl = nlev_ - tail_k0_;
+  a.coarse_steps = opt_.coarse_steps;
+  // The prolongation into levels below the finest (every tail level).
+  a.alias = opt_.reference_vcycle ? 1 : 0;
+  int64_t off = 0;
+  for (int j = 0; j < a.nl; ++j) {
+    Level<TM> &lv = L_[tail_k0_ + j];
+    gsk::TailLevel<TM> &t = a.L[j];
+    t.A = lv.tA.view();
+    if (j + 1 < a.nl) {
+      t.R = lv.tR.view();
+      t.P = lv.tP.view();
+    }
+    t.diag = lv.diag.template as<TM>();
+    t.coff = lv.tcoff.template as<int64_t>();
+    t.rhs = lv.rhs;
+    t.x = lv.x;
+    t.n = lv.n;
+    t.lds_off = off;
+    off += gsk::tail_level_bytes<TM>(lv.n);
+    t.ncolors = lv.ncolors;
+    t.pre = lv.pre;
+    t.post = lv.post;
+    t.omega = lv.omega;
+  }
+  a.lp_off = off;
+  if (off + int64_t(gsk::kTailBlock) * gsk::kTailKS * static_cast<int64_t>(sizeof(TM)) != tail_lds_)
+    throw std::logic_error("V-cycle tail: LDS layout does not match setup.");
+  gsk::k_vcycle_tail<TM><<<1, gsk::kTailBlock, static_cast<size_t>(tail_lds_), stream_>>>(a, stop);
 }
 
 // rhs_{k+1} = R_k res (and corr_c = 0 when given).
@@ -1210,11 +1301,14 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
       encode_prolong(0, false, nullptr);
     } else if (which == 7) {
       encode_residual(L_[0], nullptr);
+    } else if (which == 8) {
+      encode_tail(nullptr);
     } else {
       encode_smooth(L_[which - 3], 1, false, nullptr);
     }
   };
-  if (which < 0 || which > 7) throw std::runtime_error("time_kernel: unknown kernel id.");
+  if (which < 0 || which > 8) throw std::runtime_error("time_kernel: unknown kernel id.");
+  if (which == 8 && tail_k0_ < 0) throw std::runtime_error("time_kernel: no V-cycle tail.");
   if (which == 7 && (opt_.precond != kPrecMultigrid || nlev_ < 2 || L_[0].direct))
     throw std::runtime_error("time_kernel: no level-0 residual.");
   if ((which == 5 || which == 6) && (opt_.precond != kPrecMultigrid || nlev_ < 2))
@@ -1285,6 +1379,8 @@ void Engine<TC, TM>::level_info(std::vector<LevelInfo> &out) const {
     li.p_group = k + 1 < nlev_ ? lv.P.G : 0;
     li.transfer_b3 = k + 1 < nlev_ && lv.P.b3;
     li.ncolors = lv.ncolors;
+    for (int c = 0; c < lv.ncolors; ++c)
+      li.max_color_rows = std::max<int64_t>(li.max_color_rows, lv.coff[c + 1] - lv.coff[c]);
     li.kind = lv.kind;
     li.direct = lv.direct;
     li.omega = static_cast<double>(lv.omega);
@@ -1303,7 +1399,8 @@ std::string Engine<TC, TM>::describe() const {
   for (size_t k = 0; k < li.size(); ++k) {
     if (k) os << ", ";
     os << "{\"n\": " << li[k].n << ", \"nnz\": " << li[k].nnz << ", \"blocks3\": " << li[k].blocks3
-       << ", \"colors\": " << li[k].ncolors << ", \"kind\": " << li[k].kind
+       << ", \"colors\": " << li[k].ncolors << ", \"max_color_rows\": " << li[k].max_color_rows
+       << ", \"kind\": " << li[k].kind
        << ", \"direct\": " << (li[k].direct ? "true" : "false") << ", \"omega\": " << li[k].omega
        << ", \"pre\": " << li[k].pre << ", \"post\": " << li[k].post
        << ", \"group\": " << li[k].group << ", \"r_group\": " << li[k].r_group
@@ -1313,7 +1410,8 @@ std::string Engine<TC, TM>::describe() const {
   os << "], \"setup_ms\": {";
   for (size_t i = 0; i < setup_ms_.size(); ++i)
     os << (i ? ", " : "") << "\"" << setup_ms_[i].first << "\": " << setup_ms_[i].second;
-  os << "}, \"cg_vcycle_operator\": " << (cg_vop_ ? "true" : "false")
+  os << "}, \"tail_level\": " << tail_k0_ << ", \"tail_lds_bytes\": " << tail_lds_
+     << ", \"cg_vcycle_operator\": " << (cg_vop_ ? "true" : "false")
      << ", \"replace_every\": " << replace_every()
      << ", \"cg_pair_layout\": " << (cg_pairs_ ? "true" : "false")
      << ", \"cg_block3\": " << (Acg_.b3 ? "true" : "false")

@@ -63,6 +63,10 @@ struct Options {
   // earlier-colour entries only, read by the forward colours of a sweep
   // from x = 0 (0: none).
   int zero_start_lower = 1;
+  // Scalar Gauss-Seidel levels below the finest with at most this many
+  // unknowns, down to the coarsest, run their part of the V-cycle as one
+  // single-workgroup kernel (same results); 0: one launch per colour.
+  int64_t tail_rows = 4096;
   // Mixed precision: the CG SpMV reads level 0's V-cycle-type operator
   // (FP32 values, products and sums in FP64) instead of the FP64 copy; the
   // FP64 operator recomputes r = b - A x every cg_replace_every iterations
@@ -123,6 +127,7 @@ struct SolveResult {
 struct LevelInfo {
   int64_t n = 0, nnz = 0, blocks3 = 0;
   int ncolors = 0, kind = 0;
+  int64_t max_color_rows = 0;  // rows (block rows) of the largest colour
   bool direct = false;
   double omega = 1.0;
   int pre = 1, post = 1;

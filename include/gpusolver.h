@@ -253,6 +253,12 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     as planes of value pairs (16-byte loads) and its SpMV
  *                     runs as a persistent launch over the slices (row sums
  *                     unchanged; p.Ap grouped by workgroup) (1)
+ *   "tail_rows"       the scalar Gauss-Seidel levels below the finest with
+ *                     at most this many unknowns, down to the coarsest, run
+ *                     their part of the V-cycle (sweeps, residual,
+ *                     transfers, coarse sweeps) as one single-workgroup
+ *                     kernel instead of one launch per colour; same results
+ *                     bit for bit; 0: off (4096)
  *   "zero_start_lower" k: Gauss-Seidel levels below index k keep a second
  *                     operator with only the entries of earlier colours; the
  *                     forward colours of a sweep from x = 0 read it (the
@@ -342,7 +348,8 @@ int gs_solver_spmv(gs_solver_h s, int64_t n, const double *x, double *y);
 /* Average ms per launch of a hot kernel timed with HIP events on the
  * solver's stream: 0 CG SpMV, 1 V-cycle, 2 PCG iteration, 3 / 4 level-0 /
  * level-1 SGS or block-GS sweep, 5 level-0 restriction, 6 level-0
- * prolongation, 7 level-0 residual (rhs - A x in the V-cycle type).
+ * prolongation, 7 level-0 residual (rhs - A x in the V-cycle type), 8 the
+ * single-workgroup V-cycle tail (its input as the last V-cycle left it).
  * *bytes: algorithmic bytes per launch. */
 int gs_solver_time_kernel(gs_solver_h s, int32_t which, int32_t reps, double *ms, double *bytes);
 int gs_solver_describe(gs_solver_h s, char *buf, int64_t cap);
