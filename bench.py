@@ -111,6 +111,14 @@ def progress(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def kernel_time(s, which: int, reps: int):
+    """(ms per launch, algorithmic bytes) of one of the solver's kernels:
+    the median of three batches of `reps` back-to-back launches timed with
+    HIP events, so one disturbed batch does not set the figure."""
+    runs = [s.time_kernel(which, reps) for _ in range(3)]
+    return float(np.median([r[0] for r in runs])), runs[0][1]
+
+
 def copy_bandwidth(mib: int = 512, reps: int = 10) -> float:
     """GB/s of a plain device copy (one FP64 element per thread, the kernel
     the counter calibration uses), `reps` copies of `mib` MiB back to back
@@ -289,7 +297,7 @@ def comparison_table(config, modes, t_hier_ms):
 
 
 def spmv_entry(s, reps=50):
-    ms, by = s.time_kernel(0, reps)
+    ms, by = kernel_time(s, 0, reps)
     gbs = by / ms / 1e6
     return {"avg_launch_ms": round(ms, 5), "algorithmic_bytes": by, "achieved_GBs": round(gbs, 1),
             "frac": round(gbs / HBM_PEAK_GBS, 4)}
@@ -435,14 +443,14 @@ def main() -> None:
     # headline kernel with the largest time per PCG iteration.
     hs = solvers[head]
     hdesc = hs.describe()
-    spmv_ms, spmv_bytes = plain.time_kernel(0, 50)
+    spmv_ms, spmv_bytes = kernel_time(plain, 0, 50)
     cands = [{"id": "cg_spmv", "kernel": "CG operator SpMV (FP64, plain mode)", "ms": spmv_ms,
               "bytes": spmv_bytes, "launches": 1, "per_iteration_ms": spmv_ms}]
     for lvl in (0, 1):
         if lvl >= len(hdesc["levels"]) or hdesc["levels"][lvl]["direct"]:
             continue
         lv = hdesc["levels"][lvl]
-        sw_ms, sw_bytes = hs.time_kernel(3 + lvl, 20)
+        sw_ms, sw_bytes = kernel_time(hs, 3 + lvl, 20)
         cands.append({"id": f"l{lvl}_sweep",
                       "kernel": f"level-{lvl} Gauss-Seidel colour launches ("
                                 f"{'lane groups of %d' % lv['group'] if lv['group'] else 'thread per row'}"
@@ -456,7 +464,7 @@ def main() -> None:
         store = "3x3 blocks" if l0["transfer_b3"] else "scalar"
         for kid, tid, what, grp in (("l0_restrict", 5, "restriction R0 (level 0 -> 1)", l0["r_group"]),
                                     ("l0_prolong", 6, "prolongation P0 (level 1 -> 0)", l0["p_group"])):
-            t_ms, t_bytes = hs.time_kernel(tid, 20)
+            t_ms, t_bytes = kernel_time(hs, tid, 20)
             cands.append({"id": kid,
                           "kernel": f"level-0 {what} ({store}, "
                                     f"{'lane groups of %d' % grp if grp else 'thread per %s, SELL-C-sigma' % unit}"
@@ -465,7 +473,7 @@ def main() -> None:
     if len(hdesc["levels"]) > 1 and not hdesc["levels"][0]["direct"]:
         # Level-0 residual rhs - A x of the V-cycle (the headline mode's
         # operator SpMV, once per V-cycle).
-        r_ms, r_bytes = hs.time_kernel(7, 20)
+        r_ms, r_bytes = kernel_time(hs, 7, 20)
         grp = hdesc["levels"][0]["group"]
         cands.append({"id": "l0_residual",
                       "kernel": f"level-0 residual SpMV rhs - A x ("
