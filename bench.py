@@ -105,6 +105,16 @@ def learned_weights(config: str, H):
     return learned.weights_for(config, H)
 
 
+def learned_smoothers(config: str) -> dict:
+    """Smoother options of the configuration's learned mode: those of the
+    trained kind it uses (for the elasticity workloads the reference's
+    hybrid block smoother -- block Gauss-Seidel on the finest level, block
+    Jacobi below -- when that model is stored); {} without a model."""
+    import learned
+    kind = learned.config_kind(config)
+    return learned.solver_options(kind) if kind else {}
+
+
 def progress(msg: str) -> None:
     """One progress line per finished phase on stderr (stdout carries only
     the result line)."""
@@ -229,17 +239,25 @@ def cpu_baseline(H, b, kinds, omega, coarse_steps):
     return xo, cpu
 
 
-def make_modes(gs, sk, graph_iters):
-    """(name, precision, extra options, learned weights?) of each mode."""
+def make_modes(gs, sk, graph_iters, config):
+    """(name, precision, extra options, learned weights?) of each mode. When
+    the learned mode runs another smoother family (the hybrid block
+    smoother), "mixed_hybrid" runs that family with the plain weights, so
+    the weights' own gain is visible."""
     mixed_kw = dict(sk)
     # FP64 dense direct solve at the first level within max_direct unknowns
     # (the inverse is built on the device); deeper levels are not used.
     mixed_kw["coarse_direct"] = 1
     mixed_kw["direct_truncate"] = 1
-    return [("plain_fp64", gs.PREC_DOUBLE, dict(sk), False),
-            ("fp64_direct", gs.PREC_DOUBLE, mixed_kw, False),
-            ("mixed", gs.PREC_MIXED, mixed_kw, False),
-            ("mixed_learned", gs.PREC_MIXED, mixed_kw, True)]
+    learned_kw = dict(mixed_kw)
+    learned_kw.update(learned_smoothers(config))
+    modes = [("plain_fp64", gs.PREC_DOUBLE, dict(sk), False),
+             ("fp64_direct", gs.PREC_DOUBLE, mixed_kw, False),
+             ("mixed", gs.PREC_MIXED, mixed_kw, False)]
+    if learned_kw != mixed_kw:
+        modes.append(("mixed_hybrid", gs.PREC_MIXED, learned_kw, False))
+    modes.append(("mixed_learned", gs.PREC_MIXED, learned_kw, True))
+    return modes
 
 
 def run_modes(config, H, b, sk, steps, warmup, graph_iters, names):
@@ -248,7 +266,7 @@ def run_modes(config, H, b, sk, steps, warmup, graph_iters, names):
     fallback."""
     import gpusolver as gs
     modes, solvers, sols = {}, {}, {}
-    for name, prec, kw, use_learned in make_modes(gs, sk, graph_iters):
+    for name, prec, kw, use_learned in make_modes(gs, sk, graph_iters, config):
         if name not in names:
             continue
         lw = None
@@ -313,7 +331,7 @@ def other_config(name, steps, warmup, graph_iters, with_oracle):
     H = gs.Hierarchy.smoothed_aggregation(A, eps_strong=0.1, max_levels=10, device=True, **hk)
     t_hier = (time.perf_counter() - t0) * 1e3
     modes, solvers, sols = run_modes(name, H, b, sk, steps, warmup, graph_iters,
-                                     ("plain_fp64", "mixed", "mixed_learned"))
+                                     ("plain_fp64", "mixed", "mixed_hybrid", "mixed_learned"))
     out = {"description": desc, "n": int(A.rows), "nnz": int(A.nnz), "levels": H.levels,
            "hierarchy_ms": round(t_hier, 1),
            "cg_spmv": spmv_entry(solvers["plain_fp64"]),
@@ -353,6 +371,7 @@ def probe(a) -> None:
 
     mk = dict(sk)
     mk.update(coarse_direct=1, direct_truncate=1)
+    mk.update(learned_smoothers(a.config))
     if a.probe == "headline-solve":
         m = gs.Solver(gs.PREC_MIXED, graph_iters=0, **mk)
         m.setup(H)
@@ -427,7 +446,8 @@ def main() -> None:
              f"{t_hier_host:.1f} s on the host")
 
     modes, solvers, sols = run_modes(a.config, H, b, sk, a.steps, a.warmup, a.graph_iters,
-                                     ("plain_fp64", "fp64_direct", "mixed", "mixed_learned"))
+                                     ("plain_fp64", "fp64_direct", "mixed", "mixed_hybrid",
+                                      "mixed_learned"))
     plain = solvers["plain_fp64"]
     head = "mixed_learned" if "mixed_learned" in modes else "mixed"
     hm = modes[head]
@@ -450,13 +470,22 @@ def main() -> None:
         if lvl >= len(hdesc["levels"]) or hdesc["levels"][lvl]["direct"]:
             continue
         lv = hdesc["levels"][lvl]
+        if lv["pre"] + lv["post"] == 0:
+            continue  # an unsmoothed level
         sw_ms, sw_bytes = kernel_time(hs, 3 + lvl, 20)
-        cands.append({"id": f"l{lvl}_sweep",
-                      "kernel": f"level-{lvl} Gauss-Seidel colour launches ("
-                                f"{'lane groups of %d' % lv['group'] if lv['group'] else 'thread per row'}"
-                                f", FP32, headline mode)",
-                      "ms": sw_ms, "bytes": sw_bytes, "launches": 2 * lv["colors"] - 1,
-                      "per_iteration_ms": sw_ms * (lv["pre"] + lv["post"])})
+        store = 'lane groups of %d' % lv['group'] if lv['group'] else 'thread per row'
+        if lv["kind"] in (1, 3):  # Gauss-Seidel: one launch per colour and direction
+            cands.append({"id": f"l{lvl}_sweep",
+                          "kernel": f"level-{lvl} Gauss-Seidel colour launches ({store}, FP32, "
+                                    f"headline mode)",
+                          "ms": sw_ms, "bytes": sw_bytes, "launches": 2 * lv["colors"] - 1,
+                          "per_iteration_ms": sw_ms * (lv["pre"] + lv["post"])})
+        else:  # Jacobi: the sweep and its copy back; the first sweep from 0 reads no matrix
+            cands.append({"id": f"l{lvl}_jacobi",
+                          "kernel": f"level-{lvl} Jacobi sweep + copy ({store}, FP32, "
+                                    f"headline mode)",
+                          "ms": sw_ms, "bytes": sw_bytes, "launches": 2,
+                          "per_iteration_ms": sw_ms * (lv["pre"] + lv["post"] - 1)})
     if len(hdesc["levels"]) > 1:
         # Level-0 transfers of the headline solver, once each per V-cycle.
         l0 = hdesc["levels"][0]

@@ -161,6 +161,10 @@ struct Level {
   int64_t sym_nt = 0, sym_kmax = 0, sym_nstrips = 0;
   DevBuf rhs_buf, x_buf, res, tmp;
   TM *rhs = nullptr, *x = nullptr;  // bound to own buffers or the CG vectors
+  // Where the level's smoothed x lies after its last encoded smoothing: x,
+  // or (Jacobi levels below the finest) the second buffer tmp, which the
+  // prolongation then reads instead of copying it back.
+  TM *xout = nullptr;
   DMat<TM> R, P;                     // transfer to level k + 1
   std::vector<int32_t> q;            // internal index -> level index
   // Single-workgroup tail (small scalar Gauss-Seidel levels): the level
@@ -228,8 +232,10 @@ class Engine final : public EngineBase {
   // zero_x: the level's x starts at zero; x_zeroed: and that zero is
   // already in memory (written by the restriction).
   void encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed = false);
+  // may_leave: a Jacobi result may stay in lv.tmp (lv.xout says where it
+  // is); only for smoothing whose result the prolongation reads next.
   void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
-                     bool x_zeroed = false);
+                     bool x_zeroed = false, bool may_leave = false);
   void encode_iteration(const int32_t *stop, bool with_replace = true);
   void encode_residual(Level<TM> &lv, const int32_t *stop);
   void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop);
@@ -788,9 +794,10 @@ static std::vector<std::pair<int, int>> sweep_sequence(int ncolors, int iters) {
 
 template <class TC, class TM>
 void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
-                                   bool x_zeroed) {
+                                   bool x_zeroed, bool may_leave) {
   const int64_t n = lv.n;
   bool x_ready = !zero_x;
+  lv.xout = lv.x;
   if ((lv.kind == kSGS || lv.kind == kBlockGS) && iters > 0) {
     const bool block = lv.kind == kBlockGS;
     if (block && !lv.A.b3)
@@ -844,36 +851,52 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
       }
     }
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {
-    if (!x_ready && iters > 0) {
-      if (!x_zeroed) launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
-      x_ready = true;
-    }
     TM *dinv = lv.dinv.template as<TM>(), *tmp = lv.tmp.template as<TM>();
-    for (int it = 0; it < iters; ++it) {
+    int it0 = 0;
+    if (!x_ready && iters > 0) {
+      // First sweep from x = 0 in closed form (k_jacobi_zero).
+      if (lv.kind == kDampedJacobi)
+        gsk::k_jacobi_zero<TM, 1><<<grid_for(n), kBlock, 0, stream_>>>(dinv, lv.rhs, lv.x, n,
+                                                                       lv.omega, stop);
+      else
+        gsk::k_jacobi_zero<TM, 3><<<grid_for(n / 3), kBlock, 0, stream_>>>(dinv, lv.rhs, lv.x,
+                                                                           n / 3, lv.omega, stop);
+      x_ready = true;
+      it0 = 1;
+    }
+    // Sweeps alternate between x and tmp (each reads one and writes the
+    // other); a result left in tmp is copied back unless may_leave.
+    TM *cur = lv.x, *oth = tmp;
+    for (int it = it0; it < iters; ++it) {
       if (lv.A.G) {
         with_group(lv.A.G, [&](auto g) {
           constexpr int G = decltype(g)::value;
           if (lv.kind == kDampedJacobi && !lv.A.b3)
             gsk::k_jacobi_w<TM, G><<<gsk::grid_rows<G>(n), kBlock, 0, stream_>>>(
-                lv.A.w.view(), dinv, lv.rhs, lv.x, tmp, lv.omega, stop);
+                lv.A.w.view(), dinv, lv.rhs, cur, oth, lv.omega, stop);
           else if (lv.kind == kBlockJacobi && lv.A.b3)
             gsk::k_block_jacobi_bw<TM, G><<<gsk::grid_rows<G>(n / 3), kBlock, 0, stream_>>>(
-                lv.A.bw.view(), dinv, lv.rhs, lv.x, tmp, lv.omega, stop);
+                lv.A.bw.view(), dinv, lv.rhs, cur, oth, lv.omega, stop);
           else
             throw std::runtime_error("Jacobi smoother: operator storage does not match the smoother.");
         });
       } else if (lv.kind == kDampedJacobi) {
-        gsk::k_jacobi<TM><<<grid_for(n), kBlock, 0, stream_>>>(
-            lv.A.s.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
+        gsk::k_jacobi<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.A.s.view(), dinv, lv.rhs, cur,
+                                                               oth, lv.omega, stop);
       } else if (lv.A.b3) {
         gsk::k_block_jacobi_b3<TM><<<grid_for(n / 3), kBlock, 0, stream_>>>(
-            lv.A.b.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
+            lv.A.b.view(), dinv, lv.rhs, cur, oth, lv.omega, stop);
       } else {
         gsk::k_block_jacobi<TM><<<grid_for(n / 3), kBlock, 0, stream_>>>(
-            lv.A.s.view(), lv.dinv.template as<TM>(), lv.rhs, lv.x, lv.tmp.template as<TM>(), lv.omega, stop);
+            lv.A.s.view(), dinv, lv.rhs, cur, oth, lv.omega, stop);
       }
-      gsk::k_copy<TM><<<grid_for(n), kBlock, 0, stream_>>>(lv.tmp.template as<TM>(), lv.x, n, stop);
+      std::swap(cur, oth);
     }
+    if (cur != lv.x && !may_leave) {
+      gsk::k_copy<TM><<<grid_for(n), kBlock, 0, stream_>>>(cur, lv.x, n, stop);
+      cur = lv.x;
+    }
+    lv.xout = cur;
   }
   if (!x_ready && !x_zeroed) launch_fill<TM>(lv.x, TM(0), n, stream_, stop);
 }
@@ -895,7 +918,7 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool
       gsk::k_dense_sym_reduce<TM><<<static_cast<uint32_t>(lv.sym_nt), 1024, 0, stream_>>>(
           prow, pcol, lv.n, lv.sym_nt, lv.sym_kmax, lv.x, stop);
     } else {
-      encode_smooth(lv, opt_.coarse_steps, zero_x, stop, x_zeroed);
+      encode_smooth(lv, opt_.coarse_steps, zero_x, stop, x_zeroed, k > 0);
     }
     return;
   }
@@ -910,12 +933,13 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool
   // zeroes it itself when it is a zero-start Gauss-Seidel colour.
   const bool coarse_self_zero =
       !cl.direct && ((k + 1 == nlev_ - 1) ? opt_.coarse_steps > 0 : cl.pre > 0) &&
-      (cl.kind == kSGS || cl.kind == kBlockGS);
+      (cl.kind == kSGS || cl.kind == kBlockGS || cl.kind == kDampedJacobi ||
+       cl.kind == kBlockJacobi);
   TM *corr_c = coarse_self_zero ? nullptr : cl.x;
   encode_restrict(k, res, corr_c, stop);
   encode_vcycle(k + 1, stop, true, corr_c != nullptr);
   encode_prolong(k, opt_.reference_vcycle && k > 0, stop);
-  encode_smooth(lv, lv.post, false, stop);
+  encode_smooth(lv, lv.post, false, stop, false, k > 0);
 }
 
 // Levels tail_k0_ .. nlev_ - 1 in one single-workgroup launch, from x = 0
@@ -991,18 +1015,20 @@ template <class TC, class TM>
 void Engine<TC, TM>::encode_prolong(int k, bool alias, const int32_t *stop) {
   Level<TM> &lv = L_[k];
   Level<TM> &cl = L_[k + 1];
+  // The coarse level's correction where its last smoothing left it.
+  const TM *e = cl.xout ? cl.xout : cl.x;
   if (lv.P.b3) {
     if (lv.P.G)
       with_group(lv.P.G, [&](auto g) {
         with_bk(lv.P.K, [&](auto kk) {
           constexpr int G = decltype(g)::value, K = decltype(kk)::value;
           gsk::k_prolong_bw<TM, G, K><<<gsk::grid_rows<G>(lv.n / 3), kBlock, 0, stream_>>>(
-              lv.P.bw.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+              lv.P.bw.view(), e, lv.x, alias ? 1 : 0, stop);
         });
       });
     else
       gsk::k_prolong_b3<TM><<<grid_for(lv.n / 3), kBlock, 0, stream_>>>(
-          lv.P.sb.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+          lv.P.sb.view(), e, lv.x, alias ? 1 : 0, stop);
     return;
   }
   if (lv.P.G)
@@ -1010,13 +1036,13 @@ void Engine<TC, TM>::encode_prolong(int k, bool alias, const int32_t *stop) {
       with_k(lv.P.K, [&](auto kk) {
         constexpr int G = decltype(g)::value, K = decltype(kk)::value;
         gsk::k_prolong_w<TM, G, K><<<gsk::grid_rows<G>(lv.n), kBlock, 0, stream_>>>(
-            lv.P.w.view(), cl.x, lv.x, alias ? 1 : 0, stop);
+            lv.P.w.view(), e, lv.x, alias ? 1 : 0, stop);
       });
     });
   else if (alias)
-    gsk::k_prolong_alias<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), cl.x, lv.x, stop);
+    gsk::k_prolong_alias<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), e, lv.x, stop);
   else
-    gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), cl.x, lv.x, stop);
+    gsk::k_prolong_add<TM><<<grid_for(lv.n), kBlock, 0, stream_>>>(lv.P.s.view(), e, lv.x, stop);
 }
 
 template <class TC, class TM>
@@ -1271,7 +1297,8 @@ void Engine<TC, TM>::spmv_level_order(const double *x, double *y) {
 // Average duration (ms) of one launch of a hot kernel, timed with HIP events
 // on the solver's own stream over `reps` back-to-back launches. which: 0 =
 // CG SpMV (y = A p), 1 = one V-cycle, 2 = one full PCG iteration, 3 / 4 =
-// one symmetric (forward + backward) Gauss-Seidel sweep of level 0 / 1.
+// one smoothing sweep of level 0 / 1 from a nonzero x (Gauss-Seidel: the
+// symmetric forward + backward sweep; Jacobi: the sweep and its copy back).
 // *bytes receives the algorithmic bytes of one launch (0 when not defined);
 // for a sweep, the sum over its colour launches of the colour's matrix
 // entries and its rows' vector traffic (rhs, x read and write, diagonal or
@@ -1313,10 +1340,8 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
     throw std::runtime_error("time_kernel: no level-0 residual.");
   if ((which == 5 || which == 6) && (opt_.precond != kPrecMultigrid || nlev_ < 2))
     throw std::runtime_error("time_kernel: no level-0 transfer.");
-  if ((which == 3 || which == 4) &&
-      (which - 3 >= nlev_ || L_[which - 3].direct ||
-       (L_[which - 3].kind != kSGS && L_[which - 3].kind != kBlockGS)))
-    throw std::runtime_error("time_kernel: that level has no Gauss-Seidel sweep.");
+  if ((which == 3 || which == 4) && (which - 3 >= nlev_ || L_[which - 3].direct))
+    throw std::runtime_error("time_kernel: that level has no smoother sweep.");
   if (which == 0) *bytes = Acg_.spmv_bytes();
   // Level-0 residual: the operator's SpMV bytes plus the right-hand side read.
   if (which == 7) *bytes = L_[0].A.spmv_bytes() + L_[0].n * static_cast<double>(sizeof(TM));
@@ -1330,7 +1355,16 @@ double Engine<TC, TM>::time_kernel(int which, int reps, double *bytes) {
     *bytes = entries + (which == 5 ? L_[0].n * s + 2.0 * L_[1].n * s
                                    : L_[1].n * s + 2.0 * L_[0].n * s);
   }
-  if (which == 3 || which == 4) {
+  if ((which == 3 || which == 4) &&
+      (L_[which - 3].kind == kDampedJacobi || L_[which - 3].kind == kBlockJacobi)) {
+    // One Jacobi sweep into the second buffer (entries, rhs, x, the new x,
+    // the inverse diagonal or its 3x3 blocks) and the copy back.
+    const double s = sizeof(TM);
+    const auto &lv = L_[which - 3];
+    const double n = static_cast<double>(lv.n);
+    const double entries = lv.A.b3 ? lv.A.nblocks * (9 * s + 4) : lv.A.nnz * (s + 4);
+    *bytes = entries + 3 * n * s + (lv.kind == kBlockJacobi ? 3 * n * s : n * s) + 2 * n * s;
+  } else if (which == 3 || which == 4) {
     const double s = sizeof(TM);
     const auto &lv = L_[which - 3];
     const bool blk = lv.kind == kBlockGS;

@@ -734,6 +734,45 @@ __global__ __launch_bounds__(kBlock) void k_jacobi(Sell<T> A, const T *__restric
   xo[row] = xi + omega * c;
 }
 
+// First Jacobi sweep from x = 0 (damped, scalar or 3x3 block): every
+// product a_ij * 0 is +-0 and a row sum that starts at +0 never is -0, so
+// b - A x == b bit for bit and the sweep is x = 0 + w * (D^-1 b) with the
+// reference's operations in the reference's order -- no matrix read, no
+// second buffer. unit = 1 (scalar, dinv one per row) or 3 (dinv 9 per
+// block row).
+template <class T, int UNIT>
+__global__ __launch_bounds__(kBlock) void k_jacobi_zero(const T *__restrict__ dinv,
+                                                        const T *__restrict__ rhs, T *x,
+                                                        int64_t units, T omega,
+                                                        const int32_t *stop) {
+  const int64_t u = xcd_block() * kBlock + threadIdx.x;
+  if (u >= units) return;
+  if (UNIT == 1) {
+    const T bi = rhs[u], di = dinv[u];
+    if (stop_word(stop)) return;
+    T c = bi - T(0);
+    c = c * di;
+    x[u] = T(0) + omega * c;
+  } else {
+    T r[3], bi[9];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) r[l] = rhs[3 * u + l];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bi[e] = dinv[9 * u + e];
+    if (stop_word(stop)) return;
+#pragma unroll
+    for (int l = 0; l < 3; ++l) r[l] = r[l] - T(0);
+#pragma unroll
+    for (int l = 0; l < 3; ++l) {
+      T s = T(0);
+      s += bi[3 * l + 0] * r[0];
+      s += bi[3 * l + 1] * r[1];
+      s += bi[3 * l + 2] * r[2];
+      x[3 * u + l] = T(0) + omega * s;
+    }
+  }
+}
+
 // 3x3 block Jacobi sweep into a second buffer
 // (src/Smoothers/BlockJacobiSmoother.cc:17-87).
 template <class T>

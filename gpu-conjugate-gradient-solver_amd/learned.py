@@ -37,7 +37,7 @@ import gpusolver as gs  # noqa: E402
 MODEL_PATH = HERE / "learned_weights.json"
 # Training / evaluation problem sizes: varcoef3d grid edge, elasticity
 # vertices per edge.
-DEFAULT_SIZE = {"sgs": 64, "bgs": 48}
+DEFAULT_SIZE = {"sgs": 64, "bgs": 48, "hbgs": 48}
 HIDDEN = 64
 
 # Search grids per smoother kind; the plain weight is the reference's
@@ -47,13 +47,45 @@ HIDDEN = 64
 # level's smoothing off: its restricted residual goes straight down and the
 # prolongated correction straight back, which keeps the V-cycle symmetric;
 # for block GS this is the reference's own 0-sweep smoother).
+# "hbgs" is the reference's hybrid block smoother
+# (MultigridHybridBlockPreconditioner, src/Preconditioners/
+# MultigridHybridBlockPreconditioner.cc:9-31, used by
+# examples/elasticity/cantileverBeam.cc:589): block Gauss-Seidel on the
+# finest level and block Jacobi below it, each level's (w, sweeps) learned;
+# its `reference` kind is the all-block-Gauss-Seidel method the benchmark's
+# plain mode runs.
 KINDS = {
     "sgs": dict(smoother=gs.SGS, plain_omega=1.0, omegas=[0.7, 0.85, 1.0, 1.15, 1.3, 1.45, 1.6],
                 sweeps=[1, 2], coarse_sweeps=[0, 1, 2], block=1, coarse_steps=2),
     "bgs": dict(smoother=gs.BLOCK_GS, plain_omega=0.7,
                 omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2],
                 coarse_sweeps=[0, 1, 2], block=3, coarse_steps=0),
+    "hbgs": dict(smoother=gs.BLOCK_GS, coarse_smoother=gs.BLOCK_JACOBI, plain_omega=0.7,
+                 omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2],
+                 coarse_sweeps=[0, 1, 2], block=3, coarse_steps=0, reference="bgs"),
 }
+
+
+def solver_options(kind: str) -> dict:
+    """The smoother options of a kind's solver (the finest level's smoother
+    and, for the hybrid, the coarse levels' one)."""
+    return solver_options_of(KINDS[kind])
+
+
+def config_kind(config: str, path: Path = MODEL_PATH):
+    """The trained kind a benchmark configuration's learned mode uses: the
+    hybrid block smoother for the elasticity configurations when a hybrid
+    model is stored (else block Gauss-Seidel), SGS for the scalar ones;
+    None without a stored model."""
+    if not Path(path).exists():
+        return None
+    d = json.loads(Path(path).read_text())
+    if "elasticity" in config:
+        for kind in ("hbgs", "bgs"):
+            if kind in d:
+                return kind
+        return None
+    return "sgs" if "sgs" in d else None
 
 
 def training_problem(kind: str, seed: int, size: int):
@@ -109,13 +141,21 @@ def work_per_iteration(nnz: np.ndarray, sweeps) -> float:
     return w
 
 
+def solver_options_of(spec) -> dict:
+    """solver_options for a kind's spec."""
+    o = dict(smoother=spec["smoother"])
+    if "coarse_smoother" in spec:
+        o["coarse_smoother"] = spec["coarse_smoother"]
+    return o
+
+
 def make_solver(H, spec, omega, sweeps):
     """The mixed-precision solver of the headline configuration: FP32
     V-cycle, FP64 CG, FP64 direct solve at the first level within
     max_direct unknowns."""
-    s = gs.Solver(gs.PREC_MIXED, smoother=spec["smoother"], coarse_direct=1, direct_truncate=1,
+    s = gs.Solver(gs.PREC_MIXED, coarse_direct=1, direct_truncate=1,
                   coarse_steps=spec["coarse_steps"], omega=spec["plain_omega"], graph_iters=8,
-                  cg_block3=1 if spec["block"] == 3 else 0)
+                  cg_block3=1 if spec["block"] == 3 else 0, **solver_options_of(spec))
     s.setup(H)
     s.set_level_weights(omega=omega, sweeps=sweeps)
     return s
@@ -177,22 +217,32 @@ def load_model(kind: str, path: Path = MODEL_PATH):
     return net
 
 
+def feature_range(rows):
+    """Per-feature (min, max) of the labelled levels a model is fit on."""
+    F = np.concatenate([np.array(r["features"], np.float64) for r in rows])
+    return F.min(axis=0).tolist(), F.max(axis=0).tolist()
+
+
 def predict(H, kind: str, path: Path = MODEL_PATH):
     """Per-level (omega, sweeps) for hierarchy H from the trained model,
-    inference on the GPU."""
+    inference on the GPU. A model that stores its training range sees each
+    feature clipped to it: outside the labelled problems (e.g. a uniform
+    coefficient, contrast 0, for a model trained on heterogeneous ones) the
+    network answers as at the nearest training edge instead of
+    extrapolating."""
     spec = KINDS[kind]
     net = load_model(kind, path)
-    F = H.features(spec["block"])
+    F = np.asarray(H.features(spec["block"]), np.float64)
+    m = json.loads(Path(path).read_text())[kind]
+    if "feature_min" in m:
+        F = np.clip(F, np.asarray(m["feature_min"]), np.asarray(m["feature_max"]))
     omega, sweeps = decode(net.forward(normalize(F)), min(spec["coarse_sweeps"]))
     return omega, sweeps
 
 
 def weights_for(config: str, H):
-    kind = "bgs" if "elasticity" in config else "sgs"
-    if not MODEL_PATH.exists():
-        return None
-    d = json.loads(MODEL_PATH.read_text())
-    if kind not in d:
+    kind = config_kind(config)
+    if kind is None:
         return None
     return predict(H, kind)
 
@@ -270,6 +320,7 @@ def fit(kind: str, label_paths, epochs: int, out: Path) -> dict:
            "loss_initial": l0, "loss_final": loss,
            "label_time_gain_mean": float(np.mean([r["plain_ms"] / r["best_ms"] for r in rows])),
            "constant_policy": constant_policy(rows, spec)}
+    res["feature_min"], res["feature_max"] = feature_range(rows)
     doc = json.loads(out.read_text()) if out.exists() else {}
     doc[kind] = res
     out.write_text(json.dumps(doc))
@@ -307,15 +358,18 @@ def evaluate(kind: str, seeds, size: int, path: Path) -> dict:
         A, b, H = training_problem(kind, seed, size)
         L = H.levels
         om, sw = predict(H, kind, path)
-        policies = [("plain", [spec["plain_omega"]] * L, [1] * L),
-                    ("learned", om.tolist(), sw.tolist())]
+        policies = [("plain", spec, [spec["plain_omega"]] * L, [1] * L),
+                    ("learned", spec, om.tolist(), sw.tolist())]
         cw = constant_weights(kind, L, path)
         if cw is not None:
-            policies.append(("constant", cw[0], cw[1]))
+            policies.append(("constant", spec, cw[0], cw[1]))
+        if "reference" in spec:
+            ref = KINDS[spec["reference"]]
+            policies.append(("plain_reference", ref, [ref["plain_omega"]] * L, [1] * L))
         nnz = level_nnz(H)
         res = {}
-        for name, omk, swk in policies:
-            s = make_solver(H, spec, omk, swk)
+        for name, sp, omk, swk in policies:
+            s = make_solver(H, sp, omk, swk)
             s.solve(b, tol=1e-8, maxit=1000)
             best = None
             for _ in range(2):
@@ -335,6 +389,10 @@ def evaluate(kind: str, seeds, size: int, path: Path) -> dict:
     agg = {k: float(np.mean([r["plain"][k] / r["learned"][k] for r in rows]))
            for k in ("iterations", "ms", "work")}
     out = {"per_seed": rows, "mean_ratio_plain_over_learned": agg}
+    if all("plain_reference" in r for r in rows):
+        out["mean_ratio_plain_reference_over_learned"] = {
+            k: float(np.mean([r["plain_reference"][k] / r["learned"][k] for r in rows]))
+            for k in ("iterations", "ms")}
     if all("constant" in r for r in rows):
         out["mean_ratio_constant_over_learned"] = {
             k: float(np.mean([r["constant"][k] / r["learned"][k] for r in rows]))

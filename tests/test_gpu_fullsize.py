@@ -90,21 +90,26 @@ def test_elasticity64_plain_fp64(elasticity64):
     check(x, info, xo, ito, A, b)
 
 
-def test_elasticity64_mixed_learned_shape(elasticity64):
-    """The headline configuration on a full-size elasticity problem: mixed
-    precision, direct solve at the first level within max_direct, level 1
-    unsmoothed (the learned model's choice), against the oracle running the
+@pytest.mark.parametrize("coarse,level1", [(gs.BLOCK_GS, 0), (gs.BLOCK_JACOBI, 1)])
+def test_elasticity64_mixed_learned_shape(elasticity64, coarse, level1):
+    """The learned configurations' shape on a full-size elasticity problem:
+    mixed precision, direct solve at the first level within max_direct, and
+    level 1 either block Gauss-Seidel left unsmoothed or block Jacobi with
+    one sweep (the hybrid block smoother), against the oracle running the
     same method."""
     A, H, b = elasticity64
-    s = gs.Solver(gs.PREC_MIXED, smoother=gs.BLOCK_GS, coarse_steps=0, omega=0.7, cg_block3=1,
-                  coarse_direct=1, direct_truncate=1)
+    s = gs.Solver(gs.PREC_MIXED, smoother=gs.BLOCK_GS, coarse_smoother=coarse, coarse_steps=0,
+                  omega=0.7, cg_block3=1, coarse_direct=1, direct_truncate=1)
     s.setup(H)
     used = len(s.describe()["levels"])
-    sweeps = [1, 0] + [1] * (H.levels - 2)
-    s.set_level_weights(omega=[0.7] * H.levels, sweeps=sweeps)
+    sweeps = [1, level1] + [1] * (H.levels - 2)
+    omega = [0.96, 0.9] + [0.7] * (H.levels - 2)
+    s.set_level_weights(omega=omega, sweeps=sweeps)
+    say(f"elasticity_64 mixed, level 1 {'block Jacobi' if level1 else 'unsmoothed'}")
     x, info = s.solve(b, tol=1e-8, maxit=2000)
+    kinds = [gs.BLOCK_GS] + [coarse] * (used - 1)
     xo, ito, _, _ = orc.pcg(hier_view(H, used), b, tol=1e-8, maxit=2000, mg_double=False,
-                            kinds=gs.BLOCK_GS, omegas=0.7, pres=sweeps[:used],
+                            kinds=kinds, omegas=omega[:used], pres=sweeps[:used],
                             posts=sweeps[:used], coarse_steps=0, direct_coarse=True)
     check(x, info, xo, ito, A, b)
 
@@ -180,27 +185,31 @@ def test_hetero96_plain_fp64(hetero96):
 
 def test_hetero96_mixed_learned(hetero96):
     """The headline mode: FP32 V-cycle in FP64 CG, FP64 direct solve at the
-    first level within max_direct (13 047 unknowns), the learned per-level
-    block-GS weights; the oracle runs the same method with the same
-    weights."""
+    first level within max_direct (13 047 unknowns), the smoother family of
+    the stored model (the hybrid block smoother: block GS on level 0, block
+    Jacobi below) with its learned per-level weights; the oracle runs the
+    same method with the same weights."""
     A, H, b = hetero96
-    s = gs.Solver(gs.PREC_MIXED, smoother=gs.BLOCK_GS, coarse_steps=0, omega=0.7, cg_block3=1,
-                  coarse_direct=1, direct_truncate=1)
+    kind = learned.config_kind("hetero_elasticity_96")
+    assert kind is not None, "no trained elasticity model"
+    opts = learned.solver_options(kind)
+    s = gs.Solver(gs.PREC_MIXED, coarse_steps=0, omega=0.7, cg_block3=1, coarse_direct=1,
+                  direct_truncate=1, **opts)
     s.setup(H)
     used = len(s.describe()["levels"])
     lw = learned.weights_for("hetero_elasticity_96", H)
-    assert lw is not None, "no trained block-GS model"
     omega, sweeps = np.asarray(lw[0])[:used], np.asarray(lw[1])[:used]
     s.set_level_weights(omega=omega, sweeps=sweeps)
-    say(f"hetero_elasticity_96 mixed+learned: levels used {used} omega "
+    say(f"hetero_elasticity_96 mixed+learned ({kind}): levels used {used} omega "
         f"{np.round(omega, 3).tolist()} sweeps {sweeps.tolist()}")
     x, info = s.solve(b, tol=1e-8, maxit=2000)
     del s
     Ac = H.get("A", used - 1).to_scipy().toarray()
     inv = np.linalg.inv(Ac)
     del Ac
+    kinds = [opts["smoother"]] + [opts.get("coarse_smoother", opts["smoother"])] * (used - 1)
     xo, ito, _, _ = orc.pcg(hier_view(H, used), b, tol=1e-8, maxit=2000, mg_double=False,
-                            kinds=gs.BLOCK_GS, omegas=omega, pres=sweeps, posts=sweeps,
+                            kinds=kinds, omegas=omega, pres=sweeps, posts=sweeps,
                             coarse_steps=0, direct_coarse=True, coarse_inv=inv)
     check(x, info, xo, ito, A, b)
 

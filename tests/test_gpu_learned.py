@@ -79,26 +79,32 @@ def test_guarded_solve_falls_back_on_bad_weights():
     np.testing.assert_array_equal(x, xp)
 
 
-# Held-out device-time gain each model must show over the plain weights
-# (mean over seeds 200-202 at the model's training size, both in the mixed
-# headline configuration): measured 2.46x (block GS, 48^3) and 1.77x (SGS,
-# 128^3) over seeds 200-219.
-GAIN_BAR = {"bgs": 1.5, "sgs": 1.3}
+# Held-out device-time gain each model must show over the plain weights of
+# the reference's method (mean over seeds 200-202 at the model's training
+# size, all in the mixed headline configuration): measured over seeds
+# 200-219 2.46x (block GS, 48^3), 1.77x (SGS, 128^3) and, for the hybrid
+# block smoother, 2.7x over the all-block-GS method with the plain weights.
+GAIN_BAR = {"bgs": 1.5, "sgs": 1.3, "hbgs": 1.5}
 
 
 def test_learned_weights_gain_over_plain():
     """Held-out seeds (never labelled): the trained network's weights make
-    the mixed solve faster than the plain weights by at least the bar above
-    (device time), every learned solve still reaches 1e-8 in FP64, and the
-    network is compared with the best constant policy of its labels (it
-    must not be more than 3% slower)."""
+    the mixed solve faster than the reference method's plain weights by at
+    least the bar above (device time; for the hybrid block smoother that
+    method is all-level block GS), never slower than its own smoother
+    family's plain weights beyond timing noise, every learned solve still
+    reaches 1e-8 in FP64, and the network is compared with the best
+    constant policy of its labels (it must not be more than 3% slower)."""
     import learned
     if not learned.MODEL_PATH.exists():
         pytest.skip("no trained weights committed yet")
     doc = json.loads(learned.MODEL_PATH.read_text())
     for kind in doc:
         r = learned.evaluate(kind, range(200, 203), doc[kind]["size"], learned.MODEL_PATH)
-        assert r["mean_ratio_plain_over_learned"]["ms"] >= GAIN_BAR[kind], (kind, r)
+        key = ("mean_ratio_plain_reference_over_learned" if "reference" in learned.KINDS[kind]
+               else "mean_ratio_plain_over_learned")
+        assert r[key]["ms"] >= GAIN_BAR[kind], (kind, r)
+        assert r["mean_ratio_plain_over_learned"]["ms"] >= 0.97, (kind, r)
         assert r["mean_ratio_constant_over_learned"]["ms"] >= 0.97, (kind, r)
         for row in r["per_seed"]:
             assert row["learned"]["converged"] and row["learned"]["true_relres"] <= 1.5e-8, row

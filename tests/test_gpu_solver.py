@@ -186,17 +186,45 @@ def test_vcycle_signed_zeros_match_oracle(cfg1, elast, grp):
     assert np.array_equal(zb.view(np.uint64), zbo.view(np.uint64))
 
 
-def test_vcycle_bitexact_hybrid(elast):
+@pytest.mark.parametrize("grp", [-1.0, 0.0])
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_vcycle_bitexact_hybrid(elast, steps, grp):
     """Fine block GS, coarse block Jacobi (MultigridHybridBlockPreconditioner,
-    src/Preconditioners/MultigridHybridBlockPreconditioner.cc:9-61)."""
+    src/Preconditioners/MultigridHybridBlockPreconditioner.cc:9-61), with 1-3
+    sweeps on the coarse levels: the first sweep from x = 0 in closed form,
+    the others alternating between x and the second buffer, a result left
+    in the second buffer read by the prolongation in place. Bit for bit the
+    oracle's V-cycle (which runs every sweep as b - A x)."""
     E, rhs, H = elast
     s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, coarse_smoother=gs.BLOCK_JACOBI,
-                  omega=0.7, coarse_steps=3, group_min_scalar=0, group_min_block=0)
+                  omega=0.7, coarse_steps=steps, **group_opts(grp))
     s.setup(H)
+    sweeps = [1] + [steps] * (H.levels - 1)
+    s.set_level_weights(sweeps=sweeps)
     hv, _ = hier_view(H)
     r = gs.uniform_pm1(6, E.rows)
     kinds = [gs.BLOCK_GS] + [gs.BLOCK_JACOBI] * (H.levels - 1)
-    zo = orc.vcycle(hv, r, kinds=kinds, omegas=0.7, coarse_steps=3)
+    zo = orc.vcycle(hv, r, kinds=kinds, omegas=0.7, pres=sweeps, posts=sweeps,
+                    coarse_steps=steps)
+    np.testing.assert_array_equal(s.vcycle(r), zo)
+
+
+@pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
+@pytest.mark.parametrize("steps", [1, 2])
+def test_vcycle_bitexact_scalar_jacobi_coarse(cfg1, prec, steps):
+    """Scalar levels: SGS on the finest, damped Jacobi below (the coarse
+    levels' Jacobi sweeps as in the block hybrid), 1 or 2 sweeps."""
+    A, H = cfg1
+    s = gs.Solver(prec, smoother=gs.SGS, coarse_smoother=gs.DAMPED_JACOBI, omega=0.8,
+                  coarse_steps=steps, tail_rows=0)
+    s.setup(H)
+    sweeps = [1] + [steps] * (H.levels - 1)
+    s.set_level_weights(sweeps=sweeps)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(16, A.rows)
+    kinds = [gs.SGS] + [gs.DAMPED_JACOBI] * (H.levels - 1)
+    zo = orc.vcycle(hv, r, mg_double=prec == gs.PREC_DOUBLE, kinds=kinds, omegas=0.8,
+                    pres=sweeps, posts=sweeps, coarse_steps=steps)
     np.testing.assert_array_equal(s.vcycle(r), zo)
 
 
