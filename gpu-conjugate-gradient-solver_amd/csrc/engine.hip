@@ -231,14 +231,20 @@ class Engine final : public EngineBase {
   void build_level(int k, const gsx::Csr &Ak_level, bool coarsest);
   // zero_x: the level's x starts at zero; x_zeroed: and that zero is
   // already in memory (written by the restriction).
-  void encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed = false);
+  // first_done: the restriction into level k already wrote its first
+  // Jacobi sweep from x = 0 (closed form).
+  void encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed = false,
+                     bool first_done = false);
   // may_leave: a Jacobi result may stay in lv.tmp (lv.xout says where it
   // is); only for smoothing whose result the prolongation reads next.
   void encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
-                     bool x_zeroed = false, bool may_leave = false);
+                     bool x_zeroed = false, bool may_leave = false, bool first_done = false);
   void encode_iteration(const int32_t *stop, bool with_replace = true);
   void encode_residual(Level<TM> &lv, const int32_t *stop);
-  void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop);
+  // dinv_c: also write the coarse level's closed-form first Jacobi sweep
+  // into corr_c (the k_restrict kernels' dinv_c).
+  void encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop,
+                       const TM *dinv_c = nullptr, TM omega_c = TM(0));
   void encode_prolong(int k, bool alias, const int32_t *stop);
   void encode_tail(const int32_t *stop);
   void capture_graph();
@@ -794,7 +800,7 @@ static std::vector<std::pair<int, int>> sweep_sequence(int ncolors, int iters) {
 
 template <class TC, class TM>
 void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const int32_t *stop,
-                                   bool x_zeroed, bool may_leave) {
+                                   bool x_zeroed, bool may_leave, bool first_done) {
   const int64_t n = lv.n;
   bool x_ready = !zero_x;
   lv.xout = lv.x;
@@ -853,7 +859,10 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {
     TM *dinv = lv.dinv.template as<TM>(), *tmp = lv.tmp.template as<TM>();
     int it0 = 0;
-    if (!x_ready && iters > 0) {
+    if (first_done && iters > 0) {
+      x_ready = true;
+      it0 = 1;
+    } else if (!x_ready && iters > 0) {
       // First sweep from x = 0 in closed form (k_jacobi_zero).
       if (lv.kind == kDampedJacobi)
         gsk::k_jacobi_zero<TM, 1><<<grid_for(n), kBlock, 0, stream_>>>(dinv, lv.rhs, lv.x, n,
@@ -902,7 +911,8 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
 }
 
 template <class TC, class TM>
-void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed) {
+void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool x_zeroed,
+                                   bool first_done) {
   if (k == tail_k0_) {
     if (!zero_x) throw std::logic_error("V-cycle tail entered with a nonzero start.");
     encode_tail(stop);
@@ -918,11 +928,11 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool
       gsk::k_dense_sym_reduce<TM><<<static_cast<uint32_t>(lv.sym_nt), 1024, 0, stream_>>>(
           prow, pcol, lv.n, lv.sym_nt, lv.sym_kmax, lv.x, stop);
     } else {
-      encode_smooth(lv, opt_.coarse_steps, zero_x, stop, x_zeroed, k > 0);
+      encode_smooth(lv, opt_.coarse_steps, zero_x, stop, x_zeroed, k > 0, first_done);
     }
     return;
   }
-  encode_smooth(lv, lv.pre, zero_x, stop, x_zeroed);
+  encode_smooth(lv, lv.pre, zero_x, stop, x_zeroed, false, first_done);
   // With no pre-smoothing from x = 0 the residual is rhs bit for bit (the
   // row sums of A 0 are +0, and rhs - (+0) == rhs), so restrict rhs itself.
   const bool res_is_rhs = zero_x && lv.pre == 0;
@@ -935,9 +945,17 @@ void Engine<TC, TM>::encode_vcycle(int k, const int32_t *stop, bool zero_x, bool
       !cl.direct && ((k + 1 == nlev_ - 1) ? opt_.coarse_steps > 0 : cl.pre > 0) &&
       (cl.kind == kSGS || cl.kind == kBlockGS || cl.kind == kDampedJacobi ||
        cl.kind == kBlockJacobi);
-  TM *corr_c = coarse_self_zero ? nullptr : cl.x;
-  encode_restrict(k, res, corr_c, stop);
-  encode_vcycle(k + 1, stop, true, corr_c != nullptr);
+  // A damped / block Jacobi coarse level that smooths from zero gets its
+  // first sweep from the restriction (same bits, one launch fewer), when
+  // the transfer's rows match the smoother's (scalar rows for damped
+  // Jacobi, 3x3 block rows for block Jacobi).
+  const bool cl_smooths = !cl.direct && ((k + 1 == nlev_ - 1) ? opt_.coarse_steps > 0 : cl.pre > 0);
+  const bool fuse_first = cl_smooths && ((cl.kind == kDampedJacobi && !lv.R.b3) ||
+                                         (cl.kind == kBlockJacobi && lv.R.b3 && cl.A.b3));
+  TM *corr_c = fuse_first ? cl.x : coarse_self_zero ? nullptr : cl.x;
+  encode_restrict(k, res, corr_c, stop, fuse_first ? cl.dinv.template as<TM>() : nullptr,
+                  cl.omega);
+  encode_vcycle(k + 1, stop, true, corr_c != nullptr && !fuse_first, fuse_first);
   encode_prolong(k, opt_.reference_vcycle && k > 0, stop);
   encode_smooth(lv, lv.post, false, stop, false, k > 0);
 }
@@ -980,7 +998,8 @@ void Engine<TC, TM>::encode_tail(const int32_t *stop) {
 
 // rhs_{k+1} = R_k res (and corr_c = 0 when given).
 template <class TC, class TM>
-void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop) {
+void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int32_t *stop,
+                                     const TM *dinv_c, TM omega_c) {
   Level<TM> &lv = L_[k];
   Level<TM> &cl = L_[k + 1];
   if (lv.R.b3) {
@@ -989,12 +1008,12 @@ void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int
         with_bk(lv.R.K, [&](auto kk) {
           constexpr int G = decltype(g)::value, K = decltype(kk)::value;
           gsk::k_restrict_bw<TM, G, K><<<gsk::grid_rows<G>(cl.n / 3), kBlock, 0, stream_>>>(
-              lv.R.bw.view(), res, cl.rhs, corr_c, stop);
+              lv.R.bw.view(), res, cl.rhs, corr_c, stop, dinv_c, omega_c);
         });
       });
     else
-      gsk::k_restrict_b3<TM><<<grid_for(cl.n / 3), kBlock, 0, stream_>>>(lv.R.sb.view(), res,
-                                                                         cl.rhs, corr_c, stop);
+      gsk::k_restrict_b3<TM><<<grid_for(cl.n / 3), kBlock, 0, stream_>>>(
+          lv.R.sb.view(), res, cl.rhs, corr_c, stop, dinv_c, omega_c);
     return;
   }
   if (lv.R.G)
@@ -1002,12 +1021,12 @@ void Engine<TC, TM>::encode_restrict(int k, const TM *res, TM *corr_c, const int
       with_k(lv.R.K, [&](auto kk) {
         constexpr int G = decltype(g)::value, K = decltype(kk)::value;
         gsk::k_restrict_w<TM, G, K><<<gsk::grid_rows<G>(cl.n), kBlock, 0, stream_>>>(
-            lv.R.w.view(), res, cl.rhs, corr_c, stop);
+            lv.R.w.view(), res, cl.rhs, corr_c, stop, dinv_c, omega_c);
       });
     });
   else
     gsk::k_restrict<TM><<<grid_for(cl.n), kBlock, 0, stream_>>>(lv.R.s.view(), res, cl.rhs,
-                                                                corr_c, stop);
+                                                                corr_c, stop, dinv_c, omega_c);
 }
 
 // x_k += P_k x_{k+1} (alias: the reference's coarse-level x = 2 P e).

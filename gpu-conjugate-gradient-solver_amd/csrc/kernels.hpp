@@ -491,19 +491,46 @@ __global__ __launch_bounds__(kBlock) void k_residual_b3(Bsr3<T> A, const T *__re
   res[3 * b + 2] = b2 - v[2];
 }
 
+// First damped Jacobi sweep from x = 0 of one row (k_jacobi_zero, UNIT 1):
+// b - A 0 == b bit for bit, so x = 0 + w ((b - 0) d^-1).
+template <class T>
+__device__ __forceinline__ T jacobi_zero1(T b, T di, T omega) {
+  T c = b - T(0);
+  c = c * di;
+  return T(0) + omega * c;
+}
+// Line l of the first block Jacobi sweep from x = 0 (k_jacobi_zero, UNIT
+// 3): x_l = 0 + w (sum_c Dinv[l][c] (b_c - 0)); d: row l of D_b^-1.
+template <class T>
+__device__ __forceinline__ T jacobi_zero3(const T d[3], T b0, T b1, T b2, T omega) {
+  const T r0 = b0 - T(0), r1 = b1 - T(0), r2 = b2 - T(0);
+  T s = T(0);
+  s += d[0] * r0;
+  s += d[1] * r1;
+  s += d[2] * r2;
+  return T(0) + omega * s;
+}
+
 // rhs_c = R res and corr_c = 0 (the coarse correction is zeroed in the same
-// pass, src/Smoothers/MultigridSmoother.cc:381-388).
+// pass, src/Smoothers/MultigridSmoother.cc:381-388). With dinv_c (a coarse
+// level smoothed by damped Jacobi from x = 0) corr_c instead receives that
+// level's first sweep in closed form (jacobi_zero1 of the row just summed),
+// bit for bit what the sweep would give.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict(Sell<T> R, const T *__restrict__ res,
                                                      T *__restrict__ rhs_c, T *__restrict__ corr_c,
-                                                     const int32_t *stop) {
+                                                     const int32_t *stop,
+                                                     const T *__restrict__ dinv_c = nullptr,
+                                                     T omega_c = T(0)) {
   const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   if (pos >= R.rows) return;
   const SliceHead h = sell_head(R, pos);
   const int64_t row = sell_row_of(R, pos);
+  const T di = dinv_c ? dinv_c[row] : T(0);
   if (stop_word(stop)) return;
-  rhs_c[row] = sell_dot_h(R, h, res);
-  if (corr_c) corr_c[row] = T(0);
+  const T v = sell_dot_h(R, h, res);
+  rhs_c[row] = v;
+  if (corr_c) corr_c[row] = dinv_c ? jacobi_zero1(v, di, omega_c) : T(0);
 }
 
 // x += P e (src/Smoothers/MultigridSmoother.cc:401-404).
@@ -540,11 +567,15 @@ __global__ __launch_bounds__(kBlock) void k_prolong_alias(Sell<T> P, const T *__
 // 3x3-block transfer operators (both levels keep their unknowns in node
 // triples): the same row sums as the scalar CSR rows, block by block in
 // the rows' column order, three entries per block.
+// dinv_c as in k_restrict (a coarse level smoothed by block Jacobi: 9
+// values per block row).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_restrict_b3(Bsr3<T> R, const T *__restrict__ res,
                                                         T *__restrict__ rhs_c,
                                                         T *__restrict__ corr_c,
-                                                        const int32_t *stop) {
+                                                        const int32_t *stop,
+                                                        const T *__restrict__ dinv_c = nullptr,
+                                                        T omega_c = T(0)) {
   const int64_t pos = xcd_block() * kBlock + threadIdx.x;
   if (pos >= R.nb) return;
   const SliceHead h = bsr3_head(R, pos);
@@ -553,9 +584,17 @@ __global__ __launch_bounds__(kBlock) void k_restrict_b3(Bsr3<T> R, const T *__re
   T v[3];
   bsr3_dot_h(R, h, pos, res, v, false);
 #pragma unroll
-  for (int l = 0; l < 3; ++l) {
-    rhs_c[3 * b + l] = v[l];
-    if (corr_c) corr_c[3 * b + l] = T(0);
+  for (int l = 0; l < 3; ++l) rhs_c[3 * b + l] = v[l];
+  if (!corr_c) return;
+  if (dinv_c) {
+    T bi[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) bi[e] = dinv_c[9 * b + e];
+#pragma unroll
+    for (int l = 0; l < 3; ++l) corr_c[3 * b + l] = jacobi_zero3(bi + 3 * l, v[0], v[1], v[2], omega_c);
+  } else {
+#pragma unroll
+    for (int l = 0; l < 3; ++l) corr_c[3 * b + l] = T(0);
   }
 }
 

@@ -256,27 +256,32 @@ __global__ __launch_bounds__(kBlock) void k_residual_w(Rows<T> A, const T *__res
   if (active && lane == 0) res[row] = bi - s;
 }
 
-// rhs_c = R res, corr_c = 0 (as k_restrict).
+// rhs_c = R res, corr_c = 0 or, with dinv_c, the coarse level's first
+// damped Jacobi sweep in closed form (as k_restrict).
 template <class T, int G, int K>
 __global__ __launch_bounds__(kBlock) void k_restrict_w(Rows<T> R, const T *__restrict__ res,
                                                        T *__restrict__ rhs_c, T *__restrict__ corr_c,
-                                                       const int32_t *stop) {
+                                                       const int32_t *stop,
+                                                       const T *__restrict__ dinv_c = nullptr,
+                                                       T omega_c = T(0)) {
   __shared__ T lp[kBlock * K];
   const int lane = threadIdx.x % G;
   const int g0 = (threadIdx.x - lane) * K;
   const int64_t pos = grp_row<G>(0);
   const bool active = pos < R.rows;
   int64_t rb = 0, re = 0, row = pos;
+  T di = T(0);
   if (active) {
     rb = R.ptr[pos];
     re = R.ptr[pos + 1];
     if (R.row_map) row = R.row_map[pos];
   }
+  if (active && lane == 0 && dinv_c) di = dinv_c[row];
   if (stop_word(stop)) return;
   const T s = grp_row_dot_k<T, G, K>(R, rb, re, res, lp + g0, lane);
   if (active && lane == 0) {
     rhs_c[row] = s;
-    if (corr_c) corr_c[row] = T(0);
+    if (corr_c) corr_c[row] = dinv_c ? jacobi_zero1(s, di, omega_c) : T(0);
   }
 }
 
@@ -385,6 +390,16 @@ __global__ __launch_bounds__(kBlock) void k_residual_bw(BRows<T> A, const T *__r
   if (active && lane < 3) res[3 * b + lane] = bl - s;
 }
 
+// Loads row `lane` (< 3) of block b's inverse diagonal block.
+template <class T>
+__device__ __forceinline__ void load_dinv_row(const T *__restrict__ dinv, int64_t b, int lane,
+                                              T d[3]) {
+  const T *bi = dinv + 9 * b + 3 * lane;
+  d[0] = bi[0];
+  d[1] = bi[1];
+  d[2] = bi[2];
+}
+
 // 3x3-block transfer rows, one G-lane group per block row (as
 // k_restrict_b3 / k_prolong_b3), K blocks per lane and round.
 #define GSK_TRANSFER_BLOCK_LDS(G, K)        \
@@ -392,11 +407,15 @@ __global__ __launch_bounds__(kBlock) void k_residual_bw(BRows<T> A, const T *__r
   const int lane = threadIdx.x % G;         \
   const int g0 = (threadIdx.x - lane) * K
 
+// dinv_c as in k_restrict_b3: lane l < 3 forms line l of the closed-form
+// first block Jacobi sweep from the group's three line sums.
 template <class T, int G, int K>
 __global__ __launch_bounds__(kBlock) void k_restrict_bw(BRows<T> R, const T *__restrict__ res,
                                                         T *__restrict__ rhs_c,
                                                         T *__restrict__ corr_c,
-                                                        const int32_t *stop) {
+                                                        const int32_t *stop,
+                                                        const T *__restrict__ dinv_c = nullptr,
+                                                        T omega_c = T(0)) {
   GSK_TRANSFER_BLOCK_LDS(G, K);
   const int64_t pos = grp_row<G>(0);
   const bool active = pos < R.nb;
@@ -406,11 +425,20 @@ __global__ __launch_bounds__(kBlock) void k_restrict_bw(BRows<T> R, const T *__r
     re = R.ptr[pos + 1];
     if (R.row_map) b = R.row_map[pos];
   }
+  T d[3] = {T(0), T(0), T(0)};
+  if (active && lane < 3 && dinv_c) load_dinv_row(dinv_c, b, lane, d);
   if (stop_word(stop)) return;
   const T s = grp_brow_dot<T, G, K>(R, rb, re, res, -1, lp + 9 * g0, lane);
+  T xl = T(0);
+  if (dinv_c) {
+    // The group's three line sums, from its lanes 0..2, in every lane.
+    const int base = static_cast<int>(threadIdx.x & 63) & ~(G - 1);
+    const T s0 = __shfl(s, base, 64), s1 = __shfl(s, base + 1, 64), s2 = __shfl(s, base + 2, 64);
+    xl = jacobi_zero3(d, s0, s1, s2, omega_c);
+  }
   if (active && lane < 3) {
     rhs_c[3 * b + lane] = s;
-    if (corr_c) corr_c[3 * b + lane] = T(0);
+    if (corr_c) corr_c[3 * b + lane] = xl;
   }
 }
 
@@ -447,16 +475,6 @@ __device__ __forceinline__ T block_apply(const T d[3], bool active, T rl, T *lr,
     nv += d[2] * lr[2];
   }
   return nv;
-}
-
-// Loads row `lane` (< 3) of block b's inverse diagonal block.
-template <class T>
-__device__ __forceinline__ void load_dinv_row(const T *__restrict__ dinv, int64_t b, int lane,
-                                              T d[3]) {
-  const T *bi = dinv + 9 * b + 3 * lane;
-  d[0] = bi[0];
-  d[1] = bi[1];
-  d[2] = bi[2];
 }
 
 // One colour of 3x3 block Gauss-Seidel over block rows [begin, end)

@@ -37,7 +37,7 @@ import gpusolver as gs  # noqa: E402
 MODEL_PATH = HERE / "learned_weights.json"
 # Training / evaluation problem sizes: varcoef3d grid edge, elasticity
 # vertices per edge.
-DEFAULT_SIZE = {"sgs": 64, "bgs": 48, "hbgs": 48}
+DEFAULT_SIZE = {"sgs": 64, "bgs": 48, "hbgs": 48, "hsgs": 128}
 HIDDEN = 64
 
 # Search grids per smoother kind; the plain weight is the reference's
@@ -53,7 +53,12 @@ HIDDEN = 64
 # examples/elasticity/cantileverBeam.cc:589): block Gauss-Seidel on the
 # finest level and block Jacobi below it, each level's (w, sweeps) learned;
 # its `reference` kind is the all-block-Gauss-Seidel method the benchmark's
-# plain mode runs.
+# plain mode runs. "hsgs" is the same pairing for scalar levels, built the
+# way that preconditioner builds its MultigridSmoother (a fine and a coarse
+# smoother type, include/GPUSolver/Smoothers/MultigridSmoother.h:59-65):
+# symmetric Gauss-Seidel on the finest level, damped Jacobi below.
+# coarse_omegas: the damping grid of the levels below the finest when it
+# differs from the finest level's.
 KINDS = {
     "sgs": dict(smoother=gs.SGS, plain_omega=1.0, omegas=[0.7, 0.85, 1.0, 1.15, 1.3, 1.45, 1.6],
                 sweeps=[1, 2], coarse_sweeps=[0, 1, 2], block=1, coarse_steps=2),
@@ -63,6 +68,10 @@ KINDS = {
     "hbgs": dict(smoother=gs.BLOCK_GS, coarse_smoother=gs.BLOCK_JACOBI, plain_omega=0.7,
                  omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1, 1.2], sweeps=[1, 2],
                  coarse_sweeps=[0, 1, 2], block=3, coarse_steps=0, reference="bgs"),
+    "hsgs": dict(smoother=gs.SGS, coarse_smoother=gs.DAMPED_JACOBI, plain_omega=1.0,
+                 omegas=[0.7, 0.85, 1.0, 1.15, 1.3, 1.45, 1.6],
+                 coarse_omegas=[0.5, 0.6, 0.7, 0.8, 0.9, 1.0, 1.1], sweeps=[1, 2],
+                 coarse_sweeps=[0, 1, 2], block=1, coarse_steps=2, reference="sgs"),
 }
 
 
@@ -74,24 +83,23 @@ def solver_options(kind: str) -> dict:
 
 def config_kind(config: str, path: Path = MODEL_PATH):
     """The trained kind a benchmark configuration's learned mode uses: the
-    hybrid block smoother for the elasticity configurations when a hybrid
-    model is stored (else block Gauss-Seidel), SGS for the scalar ones;
-    None without a stored model."""
+    hybrid smoother (Jacobi below the finest level) when its model is
+    stored, else the all-Gauss-Seidel one -- block smoothers for the
+    elasticity configurations, scalar ones for the others; None without a
+    stored model."""
     if not Path(path).exists():
         return None
     d = json.loads(Path(path).read_text())
-    if "elasticity" in config:
-        for kind in ("hbgs", "bgs"):
-            if kind in d:
-                return kind
-        return None
-    return "sgs" if "sgs" in d else None
+    for kind in (("hbgs", "bgs") if "elasticity" in config else ("hsgs", "sgs")):
+        if kind in d:
+            return kind
+    return None
 
 
 def training_problem(kind: str, seed: int, size: int):
     """Seeded training / evaluation problems, at a reduced size with the
     correlation length scaled to keep the same physical field."""
-    if kind == "sgs":
+    if KINDS[kind]["block"] == 1:
         A = gs.varcoef3d(size, seed=seed, sigma=8.0 * size / 160.0, scale=2.0)
         b = gs.uniform_pm1(seed + 1_000_000, A.rows)
         H = gs.Hierarchy.smoothed_aggregation(A, block_size=1, eps_strong=0.1, max_levels=10,
@@ -192,7 +200,8 @@ def sweep_labels(H, b, spec, maxit=400, passes=3):
     for _ in range(passes):
         changed = False
         for k in range(used - 1):  # the last level in use is solved directly
-            for om in spec["omegas"]:
+            omegas = spec["omegas"] if k == 0 else spec.get("coarse_omegas", spec["omegas"])
+            for om in omegas:
                 for sw in (spec["sweeps"] if k == 0 else spec["coarse_sweeps"]):
                     if om == omega[k] and sw == sweeps[k]:
                         continue

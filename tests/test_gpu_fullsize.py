@@ -137,22 +137,26 @@ def test_varcoef160_plain_fp64(varcoef160):
 
 def test_varcoef160_mixed_learned(varcoef160):
     """The configuration's own method: FP32 V-cycle in FP64 CG, FP64 direct
-    solve at the first level within max_direct, the learned per-level SGS
-    weights; the oracle runs the same weights."""
+    solve at the first level within max_direct, the smoother family of the
+    stored model (SGS on level 0, damped Jacobi below for the hybrid) with
+    its learned per-level weights; the oracle runs the same method and
+    weights."""
     A, H, b = varcoef160
-    s = gs.Solver(gs.PREC_MIXED, smoother=gs.SGS, coarse_steps=2, coarse_direct=1,
-                  direct_truncate=1)
+    kind = learned.config_kind("varcoef3d_160")
+    assert kind is not None, "no trained scalar model"
+    opts = learned.solver_options(kind)
+    s = gs.Solver(gs.PREC_MIXED, coarse_steps=2, coarse_direct=1, direct_truncate=1, **opts)
     s.setup(H)
     used = len(s.describe()["levels"])
     lw = learned.weights_for("varcoef3d_160", H)
-    assert lw is not None, "no trained SGS model"
     omega, sweeps = np.asarray(lw[0])[:used], np.asarray(lw[1])[:used]
     s.set_level_weights(omega=omega, sweeps=sweeps)
-    say(f"varcoef3d_160 mixed+learned: levels used {used} omega {np.round(omega, 3).tolist()} "
-        f"sweeps {sweeps.tolist()}")
+    say(f"varcoef3d_160 mixed+learned ({kind}): levels used {used} omega "
+        f"{np.round(omega, 3).tolist()} sweeps {sweeps.tolist()}")
     x, info = s.solve(b, tol=1e-8, maxit=1000)
+    kinds = [opts["smoother"]] + [opts.get("coarse_smoother", opts["smoother"])] * (used - 1)
     xo, ito, _, _ = orc.pcg(hier_view(H, used), b, tol=1e-8, maxit=1000, mg_double=False,
-                            kinds=gs.SGS, omegas=omega, pres=sweeps, posts=sweeps, coarse_steps=2,
+                            kinds=kinds, omegas=omega, pres=sweeps, posts=sweeps, coarse_steps=2,
                             direct_coarse=True)
     check(x, info, xo, ito, A, b)
 
