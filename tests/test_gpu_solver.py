@@ -209,6 +209,29 @@ def test_vcycle_bitexact_hybrid(elast, steps, grp):
     np.testing.assert_array_equal(s.vcycle(r), zo)
 
 
+@pytest.mark.parametrize("sweeps1", [0, 2])
+def test_vcycle_bitexact_hybrid_level_weights(elast, sweeps1):
+    """The learned configurations' shape on the block hybrid: level 1 with
+    block Jacobi unsmoothed (0 sweeps: the restriction zeroes its x and no
+    closed-form sweep is written) or with two sweeps (the second read from
+    the other buffer), level 2 with one; per-level damping; bit for bit the
+    oracle's V-cycle."""
+    E, rhs, H = elast
+    s = gs.Solver(gs.PREC_MIXED, smoother=gs.BLOCK_GS, coarse_smoother=gs.BLOCK_JACOBI,
+                  omega=0.7, coarse_steps=1)
+    s.setup(H)
+    L = H.levels
+    sweeps = ([1, sweeps1] + [1] * L)[:L]
+    omega = ([0.82, 1.01, 0.9] + [0.7] * L)[:L]
+    s.set_level_weights(omega=omega, sweeps=sweeps)
+    hv, _ = hier_view(H)
+    r = gs.uniform_pm1(26, E.rows)
+    kinds = [gs.BLOCK_GS] + [gs.BLOCK_JACOBI] * (L - 1)
+    zo = orc.vcycle(hv, r, mg_double=False, kinds=kinds, omegas=omega, pres=sweeps,
+                    posts=sweeps, coarse_steps=1)
+    np.testing.assert_array_equal(s.vcycle(r), zo)
+
+
 @pytest.mark.parametrize("prec", [gs.PREC_DOUBLE, gs.PREC_MIXED])
 @pytest.mark.parametrize("steps", [1, 2])
 def test_vcycle_bitexact_scalar_jacobi_coarse(cfg1, prec, steps):
