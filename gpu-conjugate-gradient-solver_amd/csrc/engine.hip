@@ -820,15 +820,22 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
       const int c = step.first;
       int reps = step.second;
       const int64_t b = lv.coff[c], e = lv.coff[c + 1];
-      const DOp<TM> &M = (low_start && step_no >= 1 && step_no <= lv.ncolors - 1) ? lv.Alow : lv.A;
+      // The first sweep's forward colours 1 .. C-1 from x = 0 with the
+      // earlier-colour operator: their own x is still zero (xz) and later
+      // colours' x is never read before its own visit writes it, so only
+      // colour 0 is written by the first kernel.
+      const int xz = (low_start && step_no >= 1 && step_no <= lv.ncolors - 1) ? 1 : 0;
+      const DOp<TM> &M = xz ? lv.Alow : lv.A;
       if (!x_ready && c == 0) {
-        // First update from x = 0: closed form for colour 0, zero elsewhere.
+        // First update from x = 0: closed form for colour 0, zero elsewhere
+        // (nowhere else with the earlier-colour operator).
+        const int64_t zn = low_start ? e : units;
         if (block)
-          gsk::k_block_gs_color_zero<TM><<<grid_for(units), kBlock, 0, stream_>>>(
-              lv.dinv.template as<TM>(), e, units, lv.rhs, lv.x, lv.omega, stop);
+          gsk::k_block_gs_color_zero<TM><<<grid_for(zn), kBlock, 0, stream_>>>(
+              lv.dinv.template as<TM>(), e, zn, lv.rhs, lv.x, lv.omega, stop);
         else
-          gsk::k_gs_color_zero<TM><<<grid_for(units), kBlock, 0, stream_>>>(
-              lv.diag.template as<TM>(), e, units, lv.rhs, lv.x, lv.omega, stop);
+          gsk::k_gs_color_zero<TM><<<grid_for(zn), kBlock, 0, stream_>>>(
+              lv.diag.template as<TM>(), e, zn, lv.rhs, lv.x, lv.omega, stop);
         x_ready = true;
         if (reps == 1) continue;
         reps = 1;  // the second update of the pair reads the fresh values
@@ -840,20 +847,20 @@ void Engine<TC, TM>::encode_smooth(Level<TM> &lv, int iters, bool zero_x, const 
           if (block)
             gsk::k_block_gs_color_bw<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
                 M.bw.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
-                stop);
+                stop, xz);
           else
             gsk::k_gs_color_w<TM, G><<<gsk::grid_rows<G>(e - b), kBlock, 0, stream_>>>(
                 M.w.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps,
-                stop);
+                stop, xz);
         });
       } else if (block) {
         constexpr int CB = gsk::kColorBlock;
         const int64_t b0 = b & ~int64_t(gsk::kSlice - 1);  // slice-aligned start
         gsk::k_block_gs_color<TM><<<static_cast<uint32_t>((e - b0 + CB - 1) / CB), CB, 0, stream_>>>(
-            M.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+            M.b.view(), lv.dinv.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop, xz);
       } else {
         gsk::k_gs_color<TM><<<grid_for(e - b), kBlock, 0, stream_>>>(
-            M.s.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop);
+            M.s.view(), lv.diag.template as<TM>(), b, e, lv.rhs, lv.x, lv.omega, reps, stop, xz);
       }
     }
   } else if (lv.kind == kDampedJacobi || lv.kind == kBlockJacobi) {

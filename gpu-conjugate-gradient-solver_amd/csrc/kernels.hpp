@@ -634,16 +634,19 @@ __global__ __launch_bounds__(kBlock) void k_prolong_b3(Bsr3<T> P, const T *__res
 // both updates, with the reference's operations in the reference's order.
 // The diagonal comes from `diag` (the last entry of each row with column ==
 // row, the value the reference's row scan finds).
+// xz: the rows' own x is still the zero the sweep started from (the
+// forward colours of a sweep from x = 0 on a level whose later colours were
+// not zeroed); it is not read.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, const T *__restrict__ diag,
                                                      int64_t begin, int64_t end,
                                                      const T *__restrict__ rhs, T *x, T omega,
-                                                     int reps, const int32_t *stop) {
+                                                     int reps, const int32_t *stop, int xz = 0) {
   const int64_t row = begin + xcd_block() * kBlock + threadIdx.x;
   if (row >= end) return;
   const SliceHead h = sell_head(A, row);
   const T bi = rhs[row], di = diag[row];
-  T xi = x[row];  // not read by the off-diagonal sum
+  T xi = xz ? T(0) : x[row];  // not read by the off-diagonal sum
   if (stop_word(stop)) return;
   const T off = sell_offdiag_h(A, h, row, x);
   const T nv = (bi - off) / di;
@@ -654,8 +657,11 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(Sell<T> A, const T *__restr
 
 // First colour of a forward sweep started from x = 0: the off-diagonal sum
 // of a colour-0 row is exactly zero, so x_i = w * (b_i / a_ii) bit for bit
-// without reading the row; rows of later colours are zeroed in the same
-// pass (the reference's setZero before the V-cycle).
+// without reading the row; rows [end0, n) of later colours are zeroed in the
+// same pass (the reference's setZero before the V-cycle). A level whose
+// forward colours read only earlier colours and not their own x (xz)
+// passes n = end0: nothing reads a later colour's x before its own visit
+// writes it.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_gs_color_zero(const T *__restrict__ diag,
                                                           int64_t end0, int64_t n,
@@ -691,7 +697,7 @@ constexpr int kColorBlock = 64;
 template <class T>
 __global__ __launch_bounds__(kColorBlock) void k_block_gs_color(
     Bsr3<T> A, const T *__restrict__ dinv, int64_t begin, int64_t end, const T *__restrict__ rhs,
-    T *x, T omega, int reps, const int32_t *stop) {
+    T *x, T omega, int reps, const int32_t *stop, int xz = 0) {
   // Threads start at the SELL slice boundary below `begin`, so every
   // wavefront covers exactly one slice (one head, one width); rows before
   // `begin` belong to another colour and exit.
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(kColorBlock) void k_block_gs_color(
 #pragma unroll
   for (int l = 0; l < 3; ++l) {
     bl[l] = rhs[3 * b + l];
-    xl[l] = x[3 * b + l];  // not read by the off-block sum
+    xl[l] = xz ? T(0) : x[3 * b + l];  // not read by the off-block sum
   }
 #pragma unroll
   for (int e = 0; e < 9; ++e) bi[e] = dinv[9 * b + e];

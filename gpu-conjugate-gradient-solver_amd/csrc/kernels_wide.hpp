@@ -317,7 +317,8 @@ template <class T, int G>
 __global__ __launch_bounds__(kBlock) void k_gs_color_w(Rows<T> A, const T *__restrict__ diag,
                                                        int64_t begin, int64_t end,
                                                        const T *__restrict__ rhs, T *x, T omega,
-                                                       int reps, const int32_t *stop) {
+                                                       int reps, const int32_t *stop,
+                                                       int xz = 0) {
   GSK_GROUP_SCALAR_LDS(G);
   const int64_t row = grp_row<G>(begin);
   const bool active = row < end;
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_color_w(Rows<T> A, const T *__res
     rb = A.ptr[row];
     re = A.ptr[row + 1];
     bi = rhs[row];
-    xi = x[row];
+    xi = xz ? T(0) : x[row];
     di = diag[row];
   }
   if (stop_word(stop)) return;
@@ -484,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color_bw(BRows<T> A, const 
                                                               int64_t begin, int64_t end,
                                                               const T *__restrict__ rhs, T *x,
                                                               T omega, int reps,
-                                                              const int32_t *stop) {
+                                                              const int32_t *stop, int xz = 0) {
   GSK_GROUP_BLOCK_LDS(G);
   const int64_t b = grp_row<G>(begin);
   const bool active = b < end;
@@ -495,7 +496,7 @@ __global__ __launch_bounds__(kBlock) void k_block_gs_color_bw(BRows<T> A, const 
     re = A.ptr[b + 1];
     if (lane < 3) {
       bl = rhs[3 * b + lane];
-      xl = x[3 * b + lane];
+      xl = xz ? T(0) : x[3 * b + lane];
       load_dinv_row(dinv, b, lane, d);
     }
   }
