@@ -627,6 +627,8 @@ int gs_solver_set_real(gs_solver_h s, const char *key, double v) {
     if (k == "omega") s->opt.omega = v;
     else if (k == "group_min_scalar") s->opt.group_min_scalar = v;
     else if (k == "group_min_block") s->opt.group_min_block = v;
+    else if (k == "full_group_min_scalar") s->opt.full_group_min_scalar = v;
+    else if (k == "full_group_min_block") s->opt.full_group_min_block = v;
     else throw std::runtime_error(std::string("unknown real option: ") + key);
   });
 }

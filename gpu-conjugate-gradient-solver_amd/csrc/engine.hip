@@ -401,6 +401,9 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
     key.resize(lv.n / 3);
     for (int64_t j = 0; j < lv.n / 3; ++j) key[j] = lv.q[3 * j] / 3;
   }
+  // A Jacobi level's operator is read only by whole-level kernels (sweep,
+  // residual), which take its own lane-group threshold.
+  const bool jacobi_level = lv.kind == kDampedJacobi || lv.kind == kBlockJacobi;
   std::unique_ptr<gsdev::DBsr3Csr> DB;
   if (dev_pack) {
     DB = std::make_unique<gsdev::DBsr3Csr>();
@@ -409,10 +412,12 @@ void Engine<TC, TM>::build_level(int k, const gsx::Csr &Ak, bool coarsest) {
       const gsdev::DCsr dAi = gsdev::relabel(dAk, lv.q, qinv);
       gsdev::bsr3_pack(dAi, key.data(), *DB);
     }
-    build_b3_device(lv.A, *DB, opt_.group_min_block, nullptr);
+    build_b3_device(lv.A, *DB, jacobi_level ? opt_.full_group_min_block : opt_.group_min_block,
+                    nullptr);
   } else {
-    lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(), opt_.group_min_scalar,
-               opt_.group_min_block);
+    lv.A.build(Ai, blocky, key.empty() ? nullptr : key.data(),
+               jacobi_level ? opt_.full_group_min_scalar : opt_.group_min_scalar,
+               jacobi_level ? opt_.full_group_min_block : opt_.group_min_block);
   }
   lv.has_low = false;
   if (dev_pack && lv.kind == kBlockGS && k < opt_.zero_start_lower && lv.ncolors > 1) {
@@ -616,15 +621,15 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
           key_f[j] = L_[k].q[3 * j] / 3;
           ord_f[j] = qf_inv[3 * j] / 3;
         }
-        L_[k].P.build_blocks(Pi, key_c.data(), opt_.group_min_block, opt_.transfer_sigma,
+        L_[k].P.build_blocks(Pi, key_c.data(), opt_.full_group_min_block, opt_.transfer_sigma,
                              lo ? ord_f.data() : nullptr);
-        L_[k].R.build_blocks(Ri, key_f.data(), opt_.group_min_block, opt_.transfer_sigma,
+        L_[k].R.build_blocks(Ri, key_f.data(), opt_.full_group_min_block, opt_.transfer_sigma,
                              lo ? ord_c.data() : nullptr);
       } else {
-        L_[k].P.build(Pi, opt_.group_min_scalar, opt_.group_max_transfer, opt_.transfer_sigma,
-                      lo ? qf_inv.data() : nullptr);
-        L_[k].R.build(Ri, opt_.group_min_scalar, opt_.group_max_transfer, opt_.transfer_sigma,
-                      lo ? qc_inv.data() : nullptr);
+        L_[k].P.build(Pi, opt_.full_group_min_scalar, opt_.group_max_transfer,
+                      opt_.transfer_sigma, lo ? qf_inv.data() : nullptr);
+        L_[k].R.build(Ri, opt_.full_group_min_scalar, opt_.group_max_transfer,
+                      opt_.transfer_sigma, lo ? qc_inv.data() : nullptr);
       }
       if (L_[k].tail_ok && L_[k + 1].tail_ok) {
         L_[k].tR.build(Ri);

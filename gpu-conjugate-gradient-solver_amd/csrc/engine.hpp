@@ -40,6 +40,14 @@ struct Options {
   // arithmetic, same order); negative: never.
   double group_min_scalar = 24.0;
   double group_min_block = 20.0;
+  // The same thresholds for operators read only by whole-level kernels:
+  // transfers and the operators of Jacobi-smoothed levels. A Gauss-Seidel
+  // level's colour launches cover a fraction of its rows each, so lane
+  // groups pay off there at shorter rows; a whole-level launch of 25-40
+  // entry scalar rows runs faster one thread per row (measured: the
+  // hybrid 3D Poisson solve 5.7 -> 4.6 ms, 32..64 all equal).
+  double full_group_min_scalar = 40.0;
+  double full_group_min_block = 20.0;
   // Largest lane group for the transfer operators' rows (16, 32 or 64);
   // 32 measured best for the ~190-entry rows of R = P^T (two rows per
   // wavefront, two load rounds per row).

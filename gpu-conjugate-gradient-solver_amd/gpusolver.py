@@ -633,7 +633,8 @@ class SolveInfo:
     wall_ms: float
 
 
-_REAL_OPTIONS = ("omega", "group_min_scalar", "group_min_block")
+_REAL_OPTIONS = ("omega", "group_min_scalar", "group_min_block", "full_group_min_scalar",
+                 "full_group_min_block")
 
 
 class Solver:

@@ -289,6 +289,9 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  * value (scalar entries per row, 3x3 blocks per block row) are processed
  * by lane groups, one row per group of 16/32/64 lanes, with the same
  * arithmetic in the same order; negative values keep one thread per row.
+ * "full_group_min_scalar" (40) and "full_group_min_block" (20): the same
+ * thresholds for the operators read only by whole-level kernels (the
+ * transfers P and R, and the operators of Jacobi-smoothed levels).
  * Takes effect at the next gs_solver_setup. */
 int gs_solver_set_int(gs_solver_h s, const char *key, int64_t value);
 int gs_solver_set_real(gs_solver_h s, const char *key, double value);

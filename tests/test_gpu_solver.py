@@ -117,7 +117,8 @@ GROUPS = [-1.0, 0.0, None]
 
 
 def group_opts(grp):
-    return {} if grp is None else dict(group_min_scalar=grp, group_min_block=grp)
+    return {} if grp is None else dict(group_min_scalar=grp, group_min_block=grp,
+                                       full_group_min_scalar=grp, full_group_min_block=grp)
 
 
 @pytest.mark.parametrize("grp", GROUPS)
@@ -164,8 +165,7 @@ def test_vcycle_signed_zeros_match_oracle(cfg1, elast, grp):
     r = gs.uniform_pm1(12, A.rows)
     r[::3] = 0.0
     r[1::7] = -0.0
-    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_steps=2, group_min_scalar=grp,
-                  group_min_block=grp)
+    s = gs.Solver(gs.PREC_DOUBLE, smoother=gs.SGS, coarse_steps=2, **group_opts(grp))
     s.setup(H)
     hv, _ = hier_view(H)
     z = s.vcycle(r)
@@ -178,7 +178,7 @@ def test_vcycle_signed_zeros_match_oracle(cfg1, elast, grp):
     rb = gs.uniform_pm1(13, E.rows)
     rb[::4] = 0.0
     sb = gs.Solver(gs.PREC_DOUBLE, smoother=gs.BLOCK_GS, omega=0.7, coarse_steps=2,
-                   group_min_scalar=grp, group_min_block=grp)
+                   **group_opts(grp))
     sb.setup(HE)
     hve, _ = hier_view(HE)
     zb = sb.vcycle(rb)
