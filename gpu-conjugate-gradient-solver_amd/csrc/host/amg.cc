@@ -45,6 +45,7 @@ struct Aggregates {
 
 std::vector<double> diagonal_of(const Csr &A) {
   std::vector<double> d(A.rows, 0.0);
+#pragma omp parallel for schedule(static)
   for (int64_t r = 0; r < A.rows; ++r)
     for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k)
       if (A.col[k] == r) d[r] += A.val[k];
@@ -67,6 +68,7 @@ Aggregates plain_aggregates(const Csr &A, double eps) {
     }
   }
   ag.id.assign(n, kRemoved);
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i)
     for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
       if (ag.strong[k]) { ag.id[i] = kUndefined; break; }
@@ -142,43 +144,62 @@ void merge_small_aggregates(const Csr &A, Aggregates &ag, int64_t min_points) {
   ag.count = keep.empty() ? 0 : keep.back();
 }
 
-// Block-max "pointwise" matrix of a block_size-blocked scalar matrix.
+// Block-max "pointwise" matrix of a block_size-blocked scalar matrix: row
+// ip holds the distinct column blocks of rows ip*bs .. ip*bs+bs-1 in
+// ascending order, each with the largest |a| of its entries. Two parallel
+// passes (distinct count, then fill and sort) straight into the CSR arrays.
 Csr pointwise_matrix(const Csr &A, int32_t bs) {
   const int64_t np = A.rows / bs;
   Csr P;
   P.rows = P.cols = np;
   P.ptr.assign(np + 1, 0);
-  std::vector<std::vector<std::pair<int32_t, double>>> rows(np);
 #pragma omp parallel
   {
     std::vector<int32_t> mark(np, -1);
 #pragma omp for schedule(static)
     for (int64_t ip = 0; ip < np; ++ip) {
-      auto &row = rows[ip];
-      for (int32_t l = 0; l < bs; ++l) {
-        const int64_t r = ip * bs + l;
+      int32_t cnt = 0;
+      for (int64_t r = ip * bs; r < (ip + 1) * bs; ++r)
+        for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
+          const int32_t cp = A.col[k] / bs;
+          if (mark[cp] != static_cast<int32_t>(ip)) {
+            mark[cp] = static_cast<int32_t>(ip);
+            ++cnt;
+          }
+        }
+      P.ptr[ip + 1] = cnt;
+    }
+  }
+  for (int64_t ip = 0; ip < np; ++ip) P.ptr[ip + 1] += P.ptr[ip];
+  P.col.resize(P.ptr[np]);
+  P.val.resize(P.ptr[np]);
+#pragma omp parallel
+  {
+    std::vector<int32_t> slot(np, -1);
+    std::vector<std::pair<int32_t, double>> row;
+#pragma omp for schedule(static)
+    for (int64_t ip = 0; ip < np; ++ip) {
+      row.clear();
+      for (int64_t r = ip * bs; r < (ip + 1) * bs; ++r)
         for (int32_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) {
           const int32_t cp = A.col[k] / bs;
           const double v = std::abs(A.val[k]);
-          if (mark[cp] < 0) {
-            mark[cp] = static_cast<int32_t>(row.size());
+          if (slot[cp] < 0) {
+            slot[cp] = static_cast<int32_t>(row.size());
             row.emplace_back(cp, v);
           } else {
-            row[mark[cp]].second = std::max(row[mark[cp]].second, v);
+            row[slot[cp]].second = std::max(row[slot[cp]].second, v);
           }
         }
-      }
-      for (auto &e : row) mark[e.first] = -1;
+      for (auto &e : row) slot[e.first] = -1;
       std::sort(row.begin(), row.end());
+      int64_t d = P.ptr[ip];
+      for (auto &e : row) {
+        P.col[d] = e.first;
+        P.val[d] = e.second;
+        ++d;
+      }
     }
-  }
-  for (int64_t ip = 0; ip < np; ++ip)
-    P.ptr[ip + 1] = P.ptr[ip] + static_cast<int32_t>(rows[ip].size());
-  P.col.resize(P.ptr[np]);
-  P.val.resize(P.ptr[np]);
-  for (int64_t ip = 0; ip < np; ++ip) {
-    int64_t d = P.ptr[ip];
-    for (auto &e : rows[ip]) { P.col[d] = e.first; P.val[d] = e.second; ++d; }
   }
   return P;
 }
@@ -226,6 +247,24 @@ void householder_qr(int m, int k, std::vector<double> &b, std::vector<double> &q
   }
 }
 
+// A copy of A whose entry arrays are copied by all threads (a large level-0
+// matrix is over a gigabyte).
+Csr parallel_copy(const Csr &A) {
+  Csr C;
+  C.rows = A.rows;
+  C.cols = A.cols;
+  C.ptr = A.ptr;
+  const int64_t nnz = A.nnz();
+  C.col.resize(nnz);
+  C.val.resize(nnz);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < nnz; ++k) {
+    C.col[k] = A.col[k];
+    C.val[k] = A.val[k];
+  }
+  return C;
+}
+
 struct CoarsenResult {
   Csr P, R, Ac;
   std::vector<int32_t> agg;      // per fine DOF
@@ -243,11 +282,13 @@ Csr smoothing_operator(const Csr &A, const std::vector<char> &strong, double ome
   M.rows = n;
   M.cols = A.cols;
   M.ptr.assign(n + 1, 0);
+#pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < n; ++i) {
     int32_t cnt = 0;
     for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) cnt += (A.col[k] == i || strong[k]) ? 1 : 0;
-    M.ptr[i + 1] = M.ptr[i] + cnt;
+    M.ptr[i + 1] = cnt;
   }
+  for (int64_t i = 0; i < n; ++i) M.ptr[i + 1] += M.ptr[i];
   M.col.resize(M.ptr[n]);
   M.val.resize(M.ptr[n]);
 #pragma omp parallel for schedule(dynamic, 1024)
@@ -293,6 +334,7 @@ CoarsenResult coarsen(const Csr &A, const SAParams &prm,
     point_count = ag.count;
     // Expand the pointwise strength to scalar entries: couplings inside the
     // point's own block are strong, others inherit the pointwise flag.
+#pragma omp parallel for schedule(static)
     for (int64_t ip = 0; ip < Ap.rows; ++ip) {
       for (int32_t l = 0; l < bs; ++l) {
         const int64_t r = ip * bs + l;
@@ -405,7 +447,7 @@ Hierarchy build_sa_hierarchy(const Csr &A, const SAParams &prm,
   if (ncols > 0 && static_cast<int64_t>(nullspace.size()) != A.rows * ncols)
     throw std::runtime_error("AMGHierarchyBuilder: near-nullspace row count mismatch.");
   Hierarchy H;
-  H.A.push_back(A);
+  H.A.push_back(parallel_copy(A));
   std::vector<double> B = nullspace;
   SAParams lp = prm;
   for (int32_t level = 0; level < prm.max_levels; ++level) {

@@ -134,6 +134,10 @@ Bsr3Csr bsr3_take_rows(const Bsr3Csr &B, const int32_t *rows, int64_t nrows);
 // Stable order of block rows by decreasing block count within windows of
 // `sigma` rows (SELL-C-sigma): position p holds block row result[p].
 std::vector<int32_t> bsr3_sorted_order(const Bsr3Csr &B, int64_t sigma);
+// Writes one byte of every 4 KiB page of [p, p + bytes) from all threads:
+// arrays allocated without initialisation map their pages in parallel
+// before a copy into them (which would take the page faults one by one).
+void touch_pages(void *p, size_t bytes);
 // dst[i] = (float)src[i], in parallel.
 void to_float(const double *src, size_t n, float *dst);
 // Parallel permuted copies with a cast: gather dst[i] = src[idx[i]],

@@ -472,6 +472,13 @@ std::vector<int32_t> bsr3_sorted_order(const Bsr3Csr &B, int64_t sigma) {
   return order;
 }
 
+void touch_pages(void *p, size_t bytes) {
+  char *c = static_cast<char *>(p);
+  const int64_t pages = static_cast<int64_t>((bytes + 4095) / 4096);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < pages; ++i) c[i * 4096] = 0;
+}
+
 void to_float(const double *src, size_t n, float *dst) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) dst[i] = static_cast<float>(src[i]);

@@ -5,13 +5,15 @@
 // 93-97: transfer_operators, then coarse_operator); the host restatement is
 // gsx::GalerkinOps. The results here equal the host's bit for bit:
 //
-//  * SpGEMM C = A B, one thread per output row, replays the host's
-//    Gustavson loop exactly: A's row entries in order, each one's B row in
-//    order; a column's first product starts its sum and later products are
-//    added in that order (a per-row open-addressing table maps columns to
-//    their first-touch slot); the row is then sorted by column. A symbolic
-//    pass counts each row's distinct columns first. The tables live in one
-//    scratch buffer, rows processed in chunks that fit it.
+//  * SpGEMM C = A B gives every output column the host's Gustavson sum:
+//    A's row entries in order, each one's B row in order; a column's first
+//    product starts its sum and later products are added in that order. A
+//    symbolic pass counts each row's distinct columns first. Rows whose A
+//    row and column count fit (at most 1024 each) run one wavefront per row
+//    with the column table, the sorted columns and the sums in LDS; the
+//    others run one thread per row with open-addressing tables in a
+//    scratch buffer (rows processed in chunks that fit it) and a heap sort
+//    of the finished row.
 //  * Transpose: entries scattered into their column's segment by atomic
 //    slot, then every segment sorted by source row, which restores the
 //    host's stable (ascending row) order; the pairs are unique, so the order
@@ -66,6 +68,8 @@ gsx::Csr download(const DCsr &d) {
   A.ptr.resize(d.rows + 1);
   A.col.resize(d.nnz);
   A.val.resize(d.nnz);
+  gsx::touch_pages(A.col.data(), A.col.size() * sizeof(int32_t));
+  gsx::touch_pages(A.val.data(), A.val.size() * sizeof(double));
   GS_HIP(hipMemcpy(A.ptr.data(), d.ptr.get(), (d.rows + 1) * sizeof(int32_t),
                    hipMemcpyDeviceToHost));
   if (d.nnz) {
@@ -127,8 +131,9 @@ __global__ void k_symbolic(const int32_t *ap, const int32_t *ac, const int32_t *
   const int64_t r = r0 + static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (r >= r1) return;
   const uint32_t H = static_cast<uint32_t>(tsize[r]);
+  if (!H) return;  // a row without products, or one the LDS path counts
   int32_t cnt = 0;
-  if (H) {
+  {
     int32_t *t = table + toff[r];
     for (uint32_t i = 0; i < H; ++i) t[i] = kEmpty;
     const uint32_t mask = H - 1;
@@ -183,6 +188,211 @@ __global__ void k_numeric(const int32_t *ap, const int32_t *ac, const double *av
     }
   }
   heap_sort(cc + base, cv + base, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// One wavefront per output row, tables in LDS: the path for rows whose
+// A row (NCAP entries) and distinct columns (DCAP) fit. The row's products
+// are numbered t = 0 .. np-1 in the host's loop order (A entry k, then its
+// B row entry); pref[k] is the number of products before A entry k and
+// bst[k] the start of its B row, so lane t finds (k, kb) by a binary
+// search. Within one A entry the columns are distinct (a B row holds each
+// column once), so the products of one k can be added in parallel; the
+// A entries are applied one after the other, which is the host's order for
+// every column. A sum starts at -0.0, the additive identity: -0 + v == v
+// bit for bit for every v (signed zeros, infinities and NaN included), the
+// same as the host's "first product starts the sum".
+// ---------------------------------------------------------------------------
+constexpr int kRowWave = 64;
+
+// Fills pref[0 .. na] and bst[0 .. na) for the A row (columns ac_row,
+// na <= NCAP); returns the product count.
+__device__ int32_t row_products(const int32_t *ac_row, int32_t na, const int32_t *bp,
+                                int32_t *pref, int32_t *bst) {
+  const int lane = threadIdx.x;
+  int32_t carry = 0;
+  if (lane == 0) pref[0] = 0;
+  for (int32_t i0 = 0; i0 < na; i0 += kRowWave) {
+    const int32_t i = i0 + lane;
+    int32_t len = 0, st = 0;
+    if (i < na) {
+      const int32_t j = ac_row[i];
+      st = bp[j];
+      len = bp[j + 1] - st;
+    }
+    int32_t s = len;
+#pragma unroll
+    for (int d = 1; d < kRowWave; d <<= 1) {
+      const int32_t u = __shfl_up(s, d, kRowWave);
+      if (lane >= d) s += u;
+    }
+    if (i < na) {
+      pref[i + 1] = carry + s;
+      bst[i] = st;
+    }
+    carry += __shfl(s, kRowWave - 1, kRowWave);
+  }
+  __syncthreads();
+  return carry;
+}
+
+// The A entry k of product t: pref[k] <= t < pref[k + 1] (pref[0] = 0,
+// pref[na] = np > t; empty B rows are passed over).
+__device__ __forceinline__ int32_t product_entry(const int32_t *pref, int32_t na, int32_t t) {
+  int32_t lo = 0, hi = na;
+  while (hi - lo > 1) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (pref[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int32_t wave_sum(int32_t v) {
+#pragma unroll
+  for (int d = kRowWave / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kRowWave);
+  return v;
+}
+
+// Distinct columns of the listed rows of A B. Stops a row as soon as more
+// than DCAP columns were found and reports -1 (the row goes to k_symbolic);
+// TS > DCAP + 64 keeps a free slot for every probe of a round.
+template <int TS, int DCAP, int NCAP>
+__global__ __launch_bounds__(kRowWave) void k_symbolic_lds(const int32_t *ap, const int32_t *ac,
+                                                           const int32_t *bp, const int32_t *bc,
+                                                           const int32_t *rows, int32_t *rowlen) {
+  static_assert(TS > DCAP + kRowWave && (TS & (TS - 1)) == 0, "table size");
+  __shared__ int32_t key[TS];
+  __shared__ int32_t pref[NCAP + 1];
+  __shared__ int32_t bst[NCAP];
+  const int lane = threadIdx.x;
+  const int32_t r = rows[blockIdx.x];
+  const int32_t a0 = ap[r], na = ap[r + 1] - a0;
+  for (int i = lane; i < TS; i += kRowWave) key[i] = kEmpty;
+  const int32_t np = row_products(ac + a0, na, bp, pref, bst);
+  int32_t cnt = 0, total = 0;
+  bool over = false;
+  for (int32_t t0 = 0; t0 < np; t0 += kRowWave) {
+    const int32_t t = t0 + lane;
+    if (t < np) {
+      const int32_t k = product_entry(pref, na, t);
+      const int32_t c = bc[bst[k] + (t - pref[k])];
+      uint32_t h = hash_col(c, TS - 1);
+      for (;;) {
+        const int32_t old = atomicCAS(&key[h], kEmpty, c);
+        if (old == kEmpty) {
+          ++cnt;
+          break;
+        }
+        if (old == c) break;
+        h = (h + 1) & (TS - 1);
+      }
+    }
+    total = wave_sum(cnt);
+    if (total > DCAP) {
+      over = true;
+      break;
+    }
+  }
+  if (lane == 0) rowlen[r] = over ? -1 : total;
+}
+
+// Values of the listed rows of C = A B (rowlen[r] in 1 .. DCAP distinct
+// columns, na <= NCAP): the columns are collected in a table, sorted
+// (bitonic, in LDS), each given its sorted position, and the products
+// added A entry by A entry.
+template <int TS, int DCAP, int NCAP>
+__global__ __launch_bounds__(kRowWave) void k_numeric_lds(
+    const int32_t *ap, const int32_t *ac, const double *av, const int32_t *bp, const int32_t *bc,
+    const double *bv, const int32_t *rows, const int32_t *rowlen, const int32_t *cp, int32_t *cc,
+    double *cv) {
+  static_assert(TS >= 2 * DCAP && (TS & (TS - 1)) == 0 && (DCAP & (DCAP - 1)) == 0, "sizes");
+  __shared__ int32_t key[TS];
+  __shared__ uint16_t pos[TS];
+  __shared__ int32_t col[DCAP];
+  __shared__ double acc[DCAP];
+  __shared__ int32_t pref[NCAP + 1];
+  __shared__ int32_t bst[NCAP];
+  __shared__ int32_t fill;
+  const int lane = threadIdx.x;
+  const int32_t r = rows[blockIdx.x];
+  const int32_t m = rowlen[r];
+  const int32_t a0 = ap[r], na = ap[r + 1] - a0;
+  for (int i = lane; i < TS; i += kRowWave) key[i] = kEmpty;
+  if (lane == 0) fill = 0;
+  const int32_t np = row_products(ac + a0, na, bp, pref, bst);
+  // The row's distinct columns.
+  for (int32_t t0 = 0; t0 < np; t0 += kRowWave) {
+    const int32_t t = t0 + lane;
+    if (t < np) {
+      const int32_t k = product_entry(pref, na, t);
+      const int32_t c = bc[bst[k] + (t - pref[k])];
+      uint32_t h = hash_col(c, TS - 1);
+      for (;;) {
+        const int32_t old = atomicCAS(&key[h], kEmpty, c);
+        if (old == kEmpty || old == c) break;
+        h = (h + 1) & (TS - 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < TS; i += kRowWave) {
+    const int32_t c = key[i];
+    if (c != kEmpty) col[atomicAdd(&fill, 1)] = c;
+  }
+  int32_t ns = 1;
+  while (ns < m) ns <<= 1;
+  __syncthreads();
+  for (int32_t i = m + lane; i < ns; i += kRowWave) col[i] = INT_MAX;
+  __syncthreads();
+  for (int32_t size = 2; size <= ns; size <<= 1) {
+    for (int32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int32_t i = lane; i < ns / 2; i += kRowWave) {
+        const int32_t lo = 2 * i - (i & (stride - 1));
+        const int32_t hi = lo + stride;
+        const int32_t x = col[lo], y = col[hi];
+        if ((x > y) == ((lo & size) == 0)) {
+          col[lo] = y;
+          col[hi] = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int32_t p = lane; p < m; p += kRowWave) {
+    const int32_t c = col[p];
+    uint32_t h = hash_col(c, TS - 1);
+    while (key[h] != c) h = (h + 1) & (TS - 1);
+    pos[h] = static_cast<uint16_t>(p);
+    acc[p] = -0.0;
+  }
+  __syncthreads();
+  for (int32_t t0 = 0; t0 < np; t0 += kRowWave) {
+    const int32_t t = t0 + lane;
+    const bool valid = t < np;
+    int32_t k = -1, p = 0;
+    double v = 0.0;
+    if (valid) {
+      k = product_entry(pref, na, t);
+      const int32_t kb = bst[k] + (t - pref[k]);
+      const int32_t c = bc[kb];
+      v = av[a0 + k] * bv[kb];
+      uint32_t h = hash_col(c, TS - 1);
+      while (key[h] != c) h = (h + 1) & (TS - 1);
+      p = pos[h];
+    }
+    const int32_t last = min(kRowWave - 1, np - 1 - t0);
+    const int32_t klo = __shfl(k, 0, kRowWave), khi = __shfl(k, last, kRowWave);
+    for (int32_t s = klo; s <= khi; ++s) {
+      if (valid && k == s) acc[p] += v;
+      __syncthreads();
+    }
+  }
+  const int64_t base = cp[r];
+  for (int32_t p = lane; p < m; p += kRowWave) {
+    cc[base + p] = col[p];
+    cv[base + p] = acc[p];
+  }
 }
 
 // Column counts of A (for its transpose).
@@ -247,7 +457,11 @@ std::vector<std::pair<int64_t, int64_t>> chunks(const std::vector<int32_t> &tsiz
   return out;
 }
 
-DCsr spgemm(const DCsr &A, const DCsr &B, DevBuf &scratch) {
+// scratch: the global-table kernels' tables, allocated (kScratchBytes) at
+// their first use.
+// b_unique: no row of B repeats a column (the LDS kernels add one A entry's
+// products in parallel; otherwise every row takes the sequential kernels).
+DCsr spgemm(const DCsr &A, const DCsr &B, DevBuf &scratch, bool b_unique) {
   if (A.cols != B.rows) throw std::runtime_error("spgemm: shape mismatch.");
   const int64_t n = A.rows;
   DCsr C;
@@ -261,42 +475,119 @@ DCsr spgemm(const DCsr &A, const DCsr &B, DevBuf &scratch) {
   k_products<<<blocks_for(n), kThreads>>>(A.p(), A.c(), B.p(), n, np_d.as<int64_t>());
   std::vector<int64_t> np(n);
   GS_HIP(hipMemcpy(np.data(), np_d.get(), n * sizeof(int64_t), hipMemcpyDeviceToHost));
-  // Symbolic pass: tables sized for the distinct-column bound.
-  std::vector<int32_t> tsize(n);
-  for (int64_t r = 0; r < n; ++r) {
-    const int64_t bound = std::min<int64_t>(np[r], B.cols);
-    tsize[r] = bound ? static_cast<int32_t>(pow2_at_least(2 * bound)) : 0;
-  }
-  std::vector<int64_t> toff;
-  DevBuf toff_d, tsize_d, rowlen_d(n * sizeof(int32_t));
-  tsize_d.upload(tsize);
-  for (const auto &ch : chunks(tsize, toff, sizeof(int32_t))) {
-    toff_d.upload(toff);
-    const int64_t m = ch.second - ch.first;
-    k_symbolic<<<blocks_for(m), kThreads>>>(A.p(), A.c(), B.p(), B.c(), ch.first, ch.second,
-                                           toff_d.as<int64_t>(), tsize_d.as<int32_t>(),
-                                           scratch.as<int32_t>(), rowlen_d.as<int32_t>());
+  std::vector<int32_t> aptr(n + 1);
+  GS_HIP(hipMemcpy(aptr.data(), A.p(), (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+  auto na = [&](int64_t r) { return static_cast<int64_t>(aptr[r + 1]) - aptr[r]; };
+  // Row lists of the LDS kernels (one wavefront per row); a row that fits
+  // none of them uses the global-table kernels.
+  // (Each list keeps its own buffer until the function returns: a launch
+  // may still read it when the next list is uploaded.)
+  std::vector<DevBuf> lists;
+  auto launch_rows = [&lists](const std::vector<int32_t> &list, auto &&go) {
+    if (list.empty()) return;
+    lists.emplace_back();
+    lists.back().upload(list);
+    go(static_cast<uint32_t>(list.size()), lists.back().as<int32_t>());
     GS_HIP(hipGetLastError());
+  };
+  DevBuf rowlen_d(n * sizeof(int32_t));
+  GS_HIP(hipMemset(rowlen_d.get(), 0, n * sizeof(int32_t)));
+  // (The LDS kernels count a row's products in int32: below 2^30.)
+  constexpr int64_t kNpMax = int64_t(1) << 30;
+  std::vector<int32_t> sym_s, sym_l;
+  for (int64_t r = 0; r < n && b_unique; ++r) {
+    if (np[r] == 0 || np[r] >= kNpMax) continue;
+    const int64_t bound = std::min<int64_t>(np[r], B.cols);
+    if (bound <= 128 && na(r) <= 256) sym_s.push_back(static_cast<int32_t>(r));
+    else if (na(r) <= 1024) sym_l.push_back(static_cast<int32_t>(r));
   }
+  launch_rows(sym_s, [&](uint32_t g, const int32_t *rows) {
+    k_symbolic_lds<256, 128, 256><<<g, kRowWave>>>(A.p(), A.c(), B.p(), B.c(), rows,
+                                                   rowlen_d.as<int32_t>());
+  });
+  launch_rows(sym_l, [&](uint32_t g, const int32_t *rows) {
+    k_symbolic_lds<2048, 1024, 1024><<<g, kRowWave>>>(A.p(), A.c(), B.p(), B.c(), rows,
+                                                      rowlen_d.as<int32_t>());
+  });
   std::vector<int32_t> rowlen(n);
   GS_HIP(hipMemcpy(rowlen.data(), rowlen_d.get(), n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  // Symbolic pass of the remaining rows (more than 1024 columns or A
+  // entries): tables in global memory sized for the distinct-column bound.
+  std::vector<int32_t> tsize(n, 0);
+  bool any_global = false;
+  {
+    std::vector<char> lds(n, 0);
+    for (int32_t r : sym_s) lds[r] = 1;
+    for (int32_t r : sym_l) lds[r] = rowlen[r] >= 0;
+    for (int64_t r = 0; r < n; ++r) {
+      if (np[r] == 0 || lds[r]) continue;
+      const int64_t bound = std::min<int64_t>(np[r], B.cols);
+      tsize[r] = static_cast<int32_t>(pow2_at_least(2 * bound));
+      any_global = true;
+    }
+  }
+  std::vector<int64_t> toff;
+  DevBuf toff_d, tsize_d;
+  if (any_global) {
+    if (!scratch.get()) scratch.alloc(kScratchBytes);
+    tsize_d.upload(tsize);
+    for (const auto &ch : chunks(tsize, toff, sizeof(int32_t))) {
+      toff_d.upload(toff);
+      const int64_t m = ch.second - ch.first;
+      k_symbolic<<<blocks_for(m), kThreads>>>(A.p(), A.c(), B.p(), B.c(), ch.first, ch.second,
+                                             toff_d.as<int64_t>(), tsize_d.as<int32_t>(),
+                                             scratch.as<int32_t>(), rowlen_d.as<int32_t>());
+      GS_HIP(hipGetLastError());
+    }
+    GS_HIP(hipMemcpy(rowlen.data(), rowlen_d.get(), n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  }
   const std::vector<int32_t> cptr = row_ptr(rowlen, "spgemm: nnz exceeds int32.");
   C.nnz = cptr[n];
   C.ptr.upload(cptr);
   C.col.alloc(C.nnz * sizeof(int32_t));
   C.val.alloc(C.nnz * sizeof(double));
-  // Numeric pass: tables sized for the exact distinct count.
-  for (int64_t r = 0; r < n; ++r)
-    tsize[r] = rowlen[r] ? static_cast<int32_t>(pow2_at_least(2 * int64_t(rowlen[r]))) : 0;
-  tsize_d.upload(tsize);
-  for (const auto &ch : chunks(tsize, toff, 2 * sizeof(int32_t))) {
-    toff_d.upload(toff);
-    const int64_t m = ch.second - ch.first;
-    k_numeric<<<blocks_for(m), kThreads>>>(A.p(), A.c(), A.v(), B.p(), B.c(), B.v(), ch.first,
-                                          ch.second, toff_d.as<int64_t>(), tsize_d.as<int32_t>(),
-                                          scratch.as<int32_t>(), C.ptr.as<int32_t>(),
-                                          C.col.as<int32_t>(), C.val.as<double>());
-    GS_HIP(hipGetLastError());
+  // Numeric pass: LDS kernels by distinct-column count and A row length;
+  // the rest with global tables sized for the exact distinct count.
+  std::vector<int32_t> num_s, num_m, num_l;
+  any_global = false;
+  for (int64_t r = 0; r < n; ++r) {
+    tsize[r] = 0;
+    const int64_t m = rowlen[r];
+    if (m == 0) continue;
+    if (!b_unique || np[r] >= kNpMax) {
+      tsize[r] = static_cast<int32_t>(pow2_at_least(2 * m));
+      any_global = true;
+    } else if (m <= 128 && na(r) <= 256) num_s.push_back(static_cast<int32_t>(r));
+    else if (m <= 256 && na(r) <= 256) num_m.push_back(static_cast<int32_t>(r));
+    else if (m <= 1024 && na(r) <= 1024) num_l.push_back(static_cast<int32_t>(r));
+    else {
+      tsize[r] = static_cast<int32_t>(pow2_at_least(2 * m));
+      any_global = true;
+    }
+  }
+  auto numeric_lds = [&](const std::vector<int32_t> &list, auto kernel) {
+    launch_rows(list, [&](uint32_t g, const int32_t *rows) {
+      kernel<<<g, kRowWave>>>(A.p(), A.c(), A.v(), B.p(), B.c(), B.v(), rows,
+                              rowlen_d.as<int32_t>(), C.ptr.as<int32_t>(), C.col.as<int32_t>(),
+                              C.val.as<double>());
+    });
+  };
+  numeric_lds(num_s, k_numeric_lds<256, 128, 256>);
+  numeric_lds(num_m, k_numeric_lds<512, 256, 256>);
+  numeric_lds(num_l, k_numeric_lds<2048, 1024, 1024>);
+  if (any_global) {
+    if (!scratch.get()) scratch.alloc(kScratchBytes);
+    tsize_d.upload(tsize);
+    for (const auto &ch : chunks(tsize, toff, 2 * sizeof(int32_t))) {
+      toff_d.upload(toff);
+      const int64_t m = ch.second - ch.first;
+      k_numeric<<<blocks_for(m), kThreads>>>(A.p(), A.c(), A.v(), B.p(), B.c(), B.v(), ch.first,
+                                            ch.second, toff_d.as<int64_t>(),
+                                            tsize_d.as<int32_t>(), scratch.as<int32_t>(),
+                                            C.ptr.as<int32_t>(), C.col.as<int32_t>(),
+                                            C.val.as<double>());
+      GS_HIP(hipGetLastError());
+    }
   }
   GS_HIP(hipDeviceSynchronize());
   return C;
@@ -335,21 +626,22 @@ DCsr transpose(const DCsr &A) {
 
 void DeviceGalerkinOps::run(const gsx::Csr &A, const gsx::Csr &M, const gsx::Csr &Pt,
                             gsx::Csr &P, gsx::Csr &R, gsx::Csr &Ac) const {
-  DevBuf scratch(kScratchBytes);
+  DevBuf scratch;
   const DCsr dM = upload(M), dPt = upload(Pt);
-  const DCsr dP = spgemm(dM, dPt, scratch);
+  // The products' own rows are sorted and unique.
+  const DCsr dP = spgemm(dM, dPt, scratch, gsx::rows_sorted_unique(Pt));
   const DCsr dR = transpose(dP);
   const DCsr dA = upload(A);
-  const DCsr dAP = spgemm(dA, dP, scratch);
-  const DCsr dAc = spgemm(dR, dAP, scratch);
+  const DCsr dAP = spgemm(dA, dP, scratch, true);
+  const DCsr dAc = spgemm(dR, dAP, scratch, true);
   P = download(dP);
   R = download(dR);
   Ac = download(dAc);
 }
 
 gsx::Csr device_spgemm(const gsx::Csr &A, const gsx::Csr &B) {
-  DevBuf scratch(kScratchBytes);
-  return download(spgemm(upload(A), upload(B), scratch));
+  DevBuf scratch;
+  return download(spgemm(upload(A), upload(B), scratch, gsx::rows_sorted_unique(B)));
 }
 
 gsx::Csr device_transpose(const gsx::Csr &A) { return download(transpose(upload(A))); }

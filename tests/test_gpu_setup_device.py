@@ -35,6 +35,10 @@ CASES = [
     ("rectangular", (500, 120, 0.03), (120, 80, 0.05)),
     ("empty rows", (200, 150, 0.02), (150, 200, 0.03)),
     ("long rows", (40, 2000, 0.5), (2000, 3000, 0.02)),  # ~20 000 products per row
+    # Distinct columns per output row in (128, 256] and (256, 1024]: the
+    # wavefront-per-row kernels' middle and large LDS tables.
+    ("mid rows", (120, 400, 0.075), (400, 300, 0.03)),
+    ("wide rows", (120, 400, 0.075), (400, 800, 0.03)),
 ]
 
 
@@ -57,6 +61,29 @@ def test_device_spgemm_signed_zero_first_product():
     C = gs.spgemm(A, B, device=True)
     same(C, gs.spgemm(A, B))
     assert np.signbit(C.val[0])  # (-1 * 0) + (2 * -0) = -0 + -0 = -0
+
+
+def test_device_spgemm_row_classes_are_covered():
+    """The CASES above reach every path of the device product: rows of at
+    most 128, 256 and 1024 distinct columns (the LDS kernels) and longer
+    ones (the global-table kernels, after the LDS count gives up)."""
+    seen = set()
+    for name, sa, sb in CASES:
+        A = rand_csr(*sa, seed=1, empty_rows=(0, 5, 17) if name == "empty rows" else ())
+        B = rand_csr(*sb, seed=2, empty_rows=(3, 4) if name == "empty rows" else ())
+        C = gs.spgemm(A, B)
+        for m in np.diff(C.ptr):
+            seen.add(0 if m == 0 else 1 if m <= 128 else 2 if m <= 256 else 3 if m <= 1024 else 4)
+    assert seen == {0, 1, 2, 3, 4}
+
+
+def test_device_spgemm_repeated_columns_in_b():
+    """A B row that names a column twice (the products are added in their
+    stored order, as on the host): the device takes the sequential path."""
+    A = gs.Csr.from_scipy(sp.csr_matrix(np.array([[1.5, -2.0, 0.25], [0.0, 3.0, 1.0]])))
+    B = gs.Csr(np.array([0, 3, 5, 6], np.int32), np.array([1, 0, 1, 2, 2, 0], np.int32),
+               np.array([1e16, 1.0, -1e16, 3.0, 5.0, 7.0]), (3, 3))
+    same(gs.spgemm(A, B, device=True), gs.spgemm(A, B))
 
 
 def _hier_same(Hd, Hh):
