@@ -141,7 +141,8 @@ __global__ __launch_bounds__(kThreads) void k_row_panel(const double *__restrict
 // bound by the tile's own read and write (measured per update at n = 13 047:
 // FP64 FMAs from LDS 1.44 ms, MFMA from LDS 1.51 ms, with the tile loaded
 // first 1.23 ms, operands from the panels 0.975 ms; a fully unrolled or
-// 8-step unrolled panel loop 1.18 / 1.15 ms).
+// 8-step unrolled panel loop 1.18 / 1.15 ms; each lane's A operands as one
+// contiguous 16-byte-vector run, with the panel columns reordered, 1.24 ms).
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(kThreads) void k_rank_update(double *__restrict__ A, int64_t N,
