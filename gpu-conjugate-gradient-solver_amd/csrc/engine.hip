@@ -132,6 +132,18 @@ bool is_identity(const std::vector<int32_t> &o2n) {
   return true;
 }
 
+// permute_sorted(A, rows, cols), or A itself when both permutations are
+// the identity and A's rows are already sorted and unique (the permuted
+// copy would equal A).
+const gsx::Csr &permuted(const gsx::Csr &A, const gsx::Perm &rows, const gsx::Perm &cols,
+                         gsx::Csr &store) {
+  if (is_identity(rows.old_to_new) && is_identity(cols.old_to_new) &&
+      gsx::rows_sorted_unique(A))
+    return A;
+  store = gsx::permute_sorted(A, rows.old_to_new.data(), cols.old_to_new.data());
+  return store;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -584,8 +596,8 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   const int used_levels = opt_.precond == kPrecMultigrid ? nlev_ : 1;
   mark("validate");
   for (int k = 0; k < used_levels; ++k) {
-    const gsx::Csr Ak = gsx::permute_sorted(H.A[k], perms[k].old_to_new.data(),
-                                            perms[k].old_to_new.data());
+    gsx::Csr store;
+    const gsx::Csr &Ak = permuted(H.A[k], perms[k], perms[k], store);
     mark("level " + std::to_string(k) + " permute");
     if (opt_.precond == kPrecMultigrid) build_level(k, Ak, k == nlev_ - 1);
     else {
@@ -595,10 +607,9 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   }
   if (opt_.precond == kPrecMultigrid) {
     for (int k = 0; k + 1 < nlev_; ++k) {
-      const auto &o2n_f = perms[k].old_to_new;
-      const auto &o2n_c = perms[k + 1].old_to_new;
-      gsx::Csr Pk = gsx::permute_sorted(H.P[k], o2n_f.data(), o2n_c.data());
-      gsx::Csr Rk = gsx::permute_sorted(H.R[k], o2n_c.data(), o2n_f.data());
+      gsx::Csr p_store, r_store;
+      const gsx::Csr &Pk = permuted(H.P[k], perms[k], perms[k + 1], p_store);
+      const gsx::Csr &Rk = permuted(H.R[k], perms[k + 1], perms[k], r_store);
       if (Pk.rows != L_[k].n || Pk.cols != L_[k + 1].n || Rk.rows != L_[k + 1].n ||
           Rk.cols != L_[k].n)
         throw std::runtime_error("MultigridSmoother: transfer operator shape mismatch.");

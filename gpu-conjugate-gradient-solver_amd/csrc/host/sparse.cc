@@ -25,9 +25,11 @@ void Csr::check() const {
   for (int64_t r = 0; r < rows; ++r)
     if (ptr[r + 1] < ptr[r])
       throw std::runtime_error("HostCSRMatrix: invalid CSR row pointer.");
-  for (int32_t c : col)
-    if (c < 0 || c >= cols)
-      throw std::runtime_error("HostCSRMatrix: column index out of bounds.");
+  bool ok = true;
+  const int64_t nz = nnz();
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+  for (int64_t k = 0; k < nz; ++k) ok = ok && col[k] >= 0 && col[k] < cols;
+  if (!ok) throw std::runtime_error("HostCSRMatrix: column index out of bounds.");
 }
 
 Perm Perm::identity(int64_t n) {

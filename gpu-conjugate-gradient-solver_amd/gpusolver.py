@@ -681,7 +681,7 @@ class Solver:
                     keep.append(a)
                     arr[i] = a.ctypes.data_as(_vp)
         check(lib.gs_solver_setup(self.h, hier.h, C.cast(arr, _vp) if arr is not None else None))
-        self.n = hier.get("A", 0).rows
+        self.n = hier.level_rows(0)  # (get() would copy the whole level-0 matrix)
 
     def solve(self, b: np.ndarray, x0: np.ndarray | None = None, tol: float = 1e-8,
               maxit: int = 1000):
