@@ -570,6 +570,7 @@ int gs_solver_set_int(gs_solver_h s, const char *key, int64_t v) {
     else if (k == "vcycles") o.vcycles = static_cast<int>(v);
     else if (k == "coarse_direct") o.coarse_direct = v != 0;
     else if (k == "graph_iters") o.graph_iters = static_cast<int>(v);
+    else if (k == "graph_tail") o.graph_tail = v != 0;
     else if (k == "cg_block3") o.cg_block3 = v != 0;
     else if (k == "reference_vcycle") o.reference_vcycle = v != 0;
     else if (k == "max_direct") o.max_direct = v;
@@ -762,6 +763,7 @@ int gs_solver_result(gs_solver_h s, const char *key, double *value) {
     const gse::SolveResult &r = s->last;
     if (k == "converged") *value = r.converged ? 1.0 : 0.0;
     else if (k == "loop_iterations") *value = static_cast<double>(r.loop_iterations);
+    else if (k == "launched_iterations") *value = static_cast<double>(r.launched_iterations);
     else if (k == "breakdown") *value = r.breakdown;
     else if (k == "wall_ms") *value = r.wall_ms;
     else if (k == "zero_rhs") *value = r.zero_rhs ? 1.0 : 0.0;

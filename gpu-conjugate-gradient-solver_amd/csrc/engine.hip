@@ -194,8 +194,7 @@ class Engine final : public EngineBase {
     GS_HIP(hipStreamCreate(&stream_));
   }
   ~Engine() override {
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    drop_graphs();
     if (host_ctl_) (void)hipHostFree(host_ctl_);
     if (stage_) (void)hipHostFree(stage_);
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -229,8 +228,7 @@ class Engine final : public EngineBase {
           !lv.direct && k == 0 && k + 1 < nlev_)
         throw std::runtime_error("Preconditioner requires at least one smoothing step.");
     }
-    if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
-    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    drop_graphs();
     for (int k = 0; k < nl; ++k) {
       Level<TM> &lv = L_[k];
       if (k < static_cast<int>(omega.size()) && omega[k] > 0.0)
@@ -276,6 +274,16 @@ class Engine final : public EngineBase {
   TM *rvc_ = nullptr;  // V-cycle view of r (r itself or its rounded copy)
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
+  // One PCG iteration (with the replacement pair when the mode has one),
+  // replayed for the last iterations of a solve (Options::graph_tail).
+  hipGraph_t graph1_ = nullptr;
+  hipGraphExec_t graph1_exec_ = nullptr;
+  void drop_graphs() {
+    if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
+    if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+    if (graph1_exec_) { (void)hipGraphExecDestroy(graph1_exec_); graph1_exec_ = nullptr; }
+    if (graph1_) { (void)hipGraphDestroy(graph1_); graph1_ = nullptr; }
+  }
   gsk::Ctl<TC> *host_ctl_ = nullptr;
   TC *stage_ = nullptr;  // pinned staging for b and x (2 n values)
   int64_t stage_n_ = 0;
@@ -765,8 +773,7 @@ void Engine<TC, TM>::setup(const gsx::Hierarchy &H, const std::vector<gsx::Perm>
   }
   if (!ev0_) GS_HIP(hipEventCreate(&ev0_));
   if (!ev1_) GS_HIP(hipEventCreate(&ev1_));
-  if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
-  if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
+  drop_graphs();
   mark("vectors");
   setup_done_ = true;
 }
@@ -1168,6 +1175,22 @@ void Engine<TC, TM>::capture_graph() {
   GS_HIP(hipGraphGetNodes(graph_, nullptr, &nodes));
   graph_nodes_per_iter_ = static_cast<double>(nodes) / opt_.graph_iters;
   GS_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  const int gi = opt_.graph_iters;
+  if (opt_.graph_tail && gi > 1 && (every == 0 || every < gi || every % gi == 0)) {
+    // The single iteration is the graph's last, so it carries the pair.
+    // With a period below graph_iters the chunk carries it in every
+    // iteration as well. With a period that is a multiple of graph_iters,
+    // no replacement falls due in a tail (a tail never reaches the next
+    // multiple of graph_iters), and a pending convergence check is resolved
+    // in the iteration that set it; the chunk resolves it in its last
+    // iteration, after idle ones, from the same state. Either way the
+    // results are the chunk graph's bit for bit. (Other periods keep whole
+    // chunks: the chunk would defer a due replacement to its end.)
+    GS_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    encode_iteration(stop, every > 0);
+    GS_HIP(hipStreamEndCapture(stream_, &graph1_));
+    GS_HIP(hipGraphInstantiate(&graph1_exec_, graph1_, nullptr, nullptr, 0));
+  }
 }
 
 template <class TC, class TM>
@@ -1222,6 +1245,8 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   const int32_t *stop = &ctl->stop;
   int64_t launches = 0;  // each runs at least one iteration while stop == 0
   double log_rel0 = 0.0;
+  double tail_rel = 0.0;  // relres^2 and iteration at the previous host check
+  int64_t tail_it = -1;
   if (maxit > 0) {
     if (opt_.graph_iters > 0) capture_graph();
     for (;;) {
@@ -1246,8 +1271,35 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
           break;
         }
       }
-      if (graph_exec_) GS_HIP(hipGraphLaunch(graph_exec_, stream_));
-      else encode_iteration(stop);
+      // Iterations in this launch: a whole chunk, or, when the rate of the
+      // last chunk says fewer remain to the tolerance, that many single
+      // iterations (a short estimate costs one more host check; a long one
+      // runs no-op iterations, as a whole chunk would).
+      int64_t k = opt_.graph_iters;
+      {
+        const double rel = static_cast<double>(host_ctl_->rel);
+        const int64_t it = host_ctl_->iter;
+        if (graph1_exec_ && tail_it >= 0 && it > tail_it && rel > 0.0 && tail_rel > 0.0 &&
+            host_ctl_->tol2 > 0) {
+          const double rate = (std::log(rel) - std::log(tail_rel)) / static_cast<double>(it - tail_it);
+          const double need = std::log(static_cast<double>(host_ctl_->tol2)) - std::log(rel);
+          if (rate < 0.0 && need < 0.0) {
+            const double est = std::ceil(need / rate);
+            if (est < static_cast<double>(k)) k = std::max<int64_t>(1, static_cast<int64_t>(est));
+          }
+        }
+        tail_rel = rel;
+        tail_it = it;
+      }
+      if (graph_exec_ && k >= opt_.graph_iters) {
+        GS_HIP(hipGraphLaunch(graph_exec_, stream_));
+      } else if (graph_exec_) {
+        for (int64_t i = 0; i < k; ++i) GS_HIP(hipGraphLaunch(graph1_exec_, stream_));
+      } else {
+        encode_iteration(stop);
+        k = 1;
+      }
+      res.launched_iterations += k;
       ++launches;
     }
   } else {

@@ -26,6 +26,10 @@ struct Options {
   bool coarse_direct = false;
   int precond = kPrecMultigrid;
   int graph_iters = 8;
+  // Near convergence, replay a one-iteration graph as many times as the
+  // last chunk's rate says are left, instead of a whole chunk whose
+  // iterations past convergence are no-op launches.
+  bool graph_tail = true;
   bool cg_block3 = false;  // store the CG operator as 3x3 blocks when possible
   // Reproduce the reference's coarse-level correction aliasing
   // (solution = 2 P e below the finest level); off: x += P e.
@@ -122,6 +126,7 @@ struct LagGuard {
 struct SolveResult {
   int64_t iterations = 0;       // reference convention (src/CGSolver.cc:257-268)
   int64_t loop_iterations = 0;  // PCG updates actually applied
+  int64_t launched_iterations = 0;  // iterations launched (graph iterations past convergence are no-op launches)
   double relres2 = 0.0;         // final recursive ||r||^2 / ||b||^2
   double true_relres = 0.0;     // ||b - A x|| / ||b|| recomputed in FP64
   double solve_ms = 0.0;        // HIP-event time of the solve on the device

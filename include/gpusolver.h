@@ -223,6 +223,12 @@ int gs_solver_create(int32_t precision, gs_solver_h *out);
  *                     inverse built on the device, applied as a product
  *                     over its lower 64 x 64 tiles)
  *   "graph_iters"     PCG iterations per captured hipGraph (0: no graph)
+ *   "graph_tail"      1 (default): when the last chunk's convergence rate says
+ *                     fewer than graph_iters iterations remain, replay a
+ *                     one-iteration graph that many times instead of a
+ *                     whole chunk (same results; fewer no-op launches).
+ *                     Off for a true-residual period above graph_iters that
+ *                     is not a multiple of it.
  *   "cg_block3"       1: store the CG operator as 3x3 blocks
  *   "reference_vcycle" 1: reproduce the reference's coarse-level correction
  *                     aliasing (src/Smoothers/MultigridSmoother.cc:401-404:
@@ -331,6 +337,8 @@ int gs_solver_solve(gs_solver_h s, int64_t b_len, const double *b, int64_t x_len
  * include/GPUSolver/CGSolver.h:19-86). */
 int gs_solver_size(gs_solver_h s, int64_t *n);
 /* Result fields of the last solve: "converged", "loop_iterations",
+ * "launched_iterations" (iterations launched, no-op ones past convergence
+ * included),
  * "breakdown", "wall_ms", "zero_rhs", "fell_back", "lagging" (ended by the
  * fallback's rate check), "fallback_reason" (0 none, 1 lagging the plain
  * rate, 2 CG breakdown, 3 non-finite residual, 4 iteration budget spent),
