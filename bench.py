@@ -285,6 +285,7 @@ def run_modes(config, H, b, sk, steps, warmup, graph_iters, names):
         d = s.describe()
         m = dict(ms=host_ms, device_ms=float(np.mean([i.solve_ms for i in infos])),
                  iterations=infos[-1].iterations, true_relres=infos[-1].true_relres,
+                 launched_iterations=int(s.result("launched_iterations")),
                  setup_ms=setup_ms, launches_per_iteration=d["launches_per_iteration"],
                  levels_used=len(d["levels"]))
         if lw is not None:

@@ -208,18 +208,49 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 // X: the type of x and of the products and sum (the stored values are
 // widened to it; X = T for every operator but the CG one read in the V-cycle
 // type).
+// The slots of one SELL row (first slot at `base`, slice width w) in
+// batches of kSellBatch: the batch's column indices and values are loaded
+// together, then its x gathers, then the products are added in slot order
+// -- two dependent memory round trips per batch instead of two per slot,
+// with the arithmetic (order included) of a plain slot loop. Slots past
+// the width reload the last slot (an in-bounds address) and add nothing.
+// SKIP: the entry in column `row` contributes -0.0 instead of its product
+// (s + -0.0 == s for every s, signed zeros included), so the sum is the
+// reference's loop that skips it (src/CPU/CPUBackend.cc:62-86) without a
+// branch around the loads. NT: non-temporal matrix loads.
+constexpr int kSellBatch = 8;
+template <class T, class X, bool NT, bool SKIP>
+__device__ __forceinline__ X sell_slots(const Sell<T> &A, int64_t base, int w, const X *x,
+                                        int64_t row) {
+  X sum = X(0);
+  for (int j0 = 0; j0 < w; j0 += kSellBatch) {
+    int32_t c[kSellBatch];
+    T v[kSellBatch];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u) {
+      const int j = j0 + u < w ? j0 + u : w - 1;
+      const int64_t e = base + static_cast<int64_t>(j) * kSlice;
+      c[u] = NT ? ld_stream(A.col + e) : A.col[e];
+      v[u] = NT ? ld_stream(A.val + e) : A.val[e];
+    }
+    X xv[kSellBatch];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < kSellBatch; ++u) {
+      if (j0 + u < w) {
+        const X prod = static_cast<X>(v[u]) * xv[u];
+        sum += (SKIP && c[u] == row) ? X(-0.0) : prod;
+      }
+    }
+  }
+  return sum;
+}
+
 template <class T, class X = T>
 __device__ __forceinline__ X sell_row_dot(const Sell<T> &A, int64_t row, const X *x) {
   const int64_t s = row >> 6;
-  const int64_t base = A.slice_ptr[s] + (row & 63);
-  const int w = A.slice_w[s];
-  X sum = X(0);
-#pragma unroll 4
-  for (int j = 0; j < w; ++j) {
-    const int64_t e = base + static_cast<int64_t>(j) * kSlice;
-    sum += static_cast<X>(ld_stream(A.val + e)) * x[ld_stream(A.col + e)];
-  }
-  return sum;
+  return sell_slots<T, X, true, false>(A, A.slice_ptr[s] + (row & 63), A.slice_w[s], x, row);
 }
 
 // y(3) = A(3 rows of block row b) x, scalar-row order kept: blocks in slot
@@ -304,27 +335,14 @@ __device__ __forceinline__ SliceHead bsr3_head(const Bsr3<T> &A, int64_t b) {
 }
 template <class T>
 __device__ __forceinline__ T sell_dot_h(const Sell<T> &A, SliceHead h, const T *x) {
-  T sum = T(0);
-#pragma unroll 4
-  for (int j = 0; j < h.w; ++j) {
-    const int64_t e = h.base + static_cast<int64_t>(j) * kSlice;
-    sum += A.val[e] * x[A.col[e]];
-  }
-  return sum;
+  return sell_slots<T, T, false, false>(A, h.base, h.w, x, -1);
 }
 // Off-diagonal row sum (rowDotExcluding with exclusion [row, row + 1),
 // src/CPU/CPUBackend.cc:62-86).
 template <class T>
 __device__ __forceinline__ T sell_offdiag_h(const Sell<T> &A, SliceHead h, int64_t row,
                                             const T *x) {
-  T sum = T(0);
-#pragma unroll 4
-  for (int j = 0; j < h.w; ++j) {
-    const int64_t e = h.base + static_cast<int64_t>(j) * kSlice;
-    const int32_t c = A.col[e];
-    if (c != row) sum += A.val[e] * x[c];
-  }
-  return sum;
+  return sell_slots<T, T, false, true>(A, h.base, h.w, x, row);
 }
 // Row sums of block row b, slots in order. The slots are processed in
 // batches (8 in FP32, 4 in FP64): the block indices of a batch are loaded
