@@ -278,7 +278,13 @@ class Engine final : public EngineBase {
   // replayed for the last iterations of a solve (Options::graph_tail).
   hipGraph_t graph1_ = nullptr;
   hipGraphExec_t graph1_exec_ = nullptr;
+  // Loop iterations of the last converged solve (0: none since the last
+  // setup or weight change): the first launch of the next solve runs at
+  // most that many iterations, so a repeated solve shorter than a chunk
+  // launches no idle iterations either.
+  int64_t last_loop_iters_ = 0;
   void drop_graphs() {
+    last_loop_iters_ = 0;
     if (graph_exec_) { (void)hipGraphExecDestroy(graph_exec_); graph_exec_ = nullptr; }
     if (graph_) { (void)hipGraphDestroy(graph_); graph_ = nullptr; }
     if (graph1_exec_) { (void)hipGraphExecDestroy(graph1_exec_); graph1_exec_ = nullptr; }
@@ -1279,8 +1285,10 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
       {
         const double rel = static_cast<double>(host_ctl_->rel);
         const int64_t it = host_ctl_->iter;
-        if (graph1_exec_ && tail_it >= 0 && it > tail_it && rel > 0.0 && tail_rel > 0.0 &&
-            host_ctl_->tol2 > 0) {
+        if (graph1_exec_ && tail_it < 0 && last_loop_iters_ > 0 && last_loop_iters_ < k) {
+          k = last_loop_iters_;
+        } else if (graph1_exec_ && tail_it >= 0 && it > tail_it && rel > 0.0 && tail_rel > 0.0 &&
+                   host_ctl_->tol2 > 0) {
           const double rate = (std::log(rel) - std::log(tail_rel)) / static_cast<double>(it - tail_it);
           const double need = std::log(static_cast<double>(host_ctl_->tol2)) - std::log(rel);
           if (rate < 0.0 && need < 0.0) {
@@ -1323,6 +1331,7 @@ SolveResult Engine<TC, TM>::solve(const double *b, double *x, double tol, int64_
   res.loop_iterations = hc.iter;
   res.relres2 = static_cast<double>(hc.rel);
   res.converged = hc.stop == 1;
+  if (graph1_exec_) last_loop_iters_ = res.converged ? hc.iter : 0;
   res.breakdown = hc.breakdown;
   // History.
   const int64_t hl = std::min<int64_t>(hc.iter + 1, maxit + 1);

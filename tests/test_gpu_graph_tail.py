@@ -117,3 +117,30 @@ def test_failed_check_resumes_inside_the_tail(elast):
         assert not o[1].converged and o[1].loop_iterations == maxit
     same(out[0], out[1])
     same(out[0], out[2])
+
+
+def test_repeated_short_solve_launches_only_its_iterations():
+    """A solve shorter than a chunk (the 2D Poisson problem of 65 536
+    unknowns with the direct coarse level: 6 iterations) launches a whole
+    chunk the first time; the next solve starts from the last solve's loop
+    count and launches only the iterations that run, with the same bits."""
+    A = gs.poisson2d(256, 256)
+    H = gs.Hierarchy.smoothed_aggregation(A, block_size=1, eps_strong=0.1, max_levels=10,
+                                          min_dofs=1000)
+    b = gs.uniform_pm1(1, A.rows)
+    s = gs.Solver(gs.PREC_MIXED, smoother=gs.SGS, coarse_steps=2, coarse_direct=1,
+                  direct_truncate=1, graph_iters=8)
+    s.setup(H)
+    x1, i1 = s.solve(b, tol=1e-8)
+    l1 = int(s.result("launched_iterations"))
+    x2, i2 = s.solve(b, tol=1e-8)
+    l2 = int(s.result("launched_iterations"))
+    assert i1.converged and i2.converged
+    assert i1.loop_iterations < 8 and l1 == 8
+    assert l2 == i2.loop_iterations == i1.loop_iterations
+    np.testing.assert_array_equal(x1, x2)
+    assert i1.true_relres == i2.true_relres
+    # New weights drop the graphs and the remembered count.
+    s.set_level_weights(omega=[1.0, 1.0], sweeps=[1, 1])
+    s.solve(b, tol=1e-8)
+    assert int(s.result("launched_iterations")) == 8
